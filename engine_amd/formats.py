@@ -1,0 +1,112 @@
+"""Run-time format descriptors mirroring the reference's format templates.
+
+The reference names a format by template arguments, e.g.
+``dynamic_vbp_f<512, 2048, 8>`` (include/core/morphing/dynamic_vbp.h:83) or
+``delta_f<1024, 8, dynamic_vbp_f<512, 64, 8>>`` (include/core/morphing/delta.h:78).
+Here the same arguments fill an ``ms_format_desc`` (include/morphstore_hip.h).
+"""
+import ctypes
+
+MS_UNCOMPR, MS_STATIC_VBP, MS_DYNAMIC_VBP, MS_DELTA, MS_FOR, MS_RLE = range(6)
+KIND_NAMES = {MS_UNCOMPR: "uncompr_f", MS_STATIC_VBP: "static_vbp_f", MS_DYNAMIC_VBP: "dynamic_vbp_f",
+              MS_DELTA: "delta_f", MS_FOR: "for_f", MS_RLE: "rle_f"}
+
+MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE = range(6)
+(MS_OP_ADD, MS_OP_SUB, MS_OP_MUL, MS_OP_DIV, MS_OP_MOD, MS_OP_MIN, MS_OP_MAX,
+ MS_OP_AND, MS_OP_OR, MS_OP_XOR) = range(10)
+
+MS_OK, MS_ERR_INVALID, MS_ERR_NOMEM, MS_ERR_HIP, MS_ERR_SIZE, MS_ERR_CAPACITY, MS_ERR_NODEVICE, MS_ERR_RANGE = range(8)
+
+
+class FormatDesc(ctypes.Structure):
+    """ms_format_desc."""
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("kind", "bw", "block", "page", "step",
+                 "inner_kind", "inner_bw", "inner_block", "inner_page", "inner_step")]
+
+    def name(self):
+        k = self.kind
+        if k == MS_UNCOMPR:
+            return "uncompr_f"
+        if k == MS_STATIC_VBP:
+            return f"static_vbp_f<vbp_l<{self.bw},{self.step}>>"
+        if k == MS_DYNAMIC_VBP:
+            return f"dynamic_vbp_f<{self.block},{self.page},{self.step}>"
+        if k == MS_RLE:
+            return "rle_f"
+        inner = FormatDesc(self.inner_kind, self.inner_bw, self.inner_block, self.inner_page,
+                           self.inner_step).name()
+        if k == MS_DELTA:
+            return f"delta_f<{self.block},{self.step},{inner}>"
+        return f"for_f<{self.block},{self.page},{inner}>"
+
+    __repr__ = name
+
+    def block_size(self):
+        """F::m_BlockSize (None for rle_f, which has none)."""
+        if self.kind == MS_UNCOMPR:
+            return 1
+        if self.kind == MS_STATIC_VBP:
+            return 64 * self.step
+        if self.kind == MS_RLE:
+            return None
+        return self.block
+
+
+class ColMeta(ctypes.Structure):
+    """ms_col_meta (column_meta_data, include/core/storage/column_helper.h:32)."""
+    _fields_ = [("count", ctypes.c_uint64), ("size_used", ctypes.c_uint64),
+                ("size_compr", ctypes.c_uint64), ("size_alloc", ctypes.c_uint64)]
+
+
+def uncompr_f():
+    return FormatDesc(MS_UNCOMPR)
+
+
+def static_vbp_f(bw, step):
+    """static_vbp_f<vbp_l<bw, step>>."""
+    return FormatDesc(MS_STATIC_VBP, bw, 0, 0, step)
+
+
+def dynamic_vbp_f(block, page, step):
+    """dynamic_vbp_f<block, page, step>."""
+    return FormatDesc(MS_DYNAMIC_VBP, 0, block, page, step)
+
+
+def _with_inner(f, inner):
+    f.inner_kind, f.inner_bw, f.inner_block = inner.kind, inner.bw, inner.block
+    f.inner_page, f.inner_step = inner.page, inner.step
+    return f
+
+
+def delta_f(block, step, inner):
+    """delta_f<block, step, inner>."""
+    return _with_inner(FormatDesc(MS_DELTA, 0, block, 0, step), inner)
+
+
+def for_f(block, page, inner):
+    """for_f<block, page, inner>; the lane count is the inner format's step."""
+    lanes = inner.step if inner.kind != MS_UNCOMPR else 1
+    return _with_inner(FormatDesc(MS_FOR, 0, block, page, lanes), inner)
+
+
+def rle_f():
+    return FormatDesc(MS_RLE)
+
+
+# DEFAULT_*_F(ve) for a vector extension of `lanes` 64-bit elements
+# (include/core/morphing/default_formats.h:35-82).
+def default_static_vbp_f(bw, lanes=8):
+    return static_vbp_f(bw, lanes)
+
+
+def default_dynamic_vbp_f(lanes=8):
+    return dynamic_vbp_f(64 * lanes, 8 * lanes, lanes)
+
+
+def default_delta_dynamic_vbp_f(lanes=8):
+    return delta_f(1024, lanes, default_dynamic_vbp_f(lanes))
+
+
+def default_for_dynamic_vbp_f(lanes=8):
+    return for_f(1024, lanes, default_dynamic_vbp_f(lanes))

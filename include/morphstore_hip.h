@@ -1,0 +1,246 @@
+/*
+ * morphstore_hip.h -- C-ABI of the MI355X (gfx950) implementation of
+ * MorphStore's compression-aware execution path.
+ *
+ * Every entry point takes plain pointers and sizes, returns an int status
+ * (MS_OK on success) and never throws.  Device columns are opaque handles
+ * (ms_col) that own a device buffer holding the column's byte image, which
+ * is identical to the image the reference keeps in host memory, plus a
+ * derived block index that is not part of the image.  Column meta data
+ * (count, sizeUsed, sizeCompr) lives in device memory, so operators chain
+ * without host synchronisation and a chain of operators can be captured in
+ * a hipGraph.  Calls are stream ordered: `stream` is a hipStream_t (NULL =
+ * the default stream).  Output columns are allocated by the library and
+ * released by the caller with ms_col_free.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   column<F>                      include/core/storage/column.h:45-287
+ *   morph<ve,dst,src>              include/core/morphing/morph.h:109-116
+ *   my_select_wit_t::apply         include/core/operators/general_vectorized/select_compr.h:108-259
+ *   my_project_wit_t::apply        include/core/operators/general_vectorized/project_compr.h:80-261
+ *   agg_sum<ve,F>                  include/core/operators/interfaces/agg_sum.h:56-65
+ *   intersect_sorted<ve,o,l,r>     include/core/operators/interfaces/intersect.h:52-63
+ *   merge_sorted<ve,o,l,r>         include/core/operators/interfaces/merge.h:67-82
+ *   calc_binary<op,ve,o,l,r>       include/core/operators/interfaces/calc.h:102-119
+ *   exhaustive size selector       include/core/format_selection/exhaustive_size_format_selector.h:35-141
+ */
+#ifndef MORPHSTORE_HIP_H
+#define MORPHSTORE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+enum ms_status {
+    MS_OK = 0,
+    MS_ERR_INVALID = 1,   /* invalid argument or unsupported format        */
+    MS_ERR_NOMEM = 2,     /* device or host allocation failed              */
+    MS_ERR_HIP = 3,       /* HIP runtime error                             */
+    MS_ERR_SIZE = 4,      /* operand counts differ (calc_binary)           */
+    MS_ERR_CAPACITY = 5,  /* an input exceeds a buffer or index capacity   */
+    MS_ERR_NODEVICE = 6,  /* no usable gfx950 device                       */
+    MS_ERR_RANGE = 7      /* a position or value is outside its domain     */
+};
+
+/* ------------------------------------------------------------------ */
+/* Formats                                                             */
+/* ------------------------------------------------------------------ */
+enum ms_format_kind {
+    MS_UNCOMPR = 0,      /* uncompr_f                       format.h:90-96        */
+    MS_STATIC_VBP = 1,   /* static_vbp_f<vbp_l<bw,step>>    static_vbp.h:60-68    */
+    MS_DYNAMIC_VBP = 2,  /* dynamic_vbp_f<block,page,step>  dynamic_vbp.h:83-159  */
+    MS_DELTA = 3,        /* delta_f<block,step,inner>       delta.h:78-157        */
+    MS_FOR = 4,          /* for_f<block,page,inner>         for.h:84-164          */
+    MS_RLE = 5           /* rle_f                           rle.h:42-49           */
+};
+
+/* Meta byte of a non-existent block in the last page of a dynamic_vbp
+ * morph_batch call (vbp_commons.h:54). */
+#define MS_VBP_BW_NOBLOCK 0xffu
+
+/*
+ * A format descriptor is the run-time form of the reference's format
+ * template arguments.
+ *   static_vbp_f<vbp_l<bw,step>>     : bw, step
+ *   dynamic_vbp_f<block,page,step>   : block, page, step.  `step` is the
+ *       number of interleaved lanes the payload is packed with.  The
+ *       reference ignores t_Step and packs with the vector extension's
+ *       element count (dynamic_vbp.h:220); both agree when the format is
+ *       used with its matching extension, as all DEFAULT_* formats are.
+ *   delta_f<block,step,inner>        : block, step, inner_*
+ *   for_f<block,page,inner>          : block, page, inner_*
+ *   The inner format of a cascade is described by inner_kind (MS_UNCOMPR,
+ *   MS_STATIC_VBP or MS_DYNAMIC_VBP) and inner_bw/inner_block/inner_page/
+ *   inner_step with the meaning above.
+ */
+typedef struct ms_format_desc {
+    uint32_t kind;
+    uint32_t bw;
+    uint32_t block;
+    uint32_t page;
+    uint32_t step;
+    uint32_t inner_kind;
+    uint32_t inner_bw;
+    uint32_t inner_block;
+    uint32_t inner_page;
+    uint32_t inner_step;
+} ms_format_desc;
+
+/* column_meta_data (column_helper.h:32) */
+typedef struct ms_col_meta {
+    uint64_t count;       /* m_CountLogicalValues */
+    uint64_t size_used;   /* m_SizeUsedByte       */
+    uint64_t size_compr;  /* m_SizeComprByte      */
+    uint64_t size_alloc;  /* m_SizeAllocByte      */
+} ms_col_meta;
+
+typedef struct ms_col ms_col;
+
+/* Comparison operators of select (std::equal_to, std::less, ... as used by
+ * select_t / my_select_wit_t; compares are unsigned 64-bit). */
+enum ms_cmp {
+    MS_CMP_EQ = 0, MS_CMP_NE = 1, MS_CMP_LT = 2,
+    MS_CMP_LE = 3, MS_CMP_GT = 4, MS_CMP_GE = 5
+};
+
+/* Binary operators of calc_binary (calc.h:102-119), arithmetic mod 2^64.
+ * DIV and MOD by zero yield 0 (the reference leaves them undefined). */
+enum ms_calc_op {
+    MS_OP_ADD = 0, MS_OP_SUB = 1, MS_OP_MUL = 2, MS_OP_DIV = 3,
+    MS_OP_MOD = 4, MS_OP_MIN = 5, MS_OP_MAX = 6,
+    MS_OP_AND = 7, MS_OP_OR = 8, MS_OP_XOR = 9
+};
+
+/* ------------------------------------------------------------------ */
+/* Library and column management                                       */
+/* ------------------------------------------------------------------ */
+
+/* Selects the HIP device, checks it is gfx950 and sets up the stream-ordered
+ * allocation pool.  Must be called once per process before anything else. */
+int ms_init(int device);
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *ms_last_error(void);
+
+/* Library version string. */
+const char *ms_version(void);
+
+/* get_size_max_byte_any_len<F>(count) (format.h:114-151): the pessimistic
+ * allocation size of a column of `count` values in format f. */
+int ms_size_max_byte_any_len(const ms_format_desc *f, uint64_t count,
+                             uint64_t *out);
+
+/* F::get_size_max_byte(count) for count a multiple of F::m_BlockSize. */
+int ms_size_max_byte(const ms_format_desc *f, uint64_t count, uint64_t *out);
+
+/* F::m_BlockSize (format.h:68).  rle_f has none: returns MS_ERR_INVALID. */
+int ms_block_size(const ms_format_desc *f, uint64_t *out);
+
+/* Uploads a host byte image with its meta data into a new device column and
+ * builds the column's block index.  `image` holds meta->size_used bytes. */
+int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *image,
+                  const ms_col_meta *meta, void *stream);
+
+/* Uploads `count` uncompressed values as an uncompr_f column
+ * (meta = {count, 8*count, 0}), as the reference's generators produce. */
+int ms_col_upload_values(ms_col **out, const uint64_t *values, uint64_t count,
+                         void *stream);
+
+/* Copies the column's meta data to the host (synchronises `stream`). */
+int ms_col_get_meta(const ms_col *c, ms_col_meta *meta, void *stream);
+
+/* Copies the byte image [0, size_used) to `image` (capacity bytes) and the
+ * meta data to `meta`; synchronises `stream`.  The gap between the
+ * compressed part and the uncompressed rest reads as zero bytes. */
+int ms_col_download(const ms_col *c, void *image, uint64_t capacity,
+                    ms_col_meta *meta, void *stream);
+
+/* The column's format descriptor. */
+int ms_col_format(const ms_col *c, ms_format_desc *f);
+
+/* Device address of the byte image (64-byte aligned). */
+const void *ms_col_device_data(const ms_col *c);
+
+/* An upper bound of the column's count that is known without a device
+ * round trip (exact for uploaded columns). */
+uint64_t ms_col_count_bound(const ms_col *c);
+
+/* Releases the column (stream ordered on the stream it was created on). */
+int ms_col_free(ms_col *c);
+
+/* ------------------------------------------------------------------ */
+/* Morph (morph.h:109-241, rle.h:155-297)                               */
+/* ------------------------------------------------------------------ */
+
+/* Column-level morph from in's format to `dst`.  Compression writes the
+ * compressed main part followed, at the 64-byte aligned offset behind a gap
+ * of 2*dst::get_size_max_byte(dst::m_BlockSize), by the uncompressed rest
+ * (morph.h:157-197).  A same-format morph returns a new handle sharing
+ * nothing with `in` (the reference returns `in` itself). */
+int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in,
+             void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Operators                                                           */
+/* ------------------------------------------------------------------ */
+
+/* my_select_wit_t<cmp, ve, out_f, in_f>::apply(in, val): ascending positions
+ * i with cmp(x_i, val), written through the selective write-iterator into
+ * out_f.  `lanes` is the element count of the emulated vector extension
+ * (1 scalar, 2 sse, 4 avx2, 8 avx512): the input rest's last (rest mod
+ * lanes) values are processed "scalar" and their positions appended
+ * uncompressed, as select_compr.h:209-250 does. */
+int ms_select(ms_col **out, uint32_t cmp, uint64_t val,
+              const ms_format_desc *out_f, const ms_col *in, uint32_t lanes,
+              void *stream);
+
+/* my_project_wit_t<ve, out_f, data_f, pos_f>::apply(data, pos):
+ * out_i = data[pos_i].  Any data format is accepted (the reference needs
+ * random access, i.e. uncompr_f or static_vbp_f); positions must be below
+ * data's count (else MS_ERR_RANGE is reported by a later call that reads
+ * the column's status, see ms_col_check). */
+int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_col *data,
+               const ms_col *pos, uint32_t lanes, void *stream);
+
+/* agg_sum<ve, F>(in): an uncompr_f column holding the sum mod 2^64. */
+int ms_agg_sum(ms_col **out, const ms_col *in, void *stream);
+
+/* Fused select + agg_sum: *d_result (device pointer) = sum of x_i mod 2^64
+ * over all i with cmp(x_i, val), without materialising positions.  Equal to
+ * agg_sum(project(in, select(cmp, val, in))). */
+int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val,
+                  const ms_col *in, void *stream);
+
+/* intersect_sorted<ve, out_f, l_f, r_f>(l, r): multiset intersection of two
+ * ascending columns (min multiplicity, as scalar/intersect_uncompr.h:69-80),
+ * written through the selective write-iterator into out_f. */
+int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f,
+                        const ms_col *l, const ms_col *r, uint32_t lanes,
+                        void *stream);
+
+/* merge_sorted<ve, out_f, l_f, r_f>(l, r): multiset union of two ascending
+ * columns (an equal pair is emitted once, scalar/merge_uncompr.h:69-96). */
+int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f,
+                    const ms_col *l, const ms_col *r, void *stream);
+
+/* calc_binary<op, ve, out_f, l_f, r_f>(l, r): out_i = op(l_i, r_i) mod 2^64.
+ * Counts must match (MS_ERR_SIZE otherwise, scalar/calc_uncompr.h:84-88). */
+int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f,
+                   const ms_col *l, const ms_col *r, void *stream);
+
+/* Reads the column's device status word: MS_OK, or the first error a kernel
+ * recorded while producing it (e.g. MS_ERR_RANGE for an out-of-range
+ * position in ms_project).  Synchronises `stream`. */
+int ms_col_check(const ms_col *c, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MORPHSTORE_HIP_H */

@@ -1,0 +1,906 @@
+/*
+ * ms_oracle.c -- CPU restatement of MorphStore's compression-aware path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the
+ * product.  Each function names the reference code it restates; paths are
+ * relative to the reference root.  All arithmetic is uint64_t mod 2^64.
+ */
+#include "ms_oracle.h"
+
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define U64MAX 0xFFFFFFFFFFFFFFFFull
+
+/* ------------------------------------------------------------------ */
+/* Helpers: math.h:60-153, alignment_helper.h:25-46                    */
+/* ------------------------------------------------------------------ */
+
+/* math.h:108-113: effective bit width, 0 has one effective bit. */
+unsigned oc_effective_bitwidth(uint64_t v) { return 64u - (unsigned)__builtin_clzll(v | 1u); }
+
+/* math.h:146-153 */
+static uint64_t bw_max(unsigned bw) { return bw >= 64 ? U64MAX : ((1ull << bw) - 1ull); }
+
+/* create_aligned_ptr with 64-byte alignment, relative to an aligned base. */
+static uint64_t align64(uint64_t x) { return (x + 63ull) & ~63ull; }
+
+static uint64_t round_down(uint64_t x, uint64_t f) { return x / f * f; }
+static uint64_t round_up(uint64_t x, uint64_t f) { return (x + f - 1) / f * f; }
+
+static void inner_of(const ms_format_desc *f, ms_format_desc *in)
+{
+    memset(in, 0, sizeof *in);
+    in->kind = f->inner_kind;
+    in->bw = f->inner_bw;
+    in->block = f->inner_block;
+    in->page = f->inner_page;
+    in->step = f->inner_step;
+}
+
+/* Lane count of for_f's vector extension: it decides the reference-value
+ * minimum (for.h:121-161) and packs the inner format. */
+static unsigned for_lanes(const ms_format_desc *f)
+{
+    if (f->step) return f->step;
+    if (f->inner_kind != MS_UNCOMPR && f->inner_step) return f->inner_step;
+    return 1;
+}
+
+static int valid_leaf(const ms_format_desc *f)
+{
+    switch (f->kind) {
+    case MS_UNCOMPR:
+        return 1;
+    case MS_STATIC_VBP:
+        return f->bw >= 1 && f->bw <= 64 && f->step >= 1 && f->step <= 64;
+    case MS_DYNAMIC_VBP:
+        /* dynamic_vbp.h:45-53, 85-100 */
+        return f->step >= 1 && f->step <= 64 && f->block > 0 && f->page > 0 &&
+               f->block % (64u * f->step) == 0 && f->page % (8u * f->step) == 0;
+    default:
+        return 0;
+    }
+}
+
+static int valid(const ms_format_desc *f)
+{
+    ms_format_desc in;
+    uint64_t ibs;
+    switch (f->kind) {
+    case MS_UNCOMPR:
+    case MS_STATIC_VBP:
+    case MS_DYNAMIC_VBP:
+        return valid_leaf(f);
+    case MS_RLE:
+        return 1;
+    case MS_DELTA:
+    case MS_FOR:
+        inner_of(f, &in);
+        if (!valid_leaf(&in) || f->block == 0) return 0;
+        if (oc_block_size(&in, &ibs) != MS_OK) return 0;
+        if (f->block % ibs) return 0;            /* delta.h:80-84, for.h:86-90 */
+        if (f->kind == MS_DELTA)
+            return f->step >= 1 && f->step <= 64 && f->block % f->step == 0 &&
+                   (in.kind == MS_UNCOMPR || in.step == f->step);
+        return f->page > 0 && f->page % for_lanes(f) == 0 && f->block % for_lanes(f) == 0 &&
+               (in.kind == MS_UNCOMPR || in.step == for_lanes(f));   /* for.h:50-55 */
+    default:
+        return 0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Sizes: format.h:90-151, static_vbp.h:63-67, dynamic_vbp.h:106-120,  */
+/* delta.h:92-104, for.h:100-112, rle.h:44-46, column.h:182-196        */
+/* ------------------------------------------------------------------ */
+
+int oc_block_size(const ms_format_desc *f, uint64_t *out)
+{
+    switch (f->kind) {
+    case MS_UNCOMPR: *out = 1; return MS_OK;
+    case MS_STATIC_VBP: *out = 64ull * f->step; return MS_OK;   /* vbp_format_rndaccess.h:76 */
+    case MS_DYNAMIC_VBP:
+    case MS_DELTA:
+    case MS_FOR: *out = f->block; return MS_OK;
+    default: return MS_ERR_INVALID;                              /* rle_f has no m_BlockSize */
+    }
+}
+
+int oc_size_max_byte(const ms_format_desc *f, uint64_t n, uint64_t *out)
+{
+    ms_format_desc in;
+    uint64_t x;
+    if (!valid(f)) return MS_ERR_INVALID;
+    switch (f->kind) {
+    case MS_UNCOMPR: *out = 8 * n; return MS_OK;
+    case MS_STATIC_VBP: *out = n * f->bw / 8; return MS_OK;
+    case MS_DYNAMIC_VBP: *out = 8 * n + (n / f->block) * f->page; return MS_OK;
+    case MS_DELTA:
+        inner_of(f, &in);
+        return oc_size_max_byte(&in, n, out);
+    case MS_FOR: {
+        uint64_t pages = (n + (uint64_t)f->page * f->block - 1) / ((uint64_t)f->page * f->block);
+        inner_of(f, &in);
+        if (oc_size_max_byte(&in, f->block, &x) != MS_OK) return MS_ERR_INVALID;
+        *out = pages * 8ull * f->page + (n / f->block) * x;
+        return MS_OK;
+    }
+    case MS_RLE: *out = 16 * n; return MS_OK;
+    default: return MS_ERR_INVALID;
+    }
+}
+
+int oc_size_max_byte_any_len(const ms_format_desc *f, uint64_t n, uint64_t *out)
+{
+    uint64_t bs, c;
+    if (!valid(f)) return MS_ERR_INVALID;
+    if (f->kind == MS_UNCOMPR) { *out = 8 * n; return MS_OK; }
+    if (f->kind == MS_RLE) { *out = 16 * n; return MS_OK; }   /* rle.h:170-172 */
+    oc_block_size(f, &bs);
+    oc_size_max_byte(f, round_up(n, bs) + bs, &c);
+    *out = c + 8 * (n % bs + bs) + 63;                          /* + get_size_with_alignment_padding */
+    return MS_OK;
+}
+
+int oc_rest_offset(const ms_format_desc *f, uint64_t compr_end, uint64_t *out)
+{
+    uint64_t bs, m;
+    if (f->kind == MS_UNCOMPR) { *out = align64(compr_end); return MS_OK; }
+    if (oc_block_size(f, &bs) != MS_OK) return MS_ERR_INVALID;
+    oc_size_max_byte(f, bs, &m);
+    *out = align64(compr_end + 2 * m);
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* vbp_l<bw, S> (vbp_format_rndaccess.h:60-87; the routines emitted by */
+/* vbp_routine_gen.py:61-180).  Value i is code k = i / S of lane      */
+/* j = i % S; lane j's 64-bit word w sits at word index w*S + j.  Pack  */
+/* ORs codes in unmasked (vbp_routine_gen.py:101,111); unpack masks.   */
+/* ------------------------------------------------------------------ */
+
+static void vbp_pack(const uint64_t *in, uint64_t n, unsigned bw, unsigned S, uint64_t *out)
+{
+    const uint64_t per_lane = n / S;
+    for (unsigned j = 0; j < S; j++) {
+        for (uint64_t k = 0; k < per_lane; k++) {
+            const uint64_t v = in[k * S + j];
+            const uint64_t b = k * bw;
+            const uint64_t w = b >> 6;
+            const unsigned off = (unsigned)(b & 63);
+            out[w * S + j] |= v << off;
+            if (off + bw > 64) out[(w + 1) * S + j] |= v >> (64 - off);
+        }
+    }
+}
+
+static void vbp_unpack(const uint64_t *in, uint64_t n, unsigned bw, unsigned S, uint64_t *out)
+{
+    const uint64_t per_lane = n / S;
+    const uint64_t mask = bw_max(bw);
+    for (unsigned j = 0; j < S; j++) {
+        for (uint64_t k = 0; k < per_lane; k++) {
+            const uint64_t b = k * bw;
+            const uint64_t w = b >> 6;
+            const unsigned off = (unsigned)(b & 63);
+            uint64_t v = in[w * S + j] >> off;
+            if (off + bw > 64) v |= in[(w + 1) * S + j] << (64 - off);
+            out[k * S + j] = v & mask;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* dynamic_vbp_f (dynamic_vbp.h:122-308)                                */
+/* ------------------------------------------------------------------ */
+
+/* determine_max_bitwidth (dynamic_vbp.h:122-158): bw of (OR | 1). */
+static unsigned block_bw(const uint64_t *in, uint64_t n)
+{
+    uint64_t o = 1;
+    for (uint64_t i = 0; i < n; i++) o |= in[i];
+    return oc_effective_bitwidth(o);
+}
+
+static uint64_t dvbp_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out)
+{
+    const uint64_t B = f->block, P = f->page, E = f->step;
+    const uint64_t full_pages = round_down(n, B * P) / (B * P);
+    uint64_t pos = 0, i = 0;
+    for (uint64_t p = 0; p < full_pages; p++) {
+        uint8_t *meta = out + pos;
+        pos += P;
+        for (uint64_t q = 0; q < P; q++, i += B) {
+            const unsigned bw = block_bw(in + i, B);
+            meta[q] = (uint8_t)bw;
+            vbp_pack(in + i, B, bw, (unsigned)E, (uint64_t *)(out + pos));
+            pos += B * bw / 8;
+        }
+    }
+    if (i < n) {                                  /* dynamic_vbp.h:226-256 */
+        uint8_t *meta = out + pos;
+        const uint64_t nb = (n - i) / B;
+        pos += P;
+        for (uint64_t q = 0; q < nb; q++, i += B) {
+            const unsigned bw = block_bw(in + i, B);
+            meta[q] = (uint8_t)bw;
+            vbp_pack(in + i, B, bw, (unsigned)E, (uint64_t *)(out + pos));
+            pos += B * bw / 8;
+        }
+        memset(meta + nb, MS_VBP_BW_NOBLOCK, P - nb);
+    }
+    return pos;
+}
+
+static int dvbp_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *used)
+{
+    const uint64_t B = f->block, P = f->page, E = f->step;
+    uint64_t pos = 0, done = 0;
+    while (done < n) {
+        const uint8_t *meta = in + pos;
+        pos += P;
+        for (uint64_t q = 0; q < P; q++) {
+            const unsigned bw = meta[q];
+            if (bw == MS_VBP_BW_NOBLOCK) continue;
+            if (bw < 1 || bw > 64) return MS_ERR_INVALID;
+            vbp_unpack((const uint64_t *)(in + pos), B, bw, (unsigned)E, out + done);
+            pos += B * bw / 8;
+            done += B;
+        }
+    }
+    *used = pos;
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Leaf formats as inner formats of a cascade                          */
+/* ------------------------------------------------------------------ */
+
+static uint64_t leaf_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out)
+{
+    switch (f->kind) {
+    case MS_UNCOMPR: memcpy(out, in, 8 * n); return 8 * n;
+    case MS_STATIC_VBP:
+        vbp_pack(in, n, f->bw, f->step, (uint64_t *)out);
+        return n * f->bw / 8;
+    default:
+        return dvbp_compress(f, in, n, out);
+    }
+}
+
+static int leaf_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *used)
+{
+    switch (f->kind) {
+    case MS_UNCOMPR: memcpy(out, in, 8 * n); *used = 8 * n; return MS_OK;
+    case MS_STATIC_VBP:
+        vbp_unpack((const uint64_t *)in, n, f->bw, f->step, out);
+        *used = n * f->bw / 8;
+        return MS_OK;
+    default:
+        return dvbp_decompress(f, in, n, out, used);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* delta_f (delta.h:106-266): d_i = x_i - x_{i-S}, x_{<0} = 0, carried  */
+/* over blocks, calls and write-iterator flushes; each block is one    */
+/* inner morph_batch call.                                             */
+/* ------------------------------------------------------------------ */
+
+static uint64_t delta_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out, uint64_t *last)
+{
+    ms_format_desc inner;
+    const uint64_t L = f->block, S = f->step;
+    uint64_t pos = 0;
+    uint64_t *tmp = (uint64_t *)malloc(8 * L);
+    inner_of(f, &inner);
+    for (uint64_t b = 0; b < n / L; b++) {
+        for (uint64_t i = 0; i < L; i++) {
+            const uint64_t x = in[b * L + i];
+            tmp[i] = x - last[i % S];
+            last[i % S] = x;
+        }
+        pos += leaf_compress(&inner, tmp, L, out + pos);
+    }
+    free(tmp);
+    return pos;
+}
+
+static int delta_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *used)
+{
+    ms_format_desc inner;
+    const uint64_t L = f->block, S = f->step;
+    uint64_t pos = 0, u;
+    uint64_t pre[64] = {0};
+    inner_of(f, &inner);
+    for (uint64_t b = 0; b < n / L; b++) {
+        uint64_t *o = out + b * L;
+        int rc = leaf_decompress(&inner, in + pos, L, o, &u);
+        if (rc) return rc;
+        pos += u;
+        for (uint64_t i = 0; i < L; i++) {
+            pre[i % S] += o[i];
+            o[i] = pre[i % S];
+        }
+    }
+    *used = pos;
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* for_f (for.h:121-365)                                               */
+/* ------------------------------------------------------------------ */
+
+/* determine_reference_value (for.h:121-161): a lane-wise minimum over
+ * the block's vectors, then the unsigned minimum of the lanes.  With more
+ * than one lane the vector min is signed (_mm512_min_epi64, calc_avx512.h:
+ * 69-78; the AVX2 min compares signed too); the scalar one is unsigned. */
+static uint64_t for_ref(const uint64_t *in, uint64_t L, unsigned E)
+{
+    uint64_t r = U64MAX;
+    if (E == 1) {
+        for (uint64_t i = 0; i < L; i++) if (in[i] < r) r = in[i];
+    } else {
+        for (unsigned j = 0; j < E; j++) {
+            int64_t m = (int64_t)in[j];
+            for (uint64_t i = j + E; i < L; i += E) if ((int64_t)in[i] < m) m = (int64_t)in[i];
+            if ((uint64_t)m < r) r = (uint64_t)m;
+        }
+    }
+    if (r == U64MAX) r--;                                 /* for.h:155-158 */
+    return r;
+}
+
+static uint64_t for_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out)
+{
+    ms_format_desc inner;
+    const uint64_t L = f->block, P = f->page;
+    const unsigned E = for_lanes(f);
+    uint64_t pos = 0, done = 0;
+    uint64_t *tmp = (uint64_t *)malloc(8 * L);
+    inner_of(f, &inner);
+    while (done < n) {
+        uint64_t *meta = (uint64_t *)(out + pos);
+        uint64_t nb = (n - done) / L;
+        pos += 8 * P;
+        if (nb > P) nb = P;
+        if (nb < P) meta[nb] = U64MAX;                     /* for.h:214-220 */
+        for (uint64_t q = 0; q < nb; q++, done += L) {
+            const uint64_t ref = for_ref(in + done, L, E);
+            meta[q] = ref;
+            for (uint64_t i = 0; i < L; i++) tmp[i] = in[done + i] - ref;
+            pos += leaf_compress(&inner, tmp, L, out + pos);
+        }
+    }
+    free(tmp);
+    return pos;
+}
+
+static int for_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *used)
+{
+    ms_format_desc inner;
+    const uint64_t L = f->block, P = f->page;
+    uint64_t pos = 0, done = 0, u;
+    inner_of(f, &inner);
+    while (done < n) {
+        const uint64_t *meta = (const uint64_t *)(in + pos);
+        pos += 8 * P;
+        for (uint64_t q = 0; q < P; q++) {
+            const uint64_t ref = meta[q];
+            int rc;
+            if (ref == U64MAX) break;                      /* for.h:318-324 */
+            rc = leaf_decompress(&inner, in + pos, L, out + done, &u);
+            if (rc) return rc;
+            pos += u;
+            for (uint64_t i = 0; i < L; i++) out[done + i] += ref;
+            done += L;
+        }
+    }
+    *used = pos;
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* rle_f (rle.h:95-147, 218-256): maximal runs as (value, length).     */
+/* For n == 0 this emits no run; the reference reads in[0] and emits   */
+/* one bogus run (rle.h:113-140).                                      */
+/* ------------------------------------------------------------------ */
+
+static uint64_t rle_compress(const uint64_t *in, uint64_t n, uint8_t *out8)
+{
+    uint64_t *out = (uint64_t *)out8, r = 0, start = 0;
+    if (n == 0) return 0;
+    for (uint64_t i = 1; i <= n; i++) {
+        if (i == n || in[i] != in[start]) {
+            out[2 * r] = in[start];
+            out[2 * r + 1] = i - start;
+            r++;
+            start = i;
+        }
+    }
+    return 16 * r;
+}
+
+static void rle_decompress(const uint8_t *in8, uint64_t words, uint64_t *out)
+{
+    const uint64_t *in = (const uint64_t *)in8;
+    uint64_t o = 0;
+    for (uint64_t i = 0; i + 1 < words; i += 2)
+        for (uint64_t k = 0; k < in[i + 1]; k++) out[o++] = in[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* morph_batch                                                          */
+/* ------------------------------------------------------------------ */
+
+int oc_morph_batch_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out, uint64_t *out_bytes)
+{
+    uint64_t bs, last[64] = {0};
+    if (!valid(f)) return MS_ERR_INVALID;
+    if (f->kind != MS_RLE) {
+        oc_block_size(f, &bs);
+        if (n % bs) return MS_ERR_INVALID;
+    }
+    switch (f->kind) {
+    case MS_UNCOMPR:
+    case MS_STATIC_VBP:
+    case MS_DYNAMIC_VBP: *out_bytes = leaf_compress(f, in, n, out); return MS_OK;
+    case MS_DELTA: *out_bytes = delta_compress(f, in, n, out, last); return MS_OK;
+    case MS_FOR: *out_bytes = for_compress(f, in, n, out); return MS_OK;
+    case MS_RLE: *out_bytes = rle_compress(in, n, out); return MS_OK;
+    default: return MS_ERR_INVALID;
+    }
+}
+
+int oc_morph_batch_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *in_bytes)
+{
+    if (!valid(f)) return MS_ERR_INVALID;
+    switch (f->kind) {
+    case MS_UNCOMPR:
+    case MS_STATIC_VBP:
+    case MS_DYNAMIC_VBP: return leaf_decompress(f, in, n, out, in_bytes);
+    case MS_DELTA: return delta_decompress(f, in, n, out, in_bytes);
+    case MS_FOR: return for_decompress(f, in, n, out, in_bytes);
+    default: return MS_ERR_INVALID;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Columns                                                             */
+/* ------------------------------------------------------------------ */
+
+static int col_alloc(oc_col *c, const ms_format_desc *f, uint64_t size_alloc)
+{
+    memset(c, 0, sizeof *c);
+    c->f = *f;
+    c->size_alloc = size_alloc;
+    c->data = (uint8_t *)calloc(size_alloc + 64, 1);
+    return c->data ? MS_OK : MS_ERR_NOMEM;
+}
+
+void oc_col_free(oc_col *c)
+{
+    if (c && c->data) free(c->data);
+    if (c) c->data = NULL;
+}
+
+int oc_col_from_values(oc_col *out, const uint64_t *v, uint64_t n)
+{
+    ms_format_desc u;
+    memset(&u, 0, sizeof u);
+    u.kind = MS_UNCOMPR;
+    if (col_alloc(out, &u, 8 * n)) return MS_ERR_NOMEM;
+    if (n) memcpy(out->data, v, 8 * n);
+    out->count = n;
+    out->size_used = 8 * n;          /* set_meta_data(count, size): sizeCompr = 0 */
+    out->size_compr = 0;
+    return MS_OK;
+}
+
+/* column.h:205-214 */
+uint64_t oc_count_values_uncompr(const oc_col *c)
+{
+    uint64_t r;
+    if (c->size_compr == c->size_used) return 0;
+    if (oc_rest_offset(&c->f, c->size_compr, &r)) return 0;
+    return (c->size_used - r) / 8;
+}
+
+/* morph.h:157-197 (compression), rle.h:155-188 */
+int oc_morph_compress(oc_col *out, const ms_format_desc *dst, const uint64_t *in, uint64_t n)
+{
+    uint64_t alloc, bs, nc, c = 0, r;
+    if (!valid(dst)) return MS_ERR_INVALID;
+    if (dst->kind == MS_UNCOMPR) return oc_col_from_values(out, in, n);
+    if (dst->kind == MS_RLE) {
+        if (col_alloc(out, dst, 16 * n + 16)) return MS_ERR_NOMEM;
+        c = rle_compress(in, n, out->data);
+        out->count = n;
+        out->size_used = out->size_compr = c;
+        return MS_OK;
+    }
+    oc_size_max_byte_any_len(dst, n, &alloc);
+    if (col_alloc(out, dst, alloc)) return MS_ERR_NOMEM;
+    oc_block_size(dst, &bs);
+    nc = round_down(n, bs);
+    if (nc) oc_morph_batch_compress(dst, in, nc, out->data, &c);
+    out->count = n;
+    out->size_compr = c;
+    if (n - nc) {
+        oc_rest_offset(dst, c, &r);
+        memcpy(out->data + r, in + nc, 8 * (n - nc));
+        out->size_used = r + 8 * (n - nc);
+    } else {
+        out->size_used = c;
+    }
+    return MS_OK;
+}
+
+/* morph.h:203-241 (decompression), rle.h:264-297 */
+int oc_morph_decompress(const oc_col *in, uint64_t *out)
+{
+    uint64_t rest, nc, used, r;
+    if (!valid(&in->f)) return MS_ERR_INVALID;
+    if (in->f.kind == MS_UNCOMPR) { memcpy(out, in->data, 8 * in->count); return MS_OK; }
+    if (in->f.kind == MS_RLE) { rle_decompress(in->data, in->size_used / 8, out); return MS_OK; }
+    rest = oc_count_values_uncompr(in);
+    nc = in->count - rest;
+    if (nc) {
+        int rc = oc_morph_batch_decompress(&in->f, in->data, nc, out, &used);
+        if (rc) return rc;
+    }
+    if (rest) {
+        oc_rest_offset(&in->f, in->size_compr, &r);
+        memcpy(out + nc, in->data + r, 8 * rest);
+    }
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Write-iterator output (write_iterator.h:66-196; delta.h:369-490 for */
+/* the delta carry; uncompr.h:125-221 for the unbuffered uncompr one)  */
+/* ------------------------------------------------------------------ */
+
+int oc_wit_emit(oc_col *out, const ms_format_desc *f, const uint64_t *v, uint64_t m,
+                uint64_t extra_capacity, uint64_t *append_off)
+{
+    uint64_t alloc, bs, K, pos = 0, full, rem, nc, last[64] = {0}, r, b;
+    if (!valid(f) || f->kind == MS_RLE) return MS_ERR_INVALID;   /* rle_f has no write-iterator */
+    if (f->kind == MS_UNCOMPR) {
+        if (col_alloc(out, f, 8 * m + extra_capacity)) return MS_ERR_NOMEM;
+        if (m) memcpy(out->data, v, 8 * m);
+        out->count = m;
+        out->size_used = 8 * m;
+        out->size_compr = 0;
+        *append_off = 8 * m;
+        return MS_OK;
+    }
+    oc_size_max_byte_any_len(f, m, &alloc);
+    oc_block_size(f, &bs);
+    if (col_alloc(out, f, alloc + extra_capacity + 2 * 64)) return MS_ERR_NOMEM;
+    K = round_up(bs, 2048);                                      /* m_CountBuffer */
+    full = m / K;
+    for (uint64_t c = 0; c < full; c++) {
+        if (f->kind == MS_DELTA) pos += delta_compress(f, v + c * K, K, out->data + pos, last);
+        else { oc_morph_batch_compress(f, v + c * K, K, out->data + pos, &b); pos += b; }
+    }
+    rem = m - full * K;
+    out->count = m;
+    if (rem) {
+        nc = round_down(rem, bs);
+        if (nc) {
+            if (f->kind == MS_DELTA) pos += delta_compress(f, v + full * K, nc, out->data + pos, last);
+            else { oc_morph_batch_compress(f, v + full * K, nc, out->data + pos, &b); pos += b; }
+        }
+        out->size_compr = pos;
+        oc_rest_offset(f, pos, &r);
+        if (rem - nc) {
+            memcpy(out->data + r, v + full * K + nc, 8 * (rem - nc));
+            out->size_used = r + 8 * (rem - nc);
+            *append_off = out->size_used;
+        } else {
+            out->size_used = pos;
+            *append_off = r;
+        }
+    } else {
+        out->size_compr = pos;
+        out->size_used = pos;
+        oc_rest_offset(f, pos, append_off);
+    }
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Operators                                                           */
+/* ------------------------------------------------------------------ */
+
+static int cmp_apply(uint32_t cmp, uint64_t x, uint64_t val)
+{
+    switch (cmp) {
+    case MS_CMP_EQ: return x == val;
+    case MS_CMP_NE: return x != val;
+    case MS_CMP_LT: return x < val;
+    case MS_CMP_LE: return x <= val;
+    case MS_CMP_GT: return x > val;
+    case MS_CMP_GE: return x >= val;
+    default: return 0;
+    }
+}
+
+static int decode_all(const oc_col *in, uint64_t **vals)
+{
+    *vals = (uint64_t *)malloc(8 * (in->count + 1));
+    if (!*vals) return MS_ERR_NOMEM;
+    return oc_morph_decompress(in, *vals);
+}
+
+/* select_compr.h:108-259 */
+int oc_select(oc_col *out, uint32_t cmp, uint64_t val, const ms_format_desc *out_f,
+              const oc_col *in, uint32_t lanes)
+{
+    uint64_t *x, *pos, na = 0, nb = 0, app;
+    uint64_t n = in->count, rest = oc_count_values_uncompr(in), nvec;
+    int has_rest = in->size_compr != in->size_used, rc;
+    if (cmp > MS_CMP_GE || lanes == 0 || lanes > 64) return MS_ERR_INVALID;
+    rc = decode_all(in, &x);
+    if (rc) { free(x); return rc; }
+    pos = (uint64_t *)malloc(8 * (n + 1));
+    nvec = has_rest ? (n - rest) + round_down(rest, lanes) : n;
+    for (uint64_t i = 0; i < nvec; i++) if (cmp_apply(cmp, x[i], val)) pos[na++] = i;
+    rc = oc_wit_emit(out, out_f, pos, na, 8 * lanes + 64, &app);
+    if (!rc && has_rest && rest % lanes) {
+        uint64_t *dst = (uint64_t *)(out->data + app);
+        for (uint64_t i = nvec; i < n; i++) if (cmp_apply(cmp, x[i], val)) dst[nb++] = i;
+        out->size_used = app + 8 * nb;
+        out->count = na + nb;
+    }
+    free(x);
+    free(pos);
+    return rc;
+}
+
+/* project_compr.h:80-261 (random access replaced by decompression) */
+int oc_project(oc_col *out, const ms_format_desc *out_f, const oc_col *data,
+               const oc_col *pos, uint32_t lanes)
+{
+    uint64_t *x, *p, *v, app, np = pos->count, rest = oc_count_values_uncompr(pos), nvec;
+    int has_rest = pos->size_compr != pos->size_used, rc;
+    if (lanes == 0 || lanes > 64) return MS_ERR_INVALID;
+    if ((rc = decode_all(data, &x))) { free(x); return rc; }
+    if ((rc = decode_all(pos, &p))) { free(x); free(p); return rc; }
+    v = (uint64_t *)malloc(8 * (np + 1));
+    for (uint64_t i = 0; i < np; i++) {
+        if (p[i] >= data->count) { free(x); free(p); free(v); return MS_ERR_RANGE; }
+        v[i] = x[p[i]];
+    }
+    nvec = has_rest ? (np - rest) + round_down(rest, lanes) : np;
+    rc = oc_wit_emit(out, out_f, v, nvec, 8 * lanes + 64, &app);
+    if (!rc && has_rest && rest % lanes) {
+        memcpy(out->data + app, v + nvec, 8 * (np - nvec));
+        out->size_used = app + 8 * (np - nvec);
+        out->count = np;
+    }
+    free(x); free(p); free(v);
+    return rc;
+}
+
+/* agg_sum_compr.h:78-153, general_vectorized/agg_sum_uncompr.h:102-129 */
+int oc_agg_sum(oc_col *out, const oc_col *in)
+{
+    uint64_t *x, s = 0;
+    ms_format_desc u;
+    int rc = decode_all(in, &x);
+    if (rc) { free(x); return rc; }
+    for (uint64_t i = 0; i < in->count; i++) s += x[i];
+    free(x);
+    memset(&u, 0, sizeof u);
+    u.kind = MS_UNCOMPR;
+    if (col_alloc(out, &u, 8)) return MS_ERR_NOMEM;
+    memcpy(out->data, &s, 8);
+    out->count = 1;
+    out->size_used = 8;
+    out->size_compr = in->f.kind == MS_UNCOMPR ? 0 : 8;
+    return MS_OK;
+}
+
+/* scalar/intersect_uncompr.h:41-86 */
+int oc_intersect_sorted(oc_col *out, const ms_format_desc *out_f, const oc_col *l, const oc_col *r)
+{
+    uint64_t *a, *b, *o, i = 0, j = 0, k = 0, app;
+    int rc;
+    if ((rc = decode_all(l, &a))) { free(a); return rc; }
+    if ((rc = decode_all(r, &b))) { free(a); free(b); return rc; }
+    o = (uint64_t *)malloc(8 * (l->count + 1));
+    while (i < l->count && j < r->count) {
+        if (a[i] < b[j]) i++;
+        else if (b[j] < a[i]) j++;
+        else { o[k++] = a[i]; i++; j++; }
+    }
+    rc = oc_wit_emit(out, out_f, o, k, 0, &app);
+    free(a); free(b); free(o);
+    return rc;
+}
+
+/* scalar/merge_uncompr.h:39-103 */
+int oc_merge_sorted(oc_col *out, const ms_format_desc *out_f, const oc_col *l, const oc_col *r)
+{
+    uint64_t *a, *b, *o, i = 0, j = 0, k = 0, app;
+    int rc;
+    if ((rc = decode_all(l, &a))) { free(a); return rc; }
+    if ((rc = decode_all(r, &b))) { free(a); free(b); return rc; }
+    o = (uint64_t *)malloc(8 * (l->count + r->count + 1));
+    while (i < l->count && j < r->count) {
+        if (a[i] < b[j]) o[k++] = a[i++];
+        else if (b[j] < a[i]) o[k++] = b[j++];
+        else { o[k++] = a[i]; i++; j++; }
+    }
+    while (i < l->count) o[k++] = a[i++];
+    while (j < r->count) o[k++] = b[j++];
+    rc = oc_wit_emit(out, out_f, o, k, 0, &app);
+    free(a); free(b); free(o);
+    return rc;
+}
+
+static uint64_t calc_apply(uint32_t op, uint64_t a, uint64_t b)
+{
+    switch (op) {
+    case MS_OP_ADD: return a + b;
+    case MS_OP_SUB: return a - b;
+    case MS_OP_MUL: return a * b;
+    case MS_OP_DIV: return b ? a / b : 0;
+    case MS_OP_MOD: return b ? a % b : 0;
+    case MS_OP_MIN: return a < b ? a : b;
+    case MS_OP_MAX: return a > b ? a : b;
+    case MS_OP_AND: return a & b;
+    case MS_OP_OR: return a | b;
+    default: return a ^ b;
+    }
+}
+
+/* scalar/calc_uncompr.h:70-106 */
+int oc_calc_binary(oc_col *out, uint32_t op, const ms_format_desc *out_f, const oc_col *l, const oc_col *r)
+{
+    uint64_t *a, *b, app;
+    int rc;
+    if (op > MS_OP_XOR) return MS_ERR_INVALID;
+    if (l->count != r->count) return MS_ERR_SIZE;
+    if ((rc = decode_all(l, &a))) { free(a); return rc; }
+    if ((rc = decode_all(r, &b))) { free(a); free(b); return rc; }
+    for (uint64_t i = 0; i < l->count; i++) a[i] = calc_apply(op, a[i], b[i]);
+    rc = oc_wit_emit(out, out_f, a, l->count, 0, &app);
+    free(a); free(b);
+    return rc;
+}
+
+int oc_select_sum(uint64_t *result, uint32_t cmp, uint64_t val, const oc_col *in)
+{
+    uint64_t *x, s = 0;
+    int rc = decode_all(in, &x);
+    if (rc) { free(x); return rc; }
+    for (uint64_t i = 0; i < in->count; i++) if (cmp_apply(cmp, x[i], val)) s += x[i];
+    free(x);
+    *result = s;
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Streaming fused select + sum: decodes one block at a time into a    */
+/* stack buffer and folds it, like decompress_and_process_batch with   */
+/* the select core feeding the agg_sum core.                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    const oc_col *in;
+    const uint64_t *blk_off;   /* dynamic_vbp: payload offset per block  */
+    const uint8_t *blk_bw;
+    uint64_t b0, b1;           /* block range                             */
+    uint32_t cmp;
+    uint64_t val;
+    uint64_t sum;
+} ss_job;
+
+static uint64_t fold(uint32_t cmp, uint64_t val, const uint64_t *x, uint64_t n)
+{
+    uint64_t s = 0;
+    switch (cmp) {
+    case MS_CMP_LT: for (uint64_t i = 0; i < n; i++) s += x[i] < val ? x[i] : 0; break;
+    case MS_CMP_EQ: for (uint64_t i = 0; i < n; i++) s += x[i] == val ? x[i] : 0; break;
+    default: for (uint64_t i = 0; i < n; i++) s += cmp_apply(cmp, x[i], val) ? x[i] : 0; break;
+    }
+    return s;
+}
+
+static void *ss_worker(void *arg)
+{
+    ss_job *jb = (ss_job *)arg;
+    const oc_col *in = jb->in;
+    uint64_t buf[4096];
+    uint64_t s = 0;
+    if (in->f.kind == MS_DYNAMIC_VBP) {
+        const uint64_t B = in->f.block;
+        for (uint64_t b = jb->b0; b < jb->b1; b++) {
+            vbp_unpack((const uint64_t *)(in->data + jb->blk_off[b]), B, jb->blk_bw[b], in->f.step, buf);
+            s += fold(jb->cmp, jb->val, buf, B);
+        }
+    } else if (in->f.kind == MS_STATIC_VBP) {
+        const uint64_t B = 64ull * in->f.step, bytes = B * in->f.bw / 8;
+        for (uint64_t b = jb->b0; b < jb->b1; b++) {
+            vbp_unpack((const uint64_t *)(in->data + b * bytes), B, in->f.bw, in->f.step, buf);
+            s += fold(jb->cmp, jb->val, buf, B);
+        }
+    } else {
+        const uint64_t *x = (const uint64_t *)in->data;
+        s = fold(jb->cmp, jb->val, x + jb->b0, jb->b1 - jb->b0);
+    }
+    jb->sum = s;
+    return NULL;
+}
+
+int oc_select_sum_stream(uint64_t *result, uint32_t cmp, uint64_t val, const oc_col *in, int threads)
+{
+    uint64_t nblocks = 0, s = 0, rest, nc, r;
+    uint64_t *off = NULL;
+    uint8_t *bw = NULL;
+    ss_job jobs[256];
+    pthread_t th[256];
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    if (in->f.kind != MS_DYNAMIC_VBP && in->f.kind != MS_STATIC_VBP && in->f.kind != MS_UNCOMPR)
+        return oc_select_sum(result, cmp, val, in);
+    if ((in->f.kind == MS_DYNAMIC_VBP && in->f.block > 4096) ||
+        (in->f.kind == MS_STATIC_VBP && 64ull * in->f.step > 4096))
+        return oc_select_sum(result, cmp, val, in);
+    rest = oc_count_values_uncompr(in);
+    nc = in->count - rest;
+    if (in->f.kind == MS_DYNAMIC_VBP) {
+        /* walk the page headers once to find every block's payload */
+        const uint64_t B = in->f.block, P = in->f.page;
+        uint64_t pos = 0, done = 0;
+        nblocks = nc / B;
+        off = (uint64_t *)malloc(8 * (nblocks + 1));
+        bw = (uint8_t *)malloc(nblocks + 1);
+        nblocks = 0;
+        while (done < nc) {
+            const uint8_t *meta = in->data + pos;
+            pos += P;
+            for (uint64_t q = 0; q < P; q++) {
+                if (meta[q] == MS_VBP_BW_NOBLOCK) continue;
+                off[nblocks] = pos;
+                bw[nblocks++] = meta[q];
+                pos += B * meta[q] / 8;
+                done += B;
+            }
+        }
+    } else if (in->f.kind == MS_STATIC_VBP) {
+        nblocks = nc / (64ull * in->f.step);
+    } else {
+        nblocks = in->count;   /* uncompr: the whole column is the rest */
+        rest = 0;
+    }
+    for (int t = 0; t < threads; t++) {
+        jobs[t].in = in;
+        jobs[t].blk_off = off;
+        jobs[t].blk_bw = bw;
+        jobs[t].b0 = nblocks * t / threads;
+        jobs[t].b1 = nblocks * (t + 1) / threads;
+        jobs[t].cmp = cmp;
+        jobs[t].val = val;
+        if (threads == 1) ss_worker(&jobs[0]);
+        else pthread_create(&th[t], NULL, ss_worker, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) {
+        if (threads > 1) pthread_join(th[t], NULL);
+        s += jobs[t].sum;
+    }
+    if (rest) {
+        oc_rest_offset(&in->f, in->size_compr, &r);
+        s += fold(cmp, val, (const uint64_t *)(in->data + r), rest);
+    }
+    free(off);
+    free(bw);
+    *result = s;
+    return MS_OK;
+}

@@ -1,0 +1,301 @@
+"""Pins the CPU oracle against the reference's known-answer tests, closed
+forms, bit-level re-derivations and brute force (no GPU needed)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from engine_amd import datagen
+from engine_amd.formats import (MS_CMP_EQ, MS_CMP_GE, MS_CMP_GT, MS_CMP_LE, MS_CMP_LT, MS_CMP_NE,
+                                 MS_OP_ADD, default_delta_dynamic_vbp_f, default_dynamic_vbp_f,
+                                 default_for_dynamic_vbp_f, delta_f, dynamic_vbp_f, for_f, rle_f,
+                                 static_vbp_f, uncompr_f)
+from tests import refmodel as rm
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "reference_kat.json")))
+
+
+def rng_vals(seed, n, bits):
+    return datagen.uniform_bits(seed, n, bits)
+
+
+# ------------------------------------------------------------------ KATs
+def test_kat_select(oracle):
+    k = KAT["select_less"]
+    out = oracle.select(MS_CMP_LT, k["val"], uncompr_f(), oracle.from_values(k["in"]), 1)
+    assert list(out.values()) == k["out"]
+    assert out.meta() == (2, 16, 0)
+
+
+def test_kat_project(oracle):
+    k = KAT["project"]
+    out = oracle.project(uncompr_f(), oracle.from_values(k["data"]), oracle.from_values(k["pos"]), 1)
+    assert list(out.values()) == k["out"]
+
+
+def test_kat_agg_sum(oracle):
+    k = KAT["agg_sum"]
+    out = oracle.agg_sum(oracle.from_values(k["in"]))
+    assert list(out.values()) == k["out"]
+    k = KAT["agg_sum_1_to_100"]
+    lo, hi = k["in_range"]
+    assert list(oracle.agg_sum(oracle.from_values(range(lo, hi + 1))).values()) == k["out"]
+
+
+def test_kat_intersect_merge_calc(oracle):
+    k = KAT["intersect_sorted"]
+    out = oracle.intersect_sorted(uncompr_f(), oracle.from_values(k["l"]), oracle.from_values(k["r"]))
+    assert list(out.values()) == k["out"]
+    k = KAT["merge_sorted"]
+    out = oracle.merge_sorted(uncompr_f(), oracle.from_values(k["l"]), oracle.from_values(k["r"]))
+    assert list(out.values()) == k["out"]
+    k = KAT["calc_binary_plus"]
+    out = oracle.calc_binary(MS_OP_ADD, uncompr_f(), oracle.from_values(k["l"]), oracle.from_values(k["r"]))
+    assert list(out.values()) == k["out"]
+
+
+# --------------------------------------------------------------- vbp_l
+@pytest.mark.parametrize("S", [1, 2, 4, 8])
+@pytest.mark.parametrize("bw", [1, 2, 3, 7, 12, 16, 31, 32, 33, 47, 63, 64])
+def test_static_vbp_matches_bit_strings(oracle, bw, S):
+    n = 64 * S * 2
+    v = rng_vals(bw * 100 + S, n, bw)
+    img = oracle.morph_batch_compress(static_vbp_f(bw, S), v)
+    assert img.tobytes() == rm.bitstring_pack(v, bw, S).tobytes()
+    assert len(img) == n * bw // 8
+    back, used = oracle.morph_batch_decompress(static_vbp_f(bw, S), img, n)
+    assert used == len(img)
+    assert np.array_equal(back, v)
+    assert np.array_equal(rm.bitstring_unpack(img, n, bw, S), v)
+
+
+@pytest.mark.parametrize("bw", [3, 5, 12, 20, 40])
+def test_static_vbp_unmasked_pack_matches_emitted_routine(oracle, bw):
+    """Values wider than bw corrupt neighbours exactly as the generated
+    routine's shift/or sequence does (vbp_routine_gen.py:97-112)."""
+    S = 4
+    n = 64 * S
+    v = rng_vals(7 + bw, n, min(64, bw + 6))
+    img = oracle.morph_batch_compress(static_vbp_f(bw, S), v)
+    assert img.tobytes() == rm.emitted_pack(v, bw, S).tobytes()
+
+
+# ---------------------------------------------------------- dynamic_vbp
+@pytest.mark.parametrize("B,P,S", [(512, 64, 8), (512, 2048, 8), (256, 32, 4), (64, 8, 1), (128, 16, 2)])
+@pytest.mark.parametrize("nblocks", [1, 3, 8, 70])
+def test_dynamic_vbp_closed_form(oracle, B, P, S, nblocks):
+    n = B * nblocks
+    v = datagen.dvbp_random_bw(nblocks * 13 + S, n, block=B, lo=1, hi=40)
+    v[:B] = 0                                       # an all-zero block packs with bw 1
+    f = dynamic_vbp_f(B, P, S)
+    img = oracle.morph_batch_compress(f, v)
+    assert img.tobytes() == rm.dvbp_expected(v, B, P, S).tobytes()
+    vals, metas, used = rm.dvbp_walk(img, n, B, P, S)
+    assert used == len(img)
+    assert np.array_equal(vals, v)
+    assert metas[0][0] == 1
+    back, used2 = oracle.morph_batch_decompress(f, img, n)
+    assert used2 == len(img) and np.array_equal(back, v)
+
+
+# ---------------------------------------------------------------- delta
+@pytest.mark.parametrize("sorted_", [True, False])
+@pytest.mark.parametrize("n", [1024, 3072, 10240])
+def test_delta_closed_form(oracle, sorted_, n):
+    f = default_delta_dynamic_vbp_f(8)
+    v = datagen.sorted_geometric(n, n) if sorted_ else rng_vals(n, n, 30)
+    img = oracle.morph_batch_compress(f, v)
+    d = rm.lane_delta(v, 8)
+    expect = b"".join(rm.dvbp_expected(d[i:i + 1024], 512, 64, 8).tobytes() for i in range(0, n, 1024))
+    assert img.tobytes() == expect
+    back, used = oracle.morph_batch_decompress(f, img, n)
+    assert used == len(img) and np.array_equal(back, v)
+
+
+# ------------------------------------------------------------------ for
+@pytest.mark.parametrize("nblocks", [1, 7, 8, 9, 17])
+def test_for_closed_form(oracle, nblocks):
+    f = default_for_dynamic_vbp_f(8)
+    L, P = 1024, 8
+    n = L * nblocks
+    v = datagen.for_base_offset(nblocks, n)
+    v[5] = np.uint64(2**63 + 5)                     # exercises the signed lane minimum
+    img = oracle.morph_batch_compress(f, v)
+    pos = 0
+    for p0 in range(0, nblocks, P):
+        nb = min(P, nblocks - p0)
+        refs = np.frombuffer(img[pos:pos + 8 * P].tobytes(), np.uint64)
+        pos += 8 * P
+        if nb < P:
+            assert refs[nb] == np.uint64(2**64 - 1)
+        for q in range(nb):
+            blk = v[(p0 + q) * L:(p0 + q + 1) * L]
+            ref = rm.for_reference(blk, 8)
+            assert int(refs[q]) == ref
+            with np.errstate(over="ignore"):
+                inner = rm.dvbp_expected(blk - np.uint64(ref), 512, 64, 8)
+            assert img[pos:pos + len(inner)].tobytes() == inner.tobytes()
+            pos += len(inner)
+    assert pos == len(img)
+    back, used = oracle.morph_batch_decompress(f, img, n)
+    assert used == len(img) and np.array_equal(back, v)
+
+
+def test_for_reference_value_special_case(oracle):
+    f = for_f(1024, 8, dynamic_vbp_f(512, 64, 8))
+    v = np.full(1024, 2**64 - 1, np.uint64)
+    img = oracle.morph_batch_compress(f, v)
+    assert int(np.frombuffer(img[:8].tobytes(), np.uint64)[0]) == 2**64 - 2
+    assert np.array_equal(oracle.morph_batch_decompress(f, img, 1024)[0], v)
+
+
+# ------------------------------------------------------------------ rle
+@pytest.mark.parametrize("n", [0, 1, 2, 100, 5000])
+def test_rle_maximal_runs(oracle, n):
+    v = datagen.rle_runs(n + 3, n, mean_len=5) if n else np.zeros(0, np.uint64)
+    col = oracle.morph(rle_f(), v)
+    runs = rm.rle_runs(v)
+    assert col.image().tobytes() == runs.reshape(-1).tobytes()
+    assert col.meta() == (n, 16 * len(runs), 16 * len(runs))
+    assert np.array_equal(col.values(), v)
+
+
+# ----------------------------------------------------- column layout A.6
+def test_gap_sizes_from_format_definitions(oracle):
+    # 2 * F::get_size_max_byte(F::m_BlockSize) for the config formats
+    assert oracle.rest_offset(dynamic_vbp_f(512, 2048, 8), 0) == 12288
+    assert oracle.rest_offset(static_vbp_f(12, 1), 0) == 192
+    assert oracle.rest_offset(default_delta_dynamic_vbp_f(8), 0) == 16640
+    assert oracle.rest_offset(default_for_dynamic_vbp_f(8), 0) == 16768
+    assert oracle.rest_offset(uncompr_f(), 5) == 64
+
+
+@pytest.mark.parametrize("f", [dynamic_vbp_f(512, 2048, 8), static_vbp_f(12, 1), static_vbp_f(5, 8),
+                               default_delta_dynamic_vbp_f(8), default_for_dynamic_vbp_f(8)],
+                         ids=repr)
+@pytest.mark.parametrize("n", [0, 7, 64, 1000, 4096 + 77])
+def test_column_morph_layout(oracle, f, n):
+    bits = f.bw if f.bw else 20
+    v = rng_vals(n + 1, n, bits)
+    col = oracle.morph(f, v)
+    bs = f.block_size()
+    nc = n // bs * bs
+    main = oracle.morph_batch_compress(f, v[:nc]) if nc else np.zeros(0, np.uint8)
+    assert col.size_compr == len(main)
+    img = col.image()
+    assert img[:len(main)].tobytes() == main.tobytes()
+    if n - nc:
+        r = rm.align64(len(main) + oracle.rest_offset(f, 0))
+        assert col.size_used == r + 8 * (n - nc)
+        assert np.array_equal(np.frombuffer(img[r:].tobytes(), np.uint64), v[nc:])
+        assert col.count_uncompr() == n - nc
+    else:
+        assert col.size_used == col.size_compr
+    assert np.array_equal(col.values(), v)
+
+
+# ------------------------------------------------ write-iterator A.7
+@pytest.mark.parametrize("f", [dynamic_vbp_f(512, 2048, 8), default_dynamic_vbp_f(8),
+                               default_delta_dynamic_vbp_f(8), default_for_dynamic_vbp_f(8),
+                               static_vbp_f(20, 8)], ids=repr)
+@pytest.mark.parametrize("m", [0, 100, 2048, 5000, 6144, 6144 + 1024])
+def test_write_iterator_chunking(oracle, f, m):
+    v = np.sort(rng_vals(m + 5, m, 20))
+    col, app = oracle.wit_emit(f, v)
+    K = 2048
+    bs = f.block_size()
+    parts = [v[i:i + K] for i in range(0, m // K * K, K)]
+    rem = m % K
+    nc = rem // bs * bs
+    if nc:
+        parts.append(v[m // K * K:m // K * K + nc])
+    if f.kind == 3:   # delta carries x_{i-step} across flushes: one stream
+        main = oracle.morph_batch_compress(f, v[:m // K * K + nc]) if parts else np.zeros(0, np.uint8)
+    else:
+        main = np.concatenate([oracle.morph_batch_compress(f, p) for p in parts]) if parts else np.zeros(0, np.uint8)
+    assert col.size_compr == len(main)
+    img = col.image()
+    assert img[:len(main)].tobytes() == main.tobytes()
+    r = rm.align64(len(main) + oracle.rest_offset(f, 0))
+    if rem - nc:
+        assert col.size_used == r + 8 * (rem - nc) == app
+    else:
+        assert col.size_used == len(main) and app == r
+    assert np.array_equal(col.values(), v)
+
+
+# ------------------------------------------------------------ operators
+CMPS = [MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE]
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("in_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8),
+                                  default_for_dynamic_vbp_f(8)], ids=repr)
+@pytest.mark.parametrize("out_f", [uncompr_f(), default_delta_dynamic_vbp_f(8), default_dynamic_vbp_f(8)],
+                         ids=repr)
+def test_select_brute_force(oracle, lanes, in_f, out_f):
+    n = 3 * 4096 + 13
+    v = rng_vals(17, n, 12)
+    col = oracle.morph(in_f, v)
+    for cmp in CMPS:
+        out = oracle.select(cmp, 2048, out_f, col, lanes)
+        expect = rm.select_positions(v, cmp, 2048)
+        assert np.array_equal(out.values(), expect)
+        assert oracle.select_sum(cmp, 2048, col) == rm.masked_sum(v, cmp, 2048)
+        assert oracle.select_sum_stream(cmp, 2048, col, 3) == rm.masked_sum(v, cmp, 2048)
+
+
+def test_select_scalar_remainder_layout(oracle):
+    """select_compr.h:209-250: with 8 lanes, positions from the input rest's
+    last (rest mod 8) values go raw behind the write-iterator's output."""
+    f = dynamic_vbp_f(512, 2048, 8)
+    n = 512 * 5 + 21                  # rest of 21 values: 16 vectorised, 5 scalar
+    v = rng_vals(3, n, 12)
+    col = oracle.morph(f, v)
+    out_f = default_delta_dynamic_vbp_f(8)
+    out = oracle.select(MS_CMP_LT, 4096, out_f, col, 8)         # everything matches
+    wit, app = oracle.wit_emit(out_f, np.arange(n - 5, dtype=np.uint64))
+    img = out.image()
+    assert out.size_compr == wit.size_compr
+    assert img[:wit.size_used].tobytes() == wit.image().tobytes()
+    assert np.array_equal(np.frombuffer(img[app:].tobytes(), np.uint64), np.arange(n - 5, n, dtype=np.uint64))
+    assert out.size_used == app + 40
+    # no match among the scalar remainder: sizeUsed ends where the rest would start
+    v2 = v.copy()
+    v2[-5:] = 4095
+    out2 = oracle.select(MS_CMP_LT, 4095, out_f, oracle.morph(f, v2), 8)
+    pos2 = rm.select_positions(v2, MS_CMP_LT, 4095)
+    wit2, app2 = oracle.wit_emit(out_f, pos2)
+    assert out2.size_used == app2 and np.array_equal(out2.values(), pos2)
+
+
+@pytest.mark.parametrize("data_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8)], ids=repr)
+@pytest.mark.parametrize("pos_f", [uncompr_f(), default_delta_dynamic_vbp_f(8)], ids=repr)
+def test_project_brute_force(oracle, data_f, pos_f):
+    n = 2 * 4096 + 100
+    v = rng_vals(5, n, 12)
+    p = np.flatnonzero(v < 1500).astype(np.uint64)
+    out = oracle.project(uncompr_f(), oracle.morph(data_f, v), oracle.morph(pos_f, p), 8)
+    assert np.array_equal(out.values(), v[p])
+
+
+def test_set_operators_multiset(oracle):
+    a = np.sort(rng_vals(1, 3000, 9))
+    b = np.sort(rng_vals(2, 2500, 9))
+    for out_f in (uncompr_f(), default_delta_dynamic_vbp_f(8)):
+        i = oracle.intersect_sorted(out_f, oracle.morph(default_delta_dynamic_vbp_f(8), a), oracle.from_values(b))
+        assert np.array_equal(i.values(), rm.multiset_intersect(a, b))
+        m = oracle.merge_sorted(out_f, oracle.from_values(a), oracle.from_values(b))
+        assert np.array_equal(m.values(), rm.multiset_union(a, b))
+
+
+def test_calc_all_ops(oracle):
+    a = rng_vals(11, 1000, 64)
+    b = rng_vals(12, 1000, 64)
+    b[::7] = 0
+    for op in range(10):
+        out = oracle.calc_binary(op, uncompr_f(), oracle.from_values(a), oracle.from_values(b))
+        assert np.array_equal(out.values(), rm.calc(op, a, b)), op
+    assert oracle.calc_binary(0, uncompr_f(), oracle.from_values(a), oracle.from_values(b[:10])) == 4
