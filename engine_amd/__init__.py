@@ -1,0 +1,205 @@
+"""engine_amd -- MorphStore's compression-aware execution path on MI355X.
+
+A thin Python binding of the C-ABI in include/morphstore_hip.h (the HIP
+kernels live in engine_amd/csrc, built into engine_amd/libmorphstore_hip.so).
+There is no CPU fallback: importing the library fails loudly when the shared
+object is missing, and every call needs a gfx950 device.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .formats import ColMeta, FormatDesc  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmorphstore_hip.so")
+
+_lib = None
+_inited = False
+
+# every entry point the header declares: name -> (restype, argtypes)
+P, U64, U32, VP = ctypes.POINTER, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p
+_FD, _CM = P(FormatDesc), P(ColMeta)
+_COLP = P(VP)
+ABI = {
+    "ms_init": (ctypes.c_int, [ctypes.c_int]),
+    "ms_last_error": (ctypes.c_char_p, []),
+    "ms_version": (ctypes.c_char_p, []),
+    "ms_size_max_byte_any_len": (ctypes.c_int, [_FD, U64, P(U64)]),
+    "ms_size_max_byte": (ctypes.c_int, [_FD, U64, P(U64)]),
+    "ms_block_size": (ctypes.c_int, [_FD, P(U64)]),
+    "ms_col_upload": (ctypes.c_int, [_COLP, _FD, VP, _CM, VP]),
+    "ms_col_upload_values": (ctypes.c_int, [_COLP, VP, U64, VP]),
+    "ms_col_get_meta": (ctypes.c_int, [VP, _CM, VP]),
+    "ms_col_download": (ctypes.c_int, [VP, VP, U64, _CM, VP]),
+    "ms_col_format": (ctypes.c_int, [VP, _FD]),
+    "ms_col_device_data": (VP, [VP]),
+    "ms_col_count_bound": (U64, [VP]),
+    "ms_col_free": (ctypes.c_int, [VP]),
+    "ms_col_check": (ctypes.c_int, [VP, VP]),
+    "ms_morph": (ctypes.c_int, [_COLP, _FD, VP, VP]),
+    "ms_select": (ctypes.c_int, [_COLP, U32, U64, _FD, VP, U32, VP]),
+    "ms_project": (ctypes.c_int, [_COLP, _FD, VP, VP, U32, VP]),
+    "ms_agg_sum": (ctypes.c_int, [_COLP, VP, VP]),
+    "ms_select_sum": (ctypes.c_int, [VP, U32, U64, VP, VP]),
+    "ms_intersect_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
+    "ms_merge_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
+    "ms_calc_binary": (ctypes.c_int, [_COLP, U32, _FD, VP, VP, VP]),
+}
+
+
+class MsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"status {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m engine_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in ABI.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise MsError(rc, lib().ms_last_error().decode())
+
+
+def init(device=0):
+    global _inited
+    if not _inited:
+        check(lib().ms_init(device))
+        _inited = True
+
+
+class Column:
+    """A device column (ms_col*).  Freed when garbage collected."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.ms_col_free(self.h)
+            self.h = None
+
+    def meta(self, stream=None):
+        m = ColMeta()
+        check(lib().ms_col_get_meta(self.h, ctypes.byref(m), stream))
+        return m
+
+    @property
+    def fmt(self):
+        f = FormatDesc()
+        check(lib().ms_col_format(self.h, ctypes.byref(f)))
+        return f
+
+    def image(self, stream=None):
+        m = self.meta(stream)
+        buf = np.zeros(max(m.size_used, 1), np.uint8)
+        check(lib().ms_col_download(self.h, buf.ctypes.data, len(buf), None, stream))
+        return buf[:m.size_used], (m.count, m.size_used, m.size_compr)
+
+    def values(self, stream=None):
+        """Decompressed values (through ms_morph to uncompr_f)."""
+        u = morph(FormatDesc(0), self, stream)
+        img, meta = u.image(stream)
+        return np.frombuffer(img.tobytes(), np.uint64)[:meta[0]].copy()
+
+    def check(self, stream=None):
+        check(lib().ms_col_check(self.h, stream))
+
+    @property
+    def device_ptr(self):
+        return lib().ms_col_device_data(self.h)
+
+
+def _new(fn, *args):
+    h = VP()
+    check(fn(ctypes.byref(h), *args))
+    return Column(h)
+
+
+def upload_values(values, stream=None):
+    init()
+    v = np.ascontiguousarray(values, dtype=np.uint64)
+    return _new(lib().ms_col_upload_values, v.ctypes.data, len(v), stream)
+
+
+def upload_image(fmt, image, count, size_used, size_compr, stream=None):
+    init()
+    img = np.ascontiguousarray(image, dtype=np.uint8)
+    m = ColMeta(count, size_used, size_compr, max(size_used, 1))
+    return _new(lib().ms_col_upload, ctypes.byref(fmt), img.ctypes.data if len(img) else None, ctypes.byref(m),
+                stream)
+
+
+def morph(fmt, col, stream=None):
+    return _new(lib().ms_morph, ctypes.byref(fmt), col.h, stream)
+
+
+def select(cmp, val, out_fmt, col, lanes=8, stream=None):
+    return _new(lib().ms_select, cmp, val, ctypes.byref(out_fmt), col.h, lanes, stream)
+
+
+def project(out_fmt, data, pos, lanes=8, stream=None):
+    return _new(lib().ms_project, ctypes.byref(out_fmt), data.h, pos.h, lanes, stream)
+
+
+def agg_sum(col, stream=None):
+    return _new(lib().ms_agg_sum, col.h, stream)
+
+
+def intersect_sorted(out_fmt, l, r, stream=None):
+    return _new(lib().ms_intersect_sorted, ctypes.byref(out_fmt), l.h, r.h, stream)
+
+
+def merge_sorted(out_fmt, l, r, stream=None):
+    return _new(lib().ms_merge_sorted, ctypes.byref(out_fmt), l.h, r.h, stream)
+
+
+def calc_binary(op, out_fmt, l, r, stream=None):
+    return _new(lib().ms_calc_binary, op, ctypes.byref(out_fmt), l.h, r.h, stream)
+
+
+class DeviceU64:
+    """A device uint64 (for ms_select_sum results), via the HIP runtime."""
+
+    _hip = None
+
+    def __init__(self, n=1):
+        if DeviceU64._hip is None:
+            DeviceU64._hip = ctypes.CDLL("libamdhip64.so")
+        self.n = n
+        self.p = VP()
+        rc = DeviceU64._hip.hipMalloc(ctypes.byref(self.p), ctypes.c_size_t(8 * n))
+        if rc != 0:
+            raise MsError(3, f"hipMalloc failed ({rc})")
+
+    def get(self):
+        out = np.zeros(self.n, np.uint64)
+        DeviceU64._hip.hipDeviceSynchronize()
+        rc = DeviceU64._hip.hipMemcpy(ctypes.c_void_p(out.ctypes.data), self.p, ctypes.c_size_t(8 * self.n), 2)
+        if rc != 0:
+            raise MsError(3, f"hipMemcpy failed ({rc})")
+        return out
+
+    def __del__(self):
+        if self.p and DeviceU64._hip is not None:
+            DeviceU64._hip.hipFree(self.p)
+
+
+def select_sum(cmp, val, col, result=None, stream=None):
+    """Fused select + agg_sum; returns the DeviceU64 holding the sum."""
+    r = result or DeviceU64()
+    check(lib().ms_select_sum(r.p, cmp, val, col.h, stream))
+    return r

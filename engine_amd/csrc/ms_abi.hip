@@ -1,0 +1,754 @@
+// ms_abi.hip -- the C-ABI of include/morphstore_hip.h: column management
+// and operator orchestration.  Entry points never throw; errors are status
+// codes with a per-thread message (ms_last_error).
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ms_kernels.h"
+
+static thread_local std::string g_err;
+static int g_device = -1;
+
+int ms_fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+namespace ms {
+
+void *scratch::get(size_t bytes)
+{
+    if (n >= 64) return nullptr;
+    void *p = nullptr;
+    if (hipMallocAsync(&p, bytes ? bytes : 64, s) != hipSuccess) return nullptr;
+    ptrs[n++] = p;
+    return p;
+}
+
+scratch::~scratch()
+{
+    for (int i = 0; i < n; i++) (void)hipFreeAsync(ptrs[i], s);
+}
+
+int colv_of(const ms_col *c, ms_colv &v)
+{
+    if (!c) return ms_fail(MS_ERR_INVALID, "null column");
+    v.data = c->data;
+    v.meta = c->meta;
+    v.index = c->index;
+    v.refs = c->refs;
+    v.g = c->g;
+    return MS_OK;
+}
+
+}  // namespace ms
+
+using namespace ms;
+
+// ------------------------------------------------------------------ formats
+static bool valid_leaf(const ms_format_desc &f)
+{
+    switch (f.kind) {
+    case MS_UNCOMPR: return true;
+    case MS_STATIC_VBP: return f.bw >= 1 && f.bw <= 64 && f.step >= 1 && f.step <= 64;
+    case MS_DYNAMIC_VBP:
+        return f.step >= 1 && f.step <= 64 && f.block > 0 && f.page > 0 && f.block % (64u * f.step) == 0 &&
+               f.page % (8u * f.step) == 0;
+    default: return false;
+    }
+}
+
+static bool valid_format(const ms_format_desc *fp)
+{
+    if (!fp) return false;
+    const ms_format_desc &f = *fp;
+    switch (f.kind) {
+    case MS_UNCOMPR:
+    case MS_STATIC_VBP:
+    case MS_DYNAMIC_VBP: return valid_leaf(f);
+    case MS_RLE: return true;
+    case MS_DELTA:
+    case MS_FOR: {
+        ms_format_desc in = {};
+        in.kind = f.inner_kind; in.bw = f.inner_bw; in.block = f.inner_block; in.page = f.inner_page;
+        in.step = f.inner_step;
+        if (!valid_leaf(in) || f.block == 0) return false;
+        const uint64_t ibs = in.kind == MS_UNCOMPR ? 1 : in.kind == MS_STATIC_VBP ? 64ull * in.step : in.block;
+        if (f.block % ibs) return false;
+        if (f.kind == MS_DELTA)
+            return f.step >= 1 && f.step <= 64 && f.block % f.step == 0 &&
+                   (in.kind == MS_UNCOMPR || in.step == f.step) && (64 % f.step == 0);
+        const uint32_t E = f.step ? f.step : (in.kind != MS_UNCOMPR ? in.step : 1);
+        return f.page > 0 && f.page % E == 0 && f.block % E == 0 && (in.kind == MS_UNCOMPR || in.step == E) &&
+               (64 % E == 0);
+    }
+    default: return false;
+    }
+}
+
+static uint64_t size_max(const ms_format_desc &f, uint64_t n)
+{
+    const ms_geom g = ms_make_geom(f);
+    switch (f.kind) {
+    case MS_UNCOMPR: return 8 * n;
+    case MS_STATIC_VBP: return n * f.bw / 8;
+    case MS_DYNAMIC_VBP: return 8 * n + (n / f.block) * f.page;
+    case MS_DELTA: return ms_leaf_size_max(f.inner_kind, f.inner_bw, f.inner_block, f.inner_page, n);
+    case MS_FOR: {
+        const uint64_t pages = (n + (uint64_t)f.page * f.block - 1) / ((uint64_t)f.page * f.block);
+        return pages * 8ull * f.page +
+               (n / f.block) * ms_leaf_size_max(f.inner_kind, f.inner_bw, f.inner_block, f.inner_page, f.block);
+    }
+    case MS_RLE: return 16 * n;
+    default: (void)g; return 0;
+    }
+}
+
+static uint64_t size_max_any_len(const ms_format_desc &f, uint64_t n)
+{
+    if (f.kind == MS_UNCOMPR) return 8 * n;
+    if (f.kind == MS_RLE) return 16 * n;
+    const uint64_t bs = ms_make_geom(f).bs;
+    return size_max(f, (n + bs - 1) / bs * bs + bs) + 8 * (n % bs + bs) + 63;
+}
+
+extern "C" int ms_size_max_byte_any_len(const ms_format_desc *f, uint64_t count, uint64_t *out)
+{
+    if (!valid_format(f) || !out) return ms_fail(MS_ERR_INVALID, "invalid format");
+    *out = size_max_any_len(*f, count);
+    return MS_OK;
+}
+
+extern "C" int ms_size_max_byte(const ms_format_desc *f, uint64_t count, uint64_t *out)
+{
+    if (!valid_format(f) || !out) return ms_fail(MS_ERR_INVALID, "invalid format");
+    *out = size_max(*f, count);
+    return MS_OK;
+}
+
+extern "C" int ms_block_size(const ms_format_desc *f, uint64_t *out)
+{
+    if (!valid_format(f) || !out || f->kind == MS_RLE) return ms_fail(MS_ERR_INVALID, "format has no block size");
+    *out = ms_make_geom(*f).bs;
+    return MS_OK;
+}
+
+extern "C" const char *ms_last_error(void) { return g_err.c_str(); }
+extern "C" const char *ms_version(void) { return "morphstore-hip 0.1 (gfx950)"; }
+
+extern "C" int ms_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0)
+        return ms_fail(MS_ERR_NODEVICE, "no HIP device %d", device);
+    hipDeviceProp_t prop;
+    MS_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return ms_fail(MS_ERR_NODEVICE, "device %d is %s, not gfx950", device, prop.gcnArchName);
+    MS_HIP(hipSetDevice(device));
+    hipMemPool_t pool;
+    MS_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    uint64_t thr = ~0ull;
+    MS_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    g_device = device;
+    return MS_OK;
+}
+
+static int need_init()
+{
+    if (g_device < 0) return ms_fail(MS_ERR_NODEVICE, "ms_init was not called");
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ columns
+static uint64_t idx_entries(const ms_geom &g, uint64_t count)
+{
+    if (g.kind == MS_DYNAMIC_VBP || g.kind == MS_DELTA || g.kind == MS_FOR) return count / g.inner_bs + 2;
+    return 0;
+}
+
+// One allocation: [meta 256 B][index][refs][image]
+static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_alloc, hipStream_t s, ms_col **out)
+{
+    ms_col *c = new (std::nothrow) ms_col();
+    if (!c) return ms_fail(MS_ERR_NOMEM, "host allocation");
+    c->f = f;
+    c->g = ms_make_geom(f);
+    c->count_bound = count_bound;
+    c->stream = s;
+    c->index_cap = idx_entries(c->g, count_bound);
+    c->refs_cap = f.kind == MS_FOR ? count_bound / f.block + 2 : 0;
+    c->size_alloc = size_alloc;
+    const uint64_t idx_bytes = (c->index_cap * 8 + 255) / 256 * 256;
+    const uint64_t refs_bytes = (c->refs_cap * 8 + 255) / 256 * 256;
+    const uint64_t data_bytes = (size_alloc + 1024 + 255) / 256 * 256;
+    uint8_t *base = nullptr;
+    if (hipMallocAsync((void **)&base, 256 + idx_bytes + refs_bytes + data_bytes, s) != hipSuccess) {
+        delete c;
+        return ms_fail(MS_ERR_NOMEM, "device allocation of %llu bytes", (unsigned long long)(size_alloc + 1024));
+    }
+    c->meta = (ms_dev_meta *)base;
+    c->index = c->index_cap ? (uint64_t *)(base + 256) : nullptr;
+    c->refs = c->refs_cap ? (uint64_t *)(base + 256 + idx_bytes) : nullptr;
+    c->data = base + 256 + idx_bytes + refs_bytes;
+    int rc = launch_fill_u64((uint64_t *)base, 256 / 8, 0, s);
+    if (rc) {
+        (void)hipFreeAsync(base, s);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return MS_OK;
+}
+
+static uint64_t out_capacity(const ms_format_desc &f, uint64_t count_bound)
+{
+    // pessimistic size plus room for raw appends behind the rest
+    return size_max_any_len(f, count_bound) + 8 * 64 + 256;
+}
+
+static hipStream_t S(void *stream) { return (hipStream_t)stream; }
+
+// Builds the block index of an image on the host (the producer-side kernels
+// build it on the device).
+static int host_index(const ms_format_desc &f, const ms_geom &g, const uint8_t *img, const ms_col_meta &m,
+                      std::vector<uint64_t> &idx, std::vector<uint64_t> &refs)
+{
+    const ms_parts pt = ms_parts_of(g, m.count, m.size_used, m.size_compr);
+    const uint64_t end = m.size_compr;
+    uint64_t pos = 0;
+    auto leaf_call = [&](uint64_t n) -> int {
+        // one morph_batch call of the leaf layout over n values at pos
+        if (g.leaf == MS_UNCOMPR) {
+            if (pos + 8 * n > end) return ms_fail(MS_ERR_INVALID, "image too short");
+            for (uint64_t i = 0; i < n; i += g.inner_bs) { idx.push_back((pos << 8) | 64); pos += 8 * g.inner_bs; }
+            return MS_OK;
+        }
+        if (g.leaf == MS_STATIC_VBP) {
+            for (uint64_t i = 0; i < n; i += g.inner_bs) {
+                idx.push_back((pos << 8) | g.bw);
+                pos += g.inner_bs * g.bw / 8;
+            }
+            return pos <= end ? MS_OK : ms_fail(MS_ERR_INVALID, "image too short");
+        }
+        uint64_t done = 0;
+        while (done < n) {
+            if (pos + g.leaf_page > end) return ms_fail(MS_ERR_INVALID, "dynamic_vbp page header past sizeCompr");
+            const uint8_t *meta = img + pos;
+            pos += g.leaf_page;
+            for (uint64_t q = 0; q < g.leaf_page; q++) {
+                const uint32_t bw = meta[q];
+                if (bw == MS_VBP_BW_NOBLOCK) continue;
+                if (bw < 1 || bw > 64) return ms_fail(MS_ERR_INVALID, "bit width %u in a page header", bw);
+                idx.push_back((pos << 8) | bw);
+                pos += g.inner_bs * bw / 8;
+                done += g.inner_bs;
+            }
+            if (pos > end) return ms_fail(MS_ERR_INVALID, "dynamic_vbp block past sizeCompr");
+        }
+        return MS_OK;
+    };
+    switch (f.kind) {
+    case MS_DYNAMIC_VBP: return leaf_call(pt.n_compr);
+    case MS_DELTA:
+        for (uint64_t b = 0; b < pt.n_compr / f.block; b++) MS_TRY(leaf_call(f.block));
+        return MS_OK;
+    case MS_FOR: {
+        uint64_t done = 0;
+        while (done < pt.n_compr) {
+            if (pos + 8ull * f.page > end) return ms_fail(MS_ERR_INVALID, "for_f page past sizeCompr");
+            const uint64_t *r = (const uint64_t *)(img + pos);
+            pos += 8ull * f.page;
+            for (uint64_t q = 0; q < f.page && done < pt.n_compr; q++) {
+                if (r[q] == ~0ull) break;
+                refs.push_back(r[q]);
+                MS_TRY(leaf_call(f.block));
+                done += f.block;
+            }
+        }
+        return MS_OK;
+    }
+    default: return MS_OK;
+    }
+}
+
+static int check_meta(const ms_format_desc &f, const ms_col_meta &m)
+{
+    const ms_geom g = ms_make_geom(f);
+    if (m.size_compr > m.size_used) return ms_fail(MS_ERR_INVALID, "sizeCompr > sizeUsed");
+    if (f.kind == MS_UNCOMPR) {
+        if (m.size_used != 8 * m.count) return ms_fail(MS_ERR_INVALID, "uncompr_f: sizeUsed != 8*count");
+        return MS_OK;
+    }
+    if (f.kind == MS_RLE) {
+        if (m.size_used % 16 || m.size_used != m.size_compr) return ms_fail(MS_ERR_INVALID, "rle_f: bad sizes");
+        return MS_OK;
+    }
+    if (m.size_used != m.size_compr) {
+        const uint64_t r = ms_rest_offset(g, m.size_compr);
+        if (m.size_used < r || (m.size_used - r) % 8) return ms_fail(MS_ERR_INVALID, "bad rest placement");
+    }
+    const ms_parts pt = ms_parts_of(g, m.count, m.size_used, m.size_compr);
+    if (pt.n_rest > m.count) return ms_fail(MS_ERR_INVALID, "rest larger than count");
+    if (pt.n_compr % g.bs) return ms_fail(MS_ERR_INVALID, "compressed part not a multiple of the block size");
+    if (f.kind == MS_STATIC_VBP && m.size_compr != pt.n_compr * f.bw / 8)
+        return ms_fail(MS_ERR_INVALID, "static_vbp: sizeCompr does not match count");
+    return MS_OK;
+}
+
+extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *image, const ms_col_meta *meta,
+                             void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !meta || (!image && meta->size_used)) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (!valid_format(f)) return ms_fail(MS_ERR_INVALID, "invalid format");
+    MS_TRY(check_meta(*f, *meta));
+    const ms_geom g = ms_make_geom(*f);
+    std::vector<uint64_t> idx, refs;
+    MS_TRY(host_index(*f, g, (const uint8_t *)image, *meta, idx, refs));
+    ms_col *c;
+    uint64_t alloc = meta->size_used;
+    if (f->kind != MS_UNCOMPR && f->kind != MS_RLE) alloc = std::max(alloc, size_max_any_len(*f, meta->count));
+    MS_TRY(new_col(*f, meta->count, alloc, S(stream), &c));
+    if (idx.size() > c->index_cap || refs.size() > c->refs_cap) {
+        ms_col_free(c);
+        return ms_fail(MS_ERR_CAPACITY, "index larger than expected");
+    }
+    ms_dev_meta dm = {};
+    dm.count = meta->count;
+    dm.size_used = meta->size_used;
+    dm.size_compr = meta->size_compr;
+    if (meta->size_used) MS_HIP(hipMemcpyAsync(c->data, image, meta->size_used, hipMemcpyHostToDevice, S(stream)));
+    if (!idx.empty()) MS_HIP(hipMemcpyAsync(c->index, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, S(stream)));
+    if (!refs.empty()) MS_HIP(hipMemcpyAsync(c->refs, refs.data(), refs.size() * 8, hipMemcpyHostToDevice, S(stream)));
+    MS_HIP(hipMemcpyAsync(c->meta, &dm, sizeof dm, hipMemcpyHostToDevice, S(stream)));
+    MS_HIP(hipStreamSynchronize(S(stream)));
+    *out = c;
+    return MS_OK;
+}
+
+extern "C" int ms_col_upload_values(ms_col **out, const uint64_t *values, uint64_t count, void *stream)
+{
+    ms_format_desc f = {};
+    f.kind = MS_UNCOMPR;
+    ms_col_meta m = {count, 8 * count, 0, 8 * count};
+    return ms_col_upload(out, &f, values, &m, stream);
+}
+
+extern "C" int ms_col_get_meta(const ms_col *c, ms_col_meta *meta, void *stream)
+{
+    if (!c || !meta) return ms_fail(MS_ERR_INVALID, "null argument");
+    ms_dev_meta dm;
+    MS_HIP(hipMemcpyAsync(&dm, c->meta, sizeof dm, hipMemcpyDeviceToHost, S(stream)));
+    MS_HIP(hipStreamSynchronize(S(stream)));
+    meta->count = dm.count;
+    meta->size_used = dm.size_used;
+    meta->size_compr = dm.size_compr;
+    meta->size_alloc = c->size_alloc;
+    return MS_OK;
+}
+
+extern "C" int ms_col_check(const ms_col *c, void *stream)
+{
+    if (!c) return ms_fail(MS_ERR_INVALID, "null column");
+    ms_dev_meta dm;
+    MS_HIP(hipMemcpyAsync(&dm, c->meta, sizeof dm, hipMemcpyDeviceToHost, S(stream)));
+    MS_HIP(hipStreamSynchronize(S(stream)));
+    if (dm.status) return ms_fail((int)dm.status, "a kernel producing this column reported status %llu",
+                                  (unsigned long long)dm.status);
+    return MS_OK;
+}
+
+extern "C" int ms_col_download(const ms_col *c, void *image, uint64_t capacity, ms_col_meta *meta, void *stream)
+{
+    ms_col_meta m;
+    MS_TRY(ms_col_get_meta(c, &m, stream));
+    if (meta) *meta = m;
+    if (!image) return MS_OK;
+    if (capacity < m.size_used) return ms_fail(MS_ERR_CAPACITY, "download buffer of %llu < sizeUsed %llu",
+                                               (unsigned long long)capacity, (unsigned long long)m.size_used);
+    if (m.size_used == 0) return MS_OK;
+    const ms_parts pt = ms_parts_of(c->g, m.count, m.size_used, m.size_compr);
+    if (c->f.kind == MS_UNCOMPR || c->f.kind == MS_RLE || m.size_used == m.size_compr) {
+        MS_HIP(hipMemcpyAsync(image, c->data, m.size_used, hipMemcpyDeviceToHost, S(stream)));
+    } else {
+        // the gap between the compressed part and the rest reads as zeros
+        uint8_t *h = (uint8_t *)image;
+        memset(h + m.size_compr, 0, pt.rest_off - m.size_compr);
+        if (m.size_compr) MS_HIP(hipMemcpyAsync(h, c->data, m.size_compr, hipMemcpyDeviceToHost, S(stream)));
+        MS_HIP(hipMemcpyAsync(h + pt.rest_off, c->data + pt.rest_off, m.size_used - pt.rest_off,
+                              hipMemcpyDeviceToHost, S(stream)));
+    }
+    MS_HIP(hipStreamSynchronize(S(stream)));
+    return MS_OK;
+}
+
+extern "C" int ms_col_format(const ms_col *c, ms_format_desc *f)
+{
+    if (!c || !f) return ms_fail(MS_ERR_INVALID, "null argument");
+    *f = c->f;
+    return MS_OK;
+}
+
+extern "C" const void *ms_col_device_data(const ms_col *c) { return c ? c->data : nullptr; }
+extern "C" uint64_t ms_col_count_bound(const ms_col *c) { return c ? c->count_bound : 0; }
+
+extern "C" int ms_col_free(ms_col *c)
+{
+    if (!c) return MS_OK;
+    (void)hipFreeAsync((void *)c->meta, c->stream);
+    delete c;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ helpers
+// Uncompressed values of `in` (its own image for uncompr_f, else a
+// decompressed temporary) and a device pointer to their count.
+static int values_of(const ms_col *in, scratch &sc, const uint64_t **vals, const uint64_t **d_n)
+{
+    *d_n = &in->meta->count;
+    if (in->f.kind == MS_UNCOMPR) {
+        *vals = (const uint64_t *)in->data;
+        return MS_OK;
+    }
+    uint64_t *tmp = (uint64_t *)sc.get(in->count_bound * 8 + 64);
+    if (!tmp) return ms_fail(MS_ERR_NOMEM, "decompression scratch");
+    MS_TRY(launch_decompress(in, tmp, sc.s));
+    *vals = tmp;
+    return MS_OK;
+}
+
+static uint64_t wit_chunk(const ms_format_desc &f)
+{
+    const uint64_t bs = ms_make_geom(f).bs;
+    return (bs + 2047) / 2048 * 2048;            // write_iterator.h:78-80
+}
+
+// The write-iterator flushes m_CountBuffer values per morph_batch call; the
+// formats it supports here have blocks that tile that buffer.
+static bool wit_ok(const ms_format_desc *f)
+{
+    if (f->kind == MS_UNCOMPR) return true;
+    if (f->kind == MS_RLE) return false;
+    const uint64_t bs = ms_make_geom(*f).bs;
+    return wit_chunk(*f) % bs == 0;
+}
+
+// uncompr_f write-iterator (uncompr.h:125-221): a copy; done() returns
+// (0, end, end).
+__global__ void k_emit_uncompr(const uint64_t *v, const uint64_t *d_m, ms_dev_meta *meta, uint64_t *out)
+{
+    const uint64_t m = *d_m;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = v[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        meta->count = m;
+        meta->size_used = 8 * m;
+        meta->size_compr = 0;
+        meta->aux[0] = 8 * m;
+    }
+}
+
+static uint32_t grid_of(uint64_t n)
+{
+    uint64_t g = (n + 1023) / 1024;
+    return (uint32_t)(g < 1 ? 1 : g > 8192 ? 8192 : g);
+}
+
+// Emits values[0 .. *d_m) of a write-iterator into a new column of format f
+// (not rle_f); uncompr_f is a plain copy.
+static int emit_new(const ms_format_desc &f, const uint64_t *vals, const uint64_t *d_m, uint64_t m_bound,
+                    uint64_t chunk, uint64_t extra_values, scratch &sc, ms_col **out, uint64_t **d_append)
+{
+    ms_col *c;
+    MS_TRY(new_col(f, m_bound + extra_values, out_capacity(f, m_bound + extra_values), sc.s, &c));
+    *d_append = &c->meta->aux[0];
+    int rc;
+    if (f.kind == MS_UNCOMPR) {
+        hipLaunchKernelGGL(k_emit_uncompr, dim3(grid_of(m_bound)), dim3(256), 0, sc.s, vals, d_m, c->meta,
+                           (uint64_t *)c->data);
+        rc = hipGetLastError() == hipSuccess ? MS_OK : ms_fail(MS_ERR_HIP, "emit uncompr");
+    } else {
+        rc = launch_emit(c, vals, d_m, m_bound, chunk, nullptr, sc);
+    }
+    if (rc) {
+        ms_col_free(c);
+        return rc;
+    }
+    *out = c;
+    return MS_OK;
+}
+
+// Device bookkeeping for the scalar remainder of select/project
+// (select_compr.h:209-250, project_compr.h:183-252): the last r = rest mod
+// lanes values of the input are processed scalar and their outputs appended
+// raw behind the write-iterator's output.
+__global__ void k_split_select(const ms_dev_meta *in, ms_geom g, uint32_t lanes, const uint64_t *pos,
+                               const uint64_t *d_cnt, uint64_t *d_na, uint64_t *d_flag)
+{
+    const ms_parts pt = ms_parts_of(g, in->count, in->size_used, in->size_compr);
+    const bool has_rest = g.kind == MS_UNCOMPR ? in->count != 0 : in->size_compr != in->size_used;
+    const uint64_t r = has_rest ? pt.n_rest % lanes : 0;
+    const uint64_t thr = pt.n - r;
+    const uint64_t cnt = *d_cnt;
+    uint64_t lo = 0, hi = cnt;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (pos[mid] < thr) lo = mid + 1; else hi = mid;
+    }
+    *d_na = lo;
+    *d_flag = r != 0;
+}
+
+__global__ void k_split_project(const ms_dev_meta *pos, ms_geom g, uint32_t lanes, uint64_t *d_na, uint64_t *d_flag)
+{
+    const ms_parts pt = ms_parts_of(g, pos->count, pos->size_used, pos->size_compr);
+    const bool has_rest = g.kind == MS_UNCOMPR ? pos->count != 0 : pos->size_compr != pos->size_used;
+    const uint64_t r = has_rest ? pt.n_rest % lanes : 0;
+    *d_na = pt.n - r;
+    *d_flag = r != 0;
+}
+
+__global__ void k_append_raw(ms_dev_meta *m, uint8_t *data, const uint64_t *v, const uint64_t *d_na,
+                             const uint64_t *d_total, const uint64_t *d_flag)
+{
+    const uint64_t na = *d_na, total = *d_total;
+    const uint64_t app = m->aux[0];
+    uint64_t *dst = (uint64_t *)(data + app);
+    for (uint64_t i = na + threadIdx.x; i < total; i += blockDim.x) dst[i - na] = v[i];
+    if (threadIdx.x == 0 && *d_flag) {
+        m->count = total;
+        m->size_used = app + 8 * (total - na);
+    }
+}
+
+static int finish_split(ms_col *c, const uint64_t *v, const uint64_t *d_na, const uint64_t *d_total,
+                        const uint64_t *d_flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_append_raw, dim3(1), dim3(64), 0, s, c->meta, c->data, v, d_na, d_total, d_flag);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+static ms_format_desc uncompr_desc()
+{
+    ms_format_desc f = {};
+    f.kind = MS_UNCOMPR;
+    return f;
+}
+
+// ------------------------------------------------------------------ morph
+extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in, void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !in) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (!valid_format(dst)) return ms_fail(MS_ERR_INVALID, "invalid destination format");
+    scratch sc(S(stream));
+    const uint64_t nb = in->count_bound;
+    if (memcmp(dst, &in->f, sizeof *dst) == 0 || (dst->kind == MS_UNCOMPR && in->f.kind == MS_UNCOMPR)) {
+        // same format: a copy of the image, meta and index
+        ms_col *c;
+        MS_TRY(new_col(in->f, nb, in->size_alloc, S(stream), &c));
+        const uint64_t bytes = (uint64_t)(c->data - (uint8_t *)c->meta) + in->size_alloc;
+        MS_HIP(hipMemcpyAsync(c->meta, in->meta, bytes, hipMemcpyDeviceToDevice, S(stream)));
+        *out = c;
+        return MS_OK;
+    }
+    const uint64_t *vals, *d_n;
+    MS_TRY(values_of(in, sc, &vals, &d_n));
+    ms_col *c;
+    if (dst->kind == MS_UNCOMPR) {
+        uint64_t *app;
+        return emit_new(*dst, vals, d_n, nb, 0, 0, sc, out, &app);
+    }
+    if (dst->kind == MS_RLE) {
+        MS_TRY(new_col(*dst, nb, 16 * nb + 64, S(stream), &c));
+        int rc = launch_rle_compress(c, vals, d_n, nb, sc);
+        if (rc) { ms_col_free(c); return rc; }
+        *out = c;
+        return MS_OK;
+    }
+    MS_TRY(new_col(*dst, nb, out_capacity(*dst, nb), S(stream), &c));
+    int rc = launch_emit(c, vals, d_n, nb, 0, nullptr, sc);
+    if (rc) { ms_col_free(c); return rc; }
+    *out = c;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ select
+extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_format_desc *out_f, const ms_col *in,
+                         uint32_t lanes, void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !in || cmp > MS_CMP_GE || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    scratch sc(S(stream));
+    const uint64_t nb = in->count_bound;
+    const ms_pred p = make_pred(cmp, val);
+    uint64_t *pos = (uint64_t *)sc.get(nb * 8 + 64);
+    uint64_t *d = (uint64_t *)sc.get(64);
+    if (!pos || !d) return ms_fail(MS_ERR_NOMEM, "select scratch");
+    uint64_t *d_cnt = d, *d_na = d + 1, *d_flag = d + 2;
+    MS_TRY(launch_select_positions(in, p, pos, d_cnt, sc));
+    hipLaunchKernelGGL(k_split_select, dim3(1), dim3(1), 0, S(stream), in->meta, in->g, lanes, pos, d_cnt, d_na,
+                       d_flag);
+    MS_HIP(hipGetLastError());
+    ms_col *c;
+    uint64_t *app;
+    MS_TRY(emit_new(*out_f, pos, d_na, nb, wit_chunk(*out_f), 64, sc, &c, &app));
+    int rc = finish_split(c, pos, d_na, d_cnt, d_flag, S(stream));
+    if (rc) { ms_col_free(c); return rc; }
+    *out = c;
+    return MS_OK;
+}
+
+extern "C" int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val, const ms_col *in, void *stream)
+{
+    MS_TRY(need_init());
+    if (!d_result || !in || cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "bad argument");
+    return launch_select_sum(in, make_pred(cmp, val), d_result, false, S(stream));
+}
+
+// ------------------------------------------------------------------ project
+extern "C" int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_col *data, const ms_col *pos,
+                          uint32_t lanes, void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !data || !pos || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    scratch sc(S(stream));
+    const uint64_t np = pos->count_bound;
+    const uint64_t *pv, *d_np;
+    MS_TRY(values_of(pos, sc, &pv, &d_np));
+    // data without random access is decompressed first
+    ms_col tmpdata;
+    const ms_col *dsrc = data;
+    if (data->f.kind == MS_DELTA || data->f.kind == MS_RLE) {
+        const uint64_t *dv, *d_dn;
+        MS_TRY(values_of(data, sc, &dv, &d_dn));
+        ms_dev_meta *dm = (ms_dev_meta *)sc.get(sizeof(ms_dev_meta));
+        if (!dm) return ms_fail(MS_ERR_NOMEM, "project scratch");
+        MS_TRY(launch_set_meta(dm, d_dn, 8, 0, S(stream)));
+        tmpdata = ms_col();
+        tmpdata.f = uncompr_desc();
+        tmpdata.g = ms_make_geom(tmpdata.f);
+        tmpdata.data = (uint8_t *)dv;
+        tmpdata.meta = dm;
+        tmpdata.count_bound = data->count_bound;
+        dsrc = &tmpdata;
+    }
+    uint64_t *v = (uint64_t *)sc.get(np * 8 + 64);
+    uint64_t *d = (uint64_t *)sc.get(64);
+    if (!v || !d) return ms_fail(MS_ERR_NOMEM, "project scratch");
+    uint64_t *d_na = d, *d_flag = d + 1, *d_status = d + 2;
+    MS_TRY(launch_fill_u64(d_status, 1, 0, S(stream)));
+    MS_TRY(launch_gather(dsrc, pv, d_np, np, v, d_status, sc));
+    hipLaunchKernelGGL(k_split_project, dim3(1), dim3(1), 0, S(stream), pos->meta, pos->g, lanes, d_na, d_flag);
+    MS_HIP(hipGetLastError());
+    ms_col *c;
+    uint64_t *app;
+    MS_TRY(emit_new(*out_f, v, d_na, np, wit_chunk(*out_f), 64, sc, &c, &app));
+    int rc = finish_split(c, v, d_na, d_np, d_flag, S(stream));
+    if (!rc) rc = launch_copy_u64(&c->meta->status, d_status, S(stream));
+    if (rc) { ms_col_free(c); return rc; }
+    *out = c;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ agg_sum
+__global__ void k_agg_meta(ms_dev_meta *m, uint64_t compr)
+{
+    m->count = 1;
+    m->size_used = 8;
+    m->size_compr = compr;
+}
+
+extern "C" int ms_agg_sum(ms_col **out, const ms_col *in, void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !in) return ms_fail(MS_ERR_INVALID, "bad argument");
+    ms_col *c;
+    MS_TRY(new_col(uncompr_desc(), 1, 64, S(stream), &c));
+    ms_pred all = make_pred(MS_CMP_GE, 0);
+    int rc = launch_select_sum(in, all, (uint64_t *)c->data, false, S(stream));
+    if (!rc) {
+        // agg_sum_compr.h:148-150 vs agg_sum_uncompr.h:126 (sizeCompr 8 vs 0)
+        hipLaunchKernelGGL(k_agg_meta, dim3(1), dim3(1), 0, S(stream), c->meta, in->f.kind == MS_UNCOMPR ? 0ull : 8ull);
+        if (hipGetLastError() != hipSuccess) rc = ms_fail(MS_ERR_HIP, "agg_sum meta");
+    }
+    if (rc) { ms_col_free(c); return rc; }
+    *out = c;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ set ops
+extern "C" int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f, const ms_col *l, const ms_col *r,
+                                   void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    scratch sc(S(stream));
+    const uint64_t *a, *b, *d_na, *d_nb;
+    MS_TRY(values_of(l, sc, &a, &d_na));
+    MS_TRY(values_of(r, sc, &b, &d_nb));
+    const uint64_t bound = l->count_bound;
+    uint64_t *o = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *d_cnt = (uint64_t *)sc.get(64);
+    if (!o || !d_cnt) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
+    MS_TRY(launch_intersect(a, d_na, bound, b, d_nb, r->count_bound, o, d_cnt, sc));
+    uint64_t *app;
+    return emit_new(*out_f, o, d_cnt, bound, wit_chunk(*out_f), 0, sc, out, &app);
+}
+
+extern "C" int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f, const ms_col *l, const ms_col *r,
+                               void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    scratch sc(S(stream));
+    const uint64_t *a, *b, *d_na, *d_nb;
+    MS_TRY(values_of(l, sc, &a, &d_na));
+    MS_TRY(values_of(r, sc, &b, &d_nb));
+    const uint64_t bound = l->count_bound + r->count_bound;
+    uint64_t *o = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *d_cnt = (uint64_t *)sc.get(64);
+    if (!o || !d_cnt) return ms_fail(MS_ERR_NOMEM, "merge scratch");
+    MS_TRY(launch_merge(a, d_na, l->count_bound, b, d_nb, r->count_bound, o, d_cnt, sc));
+    uint64_t *app;
+    return emit_new(*out_f, o, d_cnt, bound, wit_chunk(*out_f), 0, sc, out, &app);
+}
+
+extern "C" int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f, const ms_col *l,
+                              const ms_col *r, void *stream)
+{
+    MS_TRY(need_init());
+    if (!out || !l || !r || op > MS_OP_XOR) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    ms_col_meta ml, mr;
+    MS_TRY(ms_col_get_meta(l, &ml, stream));
+    MS_TRY(ms_col_get_meta(r, &mr, stream));
+    if (ml.count != mr.count)
+        return ms_fail(MS_ERR_SIZE, "calc: inDataLCol and inDataRCol must contain the same number of data elements");
+    scratch sc(S(stream));
+    const uint64_t *a, *b, *d_na, *d_nb;
+    MS_TRY(values_of(l, sc, &a, &d_na));
+    MS_TRY(values_of(r, sc, &b, &d_nb));
+    const uint64_t bound = l->count_bound;
+    uint64_t *o = (uint64_t *)sc.get(bound * 8 + 64);
+    if (!o) return ms_fail(MS_ERR_NOMEM, "calc scratch");
+    MS_TRY(launch_calc(op, a, b, d_na, bound, o, S(stream)));
+    uint64_t *app;
+    return emit_new(*out_f, o, d_na, bound, wit_chunk(*out_f), 0, sc, out, &app);
+}

@@ -1,0 +1,774 @@
+// ms_decode.hip -- read side of the compression-aware path on gfx950:
+//   * fused decompress + select + agg_sum (decompress_and_process_batch with
+//     the select core feeding the agg_sum core, format.h:164-195,
+//     select_compr.h:80-89, agg_sum_compr.h:57-71) for every format,
+//   * decompression (morph_t<ve, uncompr_f, F>, morph.h:203-241),
+//   * select's position list (select_compr.h:100-260) as count + compact.
+//
+// Mapping: a wave decodes a 512-value unit.  For a vertical-bit-packed
+// layout with S lanes, thread t handles lane j = t % S and the 8 codes
+// k = 8g .. 8g+7 with g = unit*(64/S) + t/S (values S*k + j).
+#include <hipcub/hipcub.hpp>
+
+#include <type_traits>
+
+#include "ms_kernels.h"
+
+namespace ms {
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+static constexpr int WG = 256;
+static constexpr int WAVES = WG / MS_WAVE;
+
+static inline uint32_t grid_for(uint64_t items, uint64_t per_wg, uint32_t cap = 8192)
+{
+    uint64_t g = (items + per_wg - 1) / per_wg;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+__device__ __forceinline__ uint32_t wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ ms_parts parts_of(const ms_colv &c)
+{
+    return ms_parts_of(c.g, c.meta->count, c.meta->size_used, c.meta->size_compr);
+}
+
+// Adds the workgroup's per-thread values to *res with one atomic.
+__device__ __forceinline__ void wg_atomic_add(uint64_t acc, uint64_t *res)
+{
+    __shared__ uint64_t part[WAVES];
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) s += part[w];
+        if (s) atomicAdd((unsigned long long *)res, (unsigned long long)s);
+    }
+}
+
+__global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s)
+{
+    if (n == 0) return MS_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, WG * 4ull, 2048)), dim3(WG), 0, s, p, n, v);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------
+// Random read of value i of the compressed part (static_vbp, or any
+// indexed layout; for_f adds the block's reference value).
+// ------------------------------------------------------------------
+__device__ __forceinline__ uint64_t read_compr(const ms_colv &c, uint64_t i)
+{
+    if (c.g.kind == MS_STATIC_VBP) {
+        const uint32_t S = c.g.lanes;
+        return vbp_read((const uint64_t *)c.data, S, (uint32_t)(i % S), i / S, c.g.bw);
+    }
+    const uint64_t b = i / c.g.inner_bs;
+    const uint64_t r = i - b * c.g.inner_bs;
+    const uint64_t e = c.index[b];
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    const uint32_t S = c.g.lanes;
+    uint64_t v = vbp_read((const uint64_t *)(c.data + (e >> 8)), S, (uint32_t)(r % S), r / S, bw);
+    if (c.g.kind == MS_FOR) v += c.refs[b / (c.g.outer_block / c.g.inner_bs)];
+    return v;
+}
+
+// ------------------------------------------------------------------
+// Fused select + sum over 8 codes of one thread.
+// ------------------------------------------------------------------
+template <int BW>
+__device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
+                                        uint64_t ref, uint64_t &acc)
+{
+    uint64_t w[vbp_traits<BW>::NW];
+    uint32_t sh;
+    vbp_load<BW>(base, S, j, g, w, sh);
+    if constexpr (BW <= 32) {
+        using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+        uint32_t c[8];
+        vbp_cut32<BW>(w, sh, c);
+        A s = 0;
+        uint32_t cnt;
+        if (cp.mode == PM_ALL) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) s += c[m];
+            cnt = 8;
+        } else {
+            const uint32_t lo = (uint32_t)cp.clo, span = (uint32_t)cp.cspan;
+            const bool out = cp.mode == PM_OUT;
+            cnt = 0;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+                s += take ? c[m] : 0u;
+                cnt += take;
+            }
+        }
+        acc += (uint64_t)s + (uint64_t)cnt * ref;
+    } else {
+        uint64_t c[8];
+        vbp_cut64<BW>(w, sh, c);
+        uint64_t s = 0, cnt = 0;
+        if (cp.mode == PM_ALL) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) s += c[m];
+            cnt = 8;
+        } else {
+            const bool out = cp.mode == PM_OUT;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const bool take = ((c[m] - cp.clo) <= cp.cspan) != out;
+                s += take ? c[m] : 0ull;
+                cnt += take;
+            }
+        }
+        acc += s + cnt * ref;
+    }
+}
+
+#define MS_BW_CASES(X)                                                                                              \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
+    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)  \
+    X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58)  \
+    X(59) X(60) X(61) X(62) X(63) X(64)
+
+__device__ __forceinline__ void ss_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
+                                            const ms_cpred &cp, uint64_t ref, uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_unit<B>(base, S, j, g, cp, ref, acc); break;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+// Decodes 8 codes as 64-bit values with a run-time bit width.
+template <int BW>
+__device__ __forceinline__ void dec_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, uint64_t (&c)[8])
+{
+    uint64_t w[vbp_traits<BW>::NW];
+    uint32_t sh;
+    vbp_load<BW>(base, S, j, g, w, sh);
+    vbp_cut64<BW>(w, sh, c);
+}
+
+__device__ __forceinline__ void dec_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
+                                             uint64_t (&c)[8])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: dec_unit<B>(base, S, j, g, c); break;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default:
+#pragma unroll
+        for (int m = 0; m < 8; m++) c[m] = 0;
+        break;
+    }
+}
+
+// Indexed layouts with 8 lanes and blocks of a multiple of 512 values:
+// dynamic_vbp_f<512k, P, 8> and for_f<L, P, dynamic_vbp_f<512k, P', 8>>.
+template <bool IS_FOR>
+__global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, uint64_t *res)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t spb = c.g.inner_bs / 512;
+    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t ib_per_ob = IS_FOR ? c.g.outer_block / c.g.inner_bs : 1;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t & 7;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    uint64_t acc = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+        const uint64_t b = u / spb, sub = u - b * spb;
+        const uint64_t e = uniform64(c.index[b]);
+        const uint32_t bw = (uint32_t)(e & 0xff);
+        const uint64_t *base = (const uint64_t *)(c.data + (e >> 8));
+        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0;
+        const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
+        const uint64_t g = sub * 8 + (t >> 3);
+        if (IS_FOR && ref + maxcode < ref) {
+            // codes wrap around 2^64: evaluate on the values themselves
+#pragma unroll 1
+            for (int m = 0; m < 8; m++) {
+                const uint64_t x = ref + vbp_read(base, 8, j, 8 * g + m, bw);
+                if (pred_eval(p, x)) acc += x;
+            }
+            continue;
+        }
+        const ms_cpred cp = code_pred(p, ref, maxcode);
+        if (cp.mode == PM_NONE) continue;
+        ss_dispatch(bw, base, 8, j, g, cp, ref, acc);
+    }
+    wg_atomic_add(acc, res);
+}
+
+// static_vbp_f<vbp_l<BW, S>> with 64 % S == 0.
+template <int BW>
+__global__ __launch_bounds__(WG) void k_select_sum_static(ms_colv c, ms_pred p, uint64_t *res)
+{
+    const ms_parts pt = parts_of(c);
+    const uint32_t S = c.g.lanes;
+    const uint64_t gpu = 64 / S;
+    const uint64_t ngroups = pt.n_compr / S / 8;
+    const uint64_t nunits = (ngroups + gpu - 1) / gpu;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t % S;
+    const uint64_t maxcode = BW >= 64 ? ~0ull : (1ull << BW) - 1ull;
+    const ms_cpred cp = code_pred(p, 0, maxcode);
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    uint64_t acc = 0;
+    if (cp.mode != PM_NONE) {
+        for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+            const uint64_t g = u * gpu + t / S;
+            if (g < ngroups) ss_unit<BW>((const uint64_t *)c.data, S, j, g, cp, 0, acc);
+        }
+    }
+    wg_atomic_add(acc, res);
+}
+
+// Uncompressed values: the whole of an uncompr_f column, or the rest part.
+__global__ __launch_bounds__(WG) void k_select_sum_uncompr(ms_colv c, ms_pred p, uint64_t *res)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t *x = (const uint64_t *)(c.data + pt.rest_off);
+    const uint64_t n = pt.n_rest;
+    uint64_t acc = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * WG;
+    uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x;
+    for (; 2 * i + 1 < n; i += stride) {
+        const u64x2 v = __builtin_nontemporal_load((const u64x2 *)x + i);
+        acc += pred_eval(p, v.x) ? v.x : 0;
+        acc += pred_eval(p, v.y) ? v.y : 0;
+    }
+    if (2 * i < n && 2 * i + 1 == n) {
+        const uint64_t v = x[2 * i];
+        acc += pred_eval(p, v) ? v : 0;
+    }
+    wg_atomic_add(acc, res);
+}
+
+// Any layout readable by read_compr, one value per thread.
+__global__ __launch_bounds__(WG) void k_select_sum_generic(ms_colv c, ms_pred p, uint64_t *res)
+{
+    const ms_parts pt = parts_of(c);
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < pt.n_compr; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t x = read_compr(c, i);
+        if (pred_eval(p, x)) acc += x;
+    }
+    wg_atomic_add(acc, res);
+}
+
+// rle_f: a run of `len` copies of v contributes len * v when v passes.
+__global__ __launch_bounds__(WG) void k_select_sum_rle(ms_colv c, ms_pred p, uint64_t *res)
+{
+    const uint64_t runs = c.meta->size_used / 16;
+    const uint64_t *w = (const uint64_t *)c.data;
+    uint64_t acc = 0;
+    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * WG) {
+        const u64x2 vl = *((const u64x2 *)w + r);
+        if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
+    }
+    wg_atomic_add(acc, res);
+}
+
+// ------------------------------------------------------------------
+// delta_f: x = lane-wise running sum of the decoded deltas (delta.h:
+// 229-261), one pass with a decoupled look-back over 2048-value tiles.
+// MODE 0 writes x, MODE 1 folds select + sum.
+// ------------------------------------------------------------------
+struct tile_status {
+    uint64_t flag;       // 0: empty, 1: aggregate ready, 2: inclusive ready
+    uint64_t agg[8];
+    uint64_t incl[8];
+    uint64_t pad[7];
+};
+
+__device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(uint64_t *p, uint64_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// SRC 0: indexed 8-lane blocks (dynamic_vbp inner with 512k-value blocks)
+// SRC 1: deltas already decoded into `arr` (any inner format)
+template <int MODE, int SRC>
+__global__ __launch_bounds__(WG) void k_delta_scan(ms_colv c, const uint64_t *arr, tile_status *st, uint64_t *ticket,
+                                                   uint64_t *out, ms_pred p, uint64_t *res)
+{
+    __shared__ uint64_t s_tile;
+    __shared__ uint64_t s_wtot[WAVES][8];
+    __shared__ uint64_t s_excl[8];
+    const ms_parts pt = parts_of(c);
+    const uint32_t S = c.g.dstep;                      // delta step
+    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t ntiles = (nunits + WAVES - 1) / WAVES;
+    if (threadIdx.x == 0) s_tile = atomicAdd((unsigned long long *)ticket, 1ull);
+    __syncthreads();
+    const uint64_t tile = s_tile;
+    if (tile >= ntiles) return;
+    const uint32_t wv = wave_index();
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t % S;
+    const uint32_t gl = t / S;
+    const uint64_t u = tile * WAVES + wv;
+    uint64_t d[8];
+    if (u < nunits) {
+        if (SRC == 0) {
+            const uint64_t spb = c.g.inner_bs / 512;
+            const uint64_t b = u / spb, sub = u - b * spb;
+            const uint64_t e = uniform64(c.index[b]);
+            dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
+        } else {
+            const uint64_t *a = arr + u * 512;
+#pragma unroll
+            for (int m = 0; m < 8; m++) d[m] = a[S * (8 * gl + m) + j];
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; m++) d[m] = 0;
+    }
+    // thread-local inclusive prefix of its 8 codes
+#pragma unroll
+    for (int m = 1; m < 8; m++) d[m] += d[m - 1];
+    // exclusive scan over the threads of lane j inside the wave (stride S)
+    uint64_t tot = d[7], incl = tot;
+    for (uint32_t o = S; o < 64; o <<= 1) {
+        const uint64_t v = __shfl_up(incl, o, MS_WAVE);
+        if (t >= o) incl += v;
+    }
+    uint64_t excl = incl - tot;
+    // wave totals of each lane: held by the last group's threads
+    if (t >= 64 - S) s_wtot[wv][j] = incl;
+    __syncthreads();
+    uint64_t wpre = 0, tile_tot = 0;
+    for (uint32_t w = 0; w < WAVES; w++) {
+        const uint64_t v = s_wtot[w][j];
+        if (w < wv) wpre += v;
+        tile_tot += v;
+    }
+    // publish the aggregate, then look back for the exclusive prefix
+    if (threadIdx.x < S) {
+        st_relaxed(&st[tile].agg[threadIdx.x], tile_tot);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (tile == 0) {
+            for (uint32_t l = 0; l < S; l++) st_relaxed(&st[0].incl[l], s_wtot[0][l] + s_wtot[1][l] + s_wtot[2][l] + s_wtot[3][l]);
+            st_release(&st[0].flag, 2);
+        } else {
+            st_release(&st[tile].flag, 1);
+        }
+    }
+    if (wv == 0) {
+        uint64_t e = 0;
+        if (tile > 0) {
+            int64_t q = (int64_t)tile - 1;
+            while (q >= 0) {
+                uint64_t f;
+                do {
+                    f = ld_acquire(&st[q].flag);
+                } while (f == 0);
+                if (t < S) e += (f == 2) ? ld_relaxed(&st[q].incl[t]) : ld_relaxed(&st[q].agg[t]);
+                if (f == 2) break;
+                q--;
+            }
+            if (t < S) st_relaxed(&st[tile].incl[t], e + tile_tot);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (t == 0) st_release(&st[tile].flag, 2);
+        }
+        if (t < S) s_excl[t] = e;
+    }
+    __syncthreads();
+    const uint64_t base = s_excl[j] + wpre + excl;
+    if (u >= nunits) return;
+    if (MODE == 0) {
+        uint64_t *o = out + u * 512;
+#pragma unroll
+        for (int m = 0; m < 8; m++) o[S * (8 * gl + m) + j] = base + d[m];
+    } else {
+        uint64_t acc = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const uint64_t x = base + d[m];
+            acc += pred_eval(p, x) ? x : 0;
+        }
+        acc = wave_sum_u64(acc);
+        if (t == 0 && acc) atomicAdd((unsigned long long *)res, (unsigned long long)acc);
+    }
+}
+
+// ------------------------------------------------------------------
+// Decompression kernels.
+// ------------------------------------------------------------------
+template <bool IS_FOR>
+__global__ __launch_bounds__(WG) void k_decomp_blocks(ms_colv c, uint64_t *out)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t spb = c.g.inner_bs / 512;
+    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t ib_per_ob = IS_FOR ? c.g.outer_block / c.g.inner_bs : 1;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t & 7, gl = t >> 3;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+        const uint64_t b = u / spb, sub = u - b * spb;
+        const uint64_t e = uniform64(c.index[b]);
+        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0;
+        uint64_t d[8];
+        dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
+        uint64_t *o = out + u * 512;
+#pragma unroll
+        for (int m = 0; m < 8; m++) o[8 * (8 * gl + m) + j] = d[m] + ref;
+    }
+}
+
+template <int BW>
+__global__ __launch_bounds__(WG) void k_decomp_static(ms_colv c, uint64_t *out)
+{
+    const ms_parts pt = parts_of(c);
+    const uint32_t S = c.g.lanes;
+    const uint64_t gpu = 64 / S;
+    const uint64_t ngroups = pt.n_compr / S / 8;
+    const uint64_t nunits = (ngroups + gpu - 1) / gpu;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t % S;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+        const uint64_t g = u * gpu + t / S;
+        if (g >= ngroups) continue;
+        uint64_t d[8];
+        dec_unit<BW>((const uint64_t *)c.data, S, j, g, d);
+#pragma unroll
+        for (int m = 0; m < 8; m++) out[S * (8 * g + m) + j] = d[m];
+    }
+}
+
+// read_compr per value: the fallback for every other layout (and deltas
+// of a delta_f whose inner format is not an 8-lane indexed layout).
+__global__ __launch_bounds__(WG) void k_decomp_generic(ms_colv c, uint64_t *out)
+{
+    const ms_parts pt = parts_of(c);
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < pt.n_compr; i += (uint64_t)gridDim.x * WG)
+        out[i] = read_compr(c, i);
+}
+
+__global__ __launch_bounds__(WG) void k_copy_rest(ms_colv c, uint64_t *out)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t *x = (const uint64_t *)(c.data + pt.rest_off);
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < pt.n_rest; i += (uint64_t)gridDim.x * WG)
+        out[pt.n_compr + i] = x[i];
+}
+
+// rle_f expansion: run starts are the exclusive prefix sums of the lengths;
+// each workgroup expands 2048 output positions.
+__global__ __launch_bounds__(WG) void k_rle_lengths(ms_colv c, uint64_t *len, uint64_t runs_bound)
+{
+    const uint64_t runs = c.meta->size_used / 16;
+    const uint64_t *w = (const uint64_t *)c.data;
+    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs_bound; r += (uint64_t)gridDim.x * WG)
+        len[r] = r < runs ? w[2 * r + 1] : 0;
+}
+
+__global__ __launch_bounds__(WG) void k_rle_expand(ms_colv c, const uint64_t *starts, uint64_t *out)
+{
+    const uint64_t runs = c.meta->size_used / 16;
+    const uint64_t n = c.meta->count;
+    const uint64_t *w = (const uint64_t *)c.data;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * 2048; t0 < n; t0 += (uint64_t)gridDim.x * 2048) {
+        for (uint64_t i = t0 + threadIdx.x; i < t0 + 2048 && i < n; i += WG) {
+            // last run with start <= i
+            uint64_t lo = 0, hi = runs;
+            while (hi - lo > 1) {
+                const uint64_t mid = (lo + hi) / 2;
+                if (starts[mid] <= i) lo = mid; else hi = mid;
+            }
+            out[i] = w[2 * lo];
+        }
+    }
+}
+
+// ------------------------------------------------------------------
+// Select positions over uncompressed values: per 512-value unit a count,
+// an exclusive scan, then compaction in index order (value i of a unit is
+// u*512 + 64m + t, so ballots in m order give ascending ranks).
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_sel_count(const uint64_t *x, const uint64_t *d_n, ms_pred p, uint64_t *cnt,
+                                                   uint64_t units_bound)
+{
+    const uint64_t n = *d_n;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < units_bound; u += nwaves) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const uint64_t i = u * 512 + 64 * m + t;
+            const bool take = i < n && pred_eval(p, x[i]);
+            c += __popcll(__ballot(take));
+        }
+        if (t == 0) cnt[u] = c;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_sel_compact(const uint64_t *x, const uint64_t *d_n, ms_pred p,
+                                                     const uint64_t *off, uint64_t *pos, uint64_t units_bound,
+                                                     uint64_t *d_cnt)
+{
+    const uint64_t n = *d_n;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    const uint64_t nunits = (n + 511) / 512;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+        uint64_t o = off[u];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const uint64_t i = u * 512 + 64 * m + t;
+            const bool take = i < n && pred_eval(p, x[i]);
+            const uint64_t bal = __ballot(take);
+            if (take) pos[o + __popcll(bal & ((1ull << t) - 1ull))] = i;
+            o += __popcll(bal);
+        }
+        if (u == nunits - 1 && t == 0) *d_cnt = o;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n == 0) *d_cnt = 0;
+    (void)units_bound;
+}
+
+// ------------------------------------------------------------------
+// Host launchers
+// ------------------------------------------------------------------
+static bool fast_blocks(const ms_colv &v)
+{
+    // 8-lane indexed layouts whose blocks hold a multiple of 512 values
+    return (v.g.kind == MS_DYNAMIC_VBP || v.g.kind == MS_FOR || v.g.kind == MS_DELTA) &&
+           v.g.leaf == MS_DYNAMIC_VBP && v.g.lanes == 8 && v.g.inner_bs % 512 == 0;
+}
+
+static bool fast_static(const ms_colv &v) { return v.g.kind == MS_STATIC_VBP && 64 % v.g.lanes == 0; }
+
+typedef void (*static_kernel_t)(ms_colv, ms_pred, uint64_t *);
+typedef void (*static_dec_t)(ms_colv, uint64_t *);
+
+static static_kernel_t static_sum_kernel(uint32_t bw)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: return k_select_sum_static<B>;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default: return nullptr;
+    }
+}
+
+static static_dec_t static_dec_kernel(uint32_t bw)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: return k_decomp_static<B>;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default: return nullptr;
+    }
+}
+
+static int delta_scan(const ms_col *in, const ms_colv &v, int mode, uint64_t *out, const ms_pred &p, uint64_t *res,
+                      hipStream_t s)
+{
+    scratch sc(s);
+    const uint64_t nb = in->count_bound;
+    const uint64_t ntiles = (nb / 512 + WAVES - 1) / WAVES + 1;
+    tile_status *st = (tile_status *)sc.get(ntiles * sizeof(tile_status) + 64);
+    uint64_t *ticket = (uint64_t *)sc.get(64);
+    if (!st || !ticket) return ms_fail(MS_ERR_NOMEM, "delta scan scratch");
+    MS_TRY(launch_fill_u64((uint64_t *)st, ntiles * sizeof(tile_status) / 8, 0, s));
+    MS_TRY(launch_fill_u64(ticket, 1, 0, s));
+    const uint64_t *arr = nullptr;
+    int src = 0;
+    if (!fast_blocks(v) || v.g.dstep != 8) {
+        // decode the deltas per value first
+        uint64_t *tmp = (uint64_t *)sc.get(nb * 8 + 64);
+        if (!tmp) return ms_fail(MS_ERR_NOMEM, "delta scratch");
+        hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, tmp);
+        arr = tmp;
+        src = 1;
+    }
+    const dim3 grid((uint32_t)ntiles);
+    if (mode == 0) {
+        if (src == 0) hipLaunchKernelGGL((k_delta_scan<0, 0>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
+        else hipLaunchKernelGGL((k_delta_scan<0, 1>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
+    } else {
+        if (src == 0) hipLaunchKernelGGL((k_delta_scan<1, 0>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
+        else hipLaunchKernelGGL((k_delta_scan<1, 1>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
+    }
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s)
+{
+    ms_colv v;
+    MS_TRY(colv_of(in, v));
+    if (!accumulate) MS_TRY(launch_fill_u64(d_result, 1, 0, s));
+    const uint64_t nb = in->count_bound;
+    if (nb == 0) return MS_OK;
+    const uint32_t grid_units = grid_for(nb / 512 + 1, WAVES * 4ull, 4096);
+    switch (v.g.kind) {
+    case MS_UNCOMPR:
+        hipLaunchKernelGGL(k_select_sum_uncompr, dim3(grid_for(nb, WG * 16ull, 4096)), dim3(WG), 0, s, v, p, d_result);
+        MS_HIP(hipGetLastError());
+        return MS_OK;
+    case MS_RLE:
+        hipLaunchKernelGGL(k_select_sum_rle, dim3(grid_for(nb, WG * 8ull, 4096)), dim3(WG), 0, s, v, p, d_result);
+        MS_HIP(hipGetLastError());
+        return MS_OK;
+    case MS_STATIC_VBP:
+        if (fast_static(v)) hipLaunchKernelGGL(static_sum_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
+        else hipLaunchKernelGGL(k_select_sum_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, p, d_result);
+        break;
+    case MS_DYNAMIC_VBP:
+    case MS_FOR:
+        if (fast_blocks(v)) {
+            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
+            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
+        } else {
+            hipLaunchKernelGGL(k_select_sum_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, p, d_result);
+        }
+        break;
+    case MS_DELTA:
+        MS_TRY(delta_scan(in, v, 1, nullptr, p, d_result, s));
+        break;
+    default:
+        return ms_fail(MS_ERR_INVALID, "select_sum: unsupported format");
+    }
+    MS_HIP(hipGetLastError());
+    // the uncompressed rest part
+    hipLaunchKernelGGL(k_select_sum_uncompr, dim3(grid_for(in->g.bs + 1, WG * 2ull, 64)), dim3(WG), 0, s, v, p, d_result);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
+{
+    ms_colv v;
+    MS_TRY(colv_of(in, v));
+    const uint64_t nb = in->count_bound;
+    if (nb == 0) return MS_OK;
+    const uint32_t grid_units = grid_for(nb / 512 + 1, WAVES * 4ull, 8192);
+    switch (v.g.kind) {
+    case MS_UNCOMPR:
+        break;
+    case MS_RLE: {
+        scratch sc(s);
+        const uint64_t runs_bound = in->size_alloc / 16 + 1;
+        uint64_t *len = (uint64_t *)sc.get(runs_bound * 8);
+        uint64_t *st = (uint64_t *)sc.get(runs_bound * 8);
+        if (!len || !st) return ms_fail(MS_ERR_NOMEM, "rle scratch");
+        hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(runs_bound, WG * 4ull)), dim3(WG), 0, s, v, len, runs_bound);
+        MS_TRY(launch_exclusive_scan(len, st, runs_bound, sc));
+        hipLaunchKernelGGL(k_rle_expand, dim3(grid_for(nb, 2048, 8192)), dim3(WG), 0, s, v, st, d_out);
+        MS_HIP(hipGetLastError());
+        return MS_OK;
+    }
+    case MS_STATIC_VBP:
+        if (fast_static(v)) hipLaunchKernelGGL(static_dec_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, d_out);
+        else hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
+        break;
+    case MS_DYNAMIC_VBP:
+    case MS_FOR:
+        if (fast_blocks(v)) {
+            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_decomp_blocks<true>, dim3(grid_units), dim3(WG), 0, s, v, d_out);
+            else hipLaunchKernelGGL(k_decomp_blocks<false>, dim3(grid_units), dim3(WG), 0, s, v, d_out);
+        } else {
+            hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
+        }
+        break;
+    case MS_DELTA:
+        MS_TRY(delta_scan(in, v, 0, d_out, ms_pred{}, nullptr, s));
+        break;
+    default:
+        return ms_fail(MS_ERR_INVALID, "decompress: unsupported format");
+    }
+    MS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_copy_rest, dim3(grid_for(nb, WG * 4ull, 4096)), dim3(WG), 0, s, v, d_out);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+__global__ void k_count_of(const ms_dev_meta *m, uint64_t *dst) { *dst = m->count; }
+
+int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc)
+{
+    hipStream_t s = sc.s;
+    const uint64_t nb = in->count_bound;
+    const uint64_t *x;
+    if (in->g.kind == MS_UNCOMPR) {
+        x = (const uint64_t *)in->data;
+    } else {
+        uint64_t *tmp = (uint64_t *)sc.get(nb * 8 + 64);
+        if (!tmp) return ms_fail(MS_ERR_NOMEM, "select scratch");
+        MS_TRY(launch_decompress(in, tmp, s));
+        x = tmp;
+    }
+    uint64_t *d_n = (uint64_t *)sc.get(64);
+    const uint64_t units = nb / 512 + 1;
+    uint64_t *cnt = (uint64_t *)sc.get(units * 8 + 64);
+    uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
+    if (!d_n || !cnt || !off) return ms_fail(MS_ERR_NOMEM, "select scratch");
+    hipLaunchKernelGGL(k_count_of, dim3(1), dim3(1), 0, s, in->meta, d_n);
+    hipLaunchKernelGGL(k_sel_count, dim3(grid_for(units, WAVES * 2ull)), dim3(WG), 0, s, x, d_n, p, cnt, units);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(cnt, off, units, sc));
+    hipLaunchKernelGGL(k_sel_compact, dim3(grid_for(units, WAVES * 2ull)), dim3(WG), 0, s, x, d_n, p, off, d_pos,
+                       units, d_cnt);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc)
+{
+    if (n == 0) return MS_OK;
+    size_t bytes = 0;
+    MS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d_in, d_out, (int)n, sc.s));
+    void *tmp = sc.get(bytes + 64);
+    if (!tmp) return ms_fail(MS_ERR_NOMEM, "scan scratch");
+    MS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, d_in, d_out, (int)n, sc.s));
+    return MS_OK;
+}
+
+}  // namespace ms

@@ -1,0 +1,402 @@
+// ms_encode.hip -- write side of the compression-aware path on gfx950.
+//
+// One emitter produces every compressed format the operators write:
+//   * column-level morph (morph.h:157-197): one morph_batch call over the
+//     compressed part, then the uncompressed rest behind the gap;
+//   * write-iterator output (write_iterator.h:66-196, delta.h:369-490):
+//     independent morph_batch calls of m_CountBuffer = round_up(bs, 2048)
+//     values, each starting fresh pages, then done()'s remainder and rest.
+// The innermost vertical-bit-packed blocks are the unit of work: a kernel
+// finds each block's bit width (dynamic_vbp.h:122-158) and the bytes of
+// page headers in front of it (dvbp meta bytes, for_f reference pages), a
+// scan places the blocks, and a pack kernel assembles each block's
+// interleaved words in LDS and writes payload, headers and the block index.
+#include "ms_kernels.h"
+
+namespace ms {
+
+static constexpr int WG = 256;
+
+static inline uint32_t grid_for(uint64_t items, uint64_t per_wg, uint32_t cap = 8192)
+{
+    uint64_t g = (items + per_wg - 1) / per_wg;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+struct emit_layout {
+    uint32_t kind;       // MS_STATIC_VBP, MS_DYNAMIC_VBP, MS_DELTA, MS_FOR
+    uint32_t leaf;       // innermost layout
+    uint32_t bw;         // static leaf bit width (64 for an uncompressed leaf)
+    uint32_t S;          // innermost lanes
+    uint32_t dstep;      // delta step
+    uint32_t E;          // for_f lanes (reference-value minimum)
+    uint64_t B;          // innermost block (values)
+    uint64_t P;          // dvbp leaf page (blocks)
+    uint64_t L;          // outer block (delta/for) or B
+    uint64_t FP;         // for_f page (blocks)
+    uint64_t bpo;        // innermost blocks per outer block
+    uint64_t K;          // values per morph_batch call (0: one call)
+    uint64_t bs;         // outer m_BlockSize
+    uint64_t gap;        // rest gap
+};
+
+struct emit_plan {
+    uint64_t m;          // values
+    uint64_t m_c;        // values in the compressed part
+    uint64_t nb;         // innermost blocks
+    uint64_t nfb;        // outer blocks (for_f)
+};
+
+__device__ __forceinline__ emit_plan plan_of(const emit_layout &l, uint64_t m)
+{
+    emit_plan p;
+    p.m = m;
+    uint64_t mc;
+    if (l.K == 0) mc = m / l.bs * l.bs;
+    else {
+        const uint64_t full = m / l.K, rem = m - full * l.K;
+        mc = full * l.K + rem / l.bs * l.bs;
+    }
+    p.m_c = mc;
+    p.nb = mc / l.B;
+    p.nfb = mc / l.L;
+    return p;
+}
+
+// Bytes of headers in front of innermost block b.
+__device__ __forceinline__ uint64_t hdr_of(const emit_layout &l, uint64_t b)
+{
+    uint64_t h = 0;
+    if (l.leaf == MS_DYNAMIC_VBP) {
+        uint64_t q;
+        if (l.kind == MS_DYNAMIC_VBP) q = l.K ? b % (l.K / l.B) : b;
+        else q = b % l.bpo;                       // each outer block is one inner call
+        if (q % l.P == 0) h += l.P;
+    }
+    if (l.kind == MS_FOR && b % l.bpo == 0) {
+        const uint64_t fb = b / l.bpo;
+        const uint64_t fq = l.K ? fb % (l.K / l.L) : fb;
+        if (fq % l.FP == 0) h += 8 * l.FP;
+    }
+    return h;
+}
+
+__device__ __forceinline__ uint64_t xform(const emit_layout &l, const uint64_t *v, uint64_t gi, uint64_t ref)
+{
+    const uint64_t x = v[gi];
+    if (l.kind == MS_DELTA) return x - (gi >= l.dstep ? v[gi - l.dstep] : 0ull);
+    if (l.kind == MS_FOR) return x - ref;
+    return x;
+}
+
+// for.h:121-161 reference value of one outer block, one wave per block.
+__global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *v, const uint64_t *d_m, uint64_t *refs)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint32_t t = threadIdx.x;
+    for (uint64_t fb = blockIdx.x; fb < pl.nfb; fb += gridDim.x) {
+        const uint64_t *x = v + fb * l.L;
+        uint64_t r;
+        if (l.E == 1) {
+            r = ~0ull;
+            for (uint64_t i = t; i < l.L; i += 64) r = x[i] < r ? x[i] : r;
+            for (int o = 32; o >= 1; o >>= 1) {
+                const uint64_t y = __shfl_xor(r, o, 64);
+                r = y < r ? y : r;
+            }
+        } else {
+            // lane j = i % E: signed minimum per lane, then unsigned over lanes
+            int64_t sm = INT64_MAX;
+            for (uint64_t i = t; i < l.L; i += 64) sm = (int64_t)x[i] < sm ? (int64_t)x[i] : sm;
+            for (uint32_t o = 32; o >= l.E; o >>= 1) {
+                const int64_t y = (int64_t)__shfl_xor((uint64_t)sm, o, 64);
+                sm = y < sm ? y : sm;
+            }
+            r = (uint64_t)sm;
+            for (uint32_t o = l.E >> 1; o >= 1; o >>= 1) {
+                const uint64_t y = __shfl_xor(r, o, 64);
+                r = y < r ? y : r;
+            }
+        }
+        if (t == 0) refs[fb] = r == ~0ull ? r - 1 : r;
+    }
+}
+
+// Bit width and header+payload bytes of every innermost block.
+__global__ __launch_bounds__(64) void k_block_sizes(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                    const uint64_t *refs, uint64_t nb_bound, uint64_t *sz,
+                                                    uint32_t *bws)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint32_t t = threadIdx.x;
+    for (uint64_t b = blockIdx.x; b < nb_bound; b += gridDim.x) {
+        if (b >= pl.nb) {
+            if (t == 0) { sz[b] = 0; bws[b] = 0; }
+            continue;
+        }
+        uint32_t bw = l.bw;
+        if (l.leaf == MS_DYNAMIC_VBP) {
+            const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
+            uint64_t o = 1;
+            for (uint64_t i = t; i < l.B; i += 64) o |= xform(l, v, b * l.B + i, ref);
+            o = wave_or_u64(o);
+            bw = 64 - __clzll(o);
+        }
+        if (t == 0) {
+            sz[b] = hdr_of(l, b) + l.B * bw / 8;
+            bws[b] = bw;
+        }
+    }
+}
+
+// Packs one innermost block per wave.  Dynamic LDS: B 64-bit words.
+__global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                    const uint64_t *refs, const uint64_t *start, const uint32_t *bws,
+                                                    uint8_t *out, uint64_t *index, uint64_t *refs_out)
+{
+    extern __shared__ unsigned long long lds[];
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint32_t t = threadIdx.x;
+    for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
+        const uint32_t bw = bws[b];
+        const uint64_t h = hdr_of(l, b);
+        const uint64_t off = start[b] + h;
+        const uint64_t fb = b / l.bpo;
+        const uint64_t ref = l.kind == MS_FOR ? refs[fb] : 0;
+        const uint64_t words = l.B * bw / 64;
+        for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
+        __syncthreads();
+        // OR every code in, unmasked (vbp_routine_gen.py:101, 111)
+        for (uint64_t i = t; i < l.B; i += 64) {
+            const uint64_t c = xform(l, v, b * l.B + i, ref);
+            const uint64_t j = i % l.S, k = i / l.S;
+            const uint64_t bit = k * bw;
+            const uint64_t w = bit >> 6;
+            const uint32_t sh = (uint32_t)(bit & 63);
+            atomicOr(&lds[w * l.S + j], (unsigned long long)(c << sh));
+            if (sh + bw > 64) atomicOr(&lds[(w + 1) * l.S + j], (unsigned long long)(c >> (64 - sh)));
+        }
+        __syncthreads();
+        uint64_t *dst = (uint64_t *)(out + off);
+        for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
+        __syncthreads();
+        if (t == 0) index[b] = (off << 8) | bw;
+        // dynamic_vbp page meta (dynamic_vbp.h:216-255)
+        if (l.leaf == MS_DYNAMIC_VBP) {
+            uint64_t q, call_end;
+            if (l.kind == MS_DYNAMIC_VBP) {
+                if (l.K) {
+                    const uint64_t bpc = l.K / l.B;
+                    q = b % bpc;
+                    call_end = (b / bpc + 1) * bpc;
+                    if (call_end > pl.nb) call_end = pl.nb;
+                } else {
+                    q = b;
+                    call_end = pl.nb;
+                }
+            } else {
+                q = b % l.bpo;
+                call_end = (b / l.bpo + 1) * l.bpo;
+            }
+            const uint64_t bpf = b - q % l.P;                       // first block of the page
+            const uint64_t meta = start[bpf] + hdr_of(l, bpf) - l.P; // meta bytes end the header
+            if (t == 0) out[meta + q % l.P] = (uint8_t)bw;
+            if (b == bpf) {
+                const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
+                for (uint64_t x = npage + t; x < l.P; x += 64) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
+            }
+        }
+        // for_f reference page (for.h:207-233)
+        if (l.kind == MS_FOR && b % l.bpo == 0) {
+            if (t == 0) refs_out[fb] = ref;
+            const uint64_t fpc = l.K ? l.K / l.L : 0;
+            const uint64_t fq = l.K ? fb % fpc : fb;
+            if (fq % l.FP == 0) {
+                uint64_t fcall_end = l.K ? (fb / fpc + 1) * fpc : pl.nfb;
+                if (fcall_end > pl.nfb) fcall_end = pl.nfb;
+                const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
+                uint64_t *rp = (uint64_t *)(out + start[b]);
+                for (uint64_t x = t; x < l.FP; x += 64)
+                    rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
+            }
+        }
+    }
+}
+
+// Static bit width: every output word straight from its codes.
+__global__ __launch_bounds__(WG) void k_pack_static(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                    uint8_t *out)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint64_t bw = l.bw, S = l.S;
+    const uint64_t nwords = pl.m_c * bw / 64;
+    uint64_t *o = (uint64_t *)out;
+    for (uint64_t W = (uint64_t)blockIdx.x * WG + threadIdx.x; W < nwords; W += (uint64_t)gridDim.x * WG) {
+        const uint64_t j = W % S, w = W / S;
+        const uint64_t b0 = 64 * w, b1 = 64 * w + 64;
+        uint64_t k = (b0 + bw - 1) / bw;          // first code starting in this word
+        uint64_t word = 0;
+        if (k > 0) {
+            const uint64_t kc = k - 1;            // code crossing in from the previous word
+            if (kc * bw + bw > b0) word |= v[kc * S + j] >> (b0 - kc * bw);
+        }
+        for (; k * bw < b1; k++) word |= v[k * S + j] << (k * bw - b0);
+        o[W] = word;
+    }
+}
+
+__global__ void k_emit_finish(emit_layout l, const uint64_t *d_m, const uint64_t *start, const uint32_t *bws,
+                              ms_dev_meta *meta, uint64_t *d_append)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    uint64_t C;
+    if (l.kind == MS_STATIC_VBP) C = pl.m_c * l.bw / 8;
+    else if (pl.nb == 0) C = 0;
+    else C = start[pl.nb - 1] + hdr_of(l, pl.nb - 1) + l.B * bws[pl.nb - 1] / 8;
+    const uint64_t R = ms_align64(C + l.gap);
+    const uint64_t rest = pl.m - pl.m_c;
+    meta->count = pl.m;
+    meta->size_compr = C;
+    meta->size_used = rest ? R + 8 * rest : C;
+    meta->aux[0] = rest ? R + 8 * rest : R;
+    meta->aux[1] = R;
+    if (d_append) *d_append = meta->aux[0];
+}
+
+__global__ __launch_bounds__(WG) void k_emit_rest(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                  const ms_dev_meta *meta, uint8_t *out)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint64_t rest = pl.m - pl.m_c;
+    uint64_t *dst = (uint64_t *)(out + meta->aux[1]);
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < rest; i += (uint64_t)gridDim.x * WG)
+        dst[i] = v[pl.m_c + i];
+}
+
+int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
+                uint64_t *d_append, scratch &sc)
+{
+    hipStream_t s = sc.s;
+    const ms_format_desc &f = out->f;
+    const ms_geom &g = out->g;
+    emit_layout l = {};
+    l.kind = f.kind;
+    l.leaf = g.leaf;
+    l.bw = g.bw;
+    l.S = g.lanes;
+    l.dstep = g.dstep;
+    l.B = g.inner_bs;
+    l.P = g.leaf_page;
+    l.bs = g.bs;
+    l.gap = g.gap;
+    l.K = chunk;
+    if (f.kind == MS_DELTA || f.kind == MS_FOR) {
+        l.L = g.outer_block;
+        l.FP = g.outer_page;
+        l.bpo = g.outer_block / g.inner_bs;
+        l.E = f.kind == MS_FOR ? (f.step ? f.step : g.lanes) : 1;
+    } else {
+        l.L = l.B;
+        l.bpo = 1;
+    }
+    if (f.kind != MS_STATIC_VBP && l.B * 8 > 65536)
+        return ms_fail(MS_ERR_INVALID, "emit: innermost blocks above 8192 values are not supported");
+    const uint64_t nb_bound = m_bound / l.B + 1;
+    if (f.kind == MS_STATIC_VBP) {
+        hipLaunchKernelGGL(k_pack_static, dim3(grid_for(m_bound * l.bw / 64 + 1, WG * 4ull)), dim3(WG), 0, s, l, d_v,
+                           d_m, out->data);
+        MS_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)nullptr,
+                           (const uint32_t *)nullptr, out->meta, d_append);
+    } else {
+        uint64_t *sz = (uint64_t *)sc.get(nb_bound * 8 + 64);
+        uint64_t *start = (uint64_t *)sc.get(nb_bound * 8 + 64);
+        uint32_t *bws = (uint32_t *)sc.get(nb_bound * 4 + 64);
+        uint64_t *refs = nullptr;
+        if (!sz || !start || !bws) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+        if (f.kind == MS_FOR) {
+            refs = (uint64_t *)sc.get((m_bound / l.L + 1) * 8 + 64);
+            if (!refs) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+            hipLaunchKernelGGL(k_for_refs, dim3(grid_for(m_bound / l.L + 1, 1, 65536)), dim3(64), 0, s, l, d_v, d_m,
+                               refs);
+            MS_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(nb_bound, 1, 65536)), dim3(64), 0, s, l, d_v, d_m,
+                           (const uint64_t *)refs, nb_bound, sz, bws);
+        MS_HIP(hipGetLastError());
+        MS_TRY(launch_exclusive_scan(sz, start, nb_bound, sc));
+        if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
+            return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
+        hipLaunchKernelGGL(k_pack_blocks, dim3(grid_for(nb_bound, 1, 65536)), dim3(64), l.B * 8, s, l, d_v, d_m,
+                           (const uint64_t *)refs, (const uint64_t *)start, (const uint32_t *)bws, out->data,
+                           out->index, out->refs);
+        MS_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)start,
+                           (const uint32_t *)bws, out->meta, d_append);
+    }
+    MS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
+                       (const ms_dev_meta *)out->meta, out->data);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------
+// rle_f (rle.h:95-147): maximal runs from run-start flags and a scan.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_rle_flags(const uint64_t *v, const uint64_t *d_m, uint64_t bound,
+                                                  uint64_t *flag)
+{
+    const uint64_t m = *d_m;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < bound; i += (uint64_t)gridDim.x * WG)
+        flag[i] = i < m && (i == 0 || v[i] != v[i - 1]) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(WG) void k_rle_starts(const uint64_t *v, const uint64_t *d_m, const uint64_t *flag,
+                                                   const uint64_t *rid, uint64_t *out_w, uint64_t *starts)
+{
+    const uint64_t m = *d_m;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < m; i += (uint64_t)gridDim.x * WG) {
+        if (flag[i]) {
+            const uint64_t r = rid[i];
+            out_w[2 * r] = v[i];
+            starts[r] = i;
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_rle_lens(const uint64_t *d_m, const uint64_t *flag, const uint64_t *rid,
+                                                 const uint64_t *starts, uint64_t *out_w, ms_dev_meta *meta)
+{
+    const uint64_t m = *d_m;
+    const uint64_t runs = m ? rid[m - 1] + flag[m - 1] : 0;
+    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * WG)
+        out_w[2 * r + 1] = (r + 1 < runs ? starts[r + 1] : m) - starts[r];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        meta->count = m;
+        meta->size_used = meta->size_compr = 16 * runs;
+    }
+}
+
+int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, scratch &sc)
+{
+    hipStream_t s = sc.s;
+    const uint64_t bound = m_bound + 1;
+    uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *rid = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *starts = (uint64_t *)sc.get(bound * 8 + 64);
+    if (!flag || !rid || !starts) return ms_fail(MS_ERR_NOMEM, "rle scratch");
+    hipLaunchKernelGGL(k_rle_flags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_v, d_m, bound, flag);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(flag, rid, bound, sc));
+    hipLaunchKernelGGL(k_rle_starts, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_v, d_m, flag, rid,
+                       (uint64_t *)out->data, starts);
+    hipLaunchKernelGGL(k_rle_lens, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_m, flag, rid, starts,
+                       (uint64_t *)out->data, out->meta);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+}  // namespace ms

@@ -1,0 +1,153 @@
+// ms_internal.h -- structures shared by the host layer and the kernels.
+//
+// A device column keeps the reference's byte image (column.h:76-109) plus:
+//   * ms_dev_meta  : count / sizeUsed / sizeCompr and a status word, in
+//                    device memory, so producers can set them without a
+//                    host round trip;
+//   * a block index: one 64-bit entry per innermost vertical-bit-packed
+//                    block of the compressed part, (payload byte offset << 8)
+//                    | bit width.  It is derived data, not part of the image;
+//   * for for_f     : a copy of each FOR block's reference value.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/morphstore_hip.h"
+
+#define MS_HD __host__ __device__ __forceinline__
+
+struct ms_dev_meta {
+    uint64_t count;
+    uint64_t size_used;
+    uint64_t size_compr;
+    uint64_t status;      // MS_OK or the first error a producing kernel saw
+    uint64_t aux[4];      // scratch slots for multi-kernel producers
+};
+
+// Innermost packing geometry of a format: what the kernels decode.
+struct ms_geom {
+    uint32_t kind;        // outer kind
+    uint32_t leaf;        // kind of the innermost layout (uncompr/static/dvbp)
+    uint32_t bw;          // static_vbp bit width (leaf == MS_STATIC_VBP)
+    uint32_t lanes;       // lanes of the innermost layout
+    uint64_t bs;          // outer m_BlockSize (1 for uncompr, 0 for rle)
+    uint64_t inner_bs;    // innermost block size in values (vbp block)
+    uint64_t leaf_page;   // dynamic_vbp blocks per page (innermost)
+    uint64_t outer_block; // delta/for block size
+    uint64_t outer_page;  // for: blocks per page
+    uint64_t gap;         // 2 * F::get_size_max_byte(F::m_BlockSize)
+    uint32_t dstep;       // delta_f: step of the differences
+    uint32_t pad;
+};
+
+MS_HD uint64_t ms_align64(uint64_t x) { return (x + 63ull) & ~63ull; }
+
+// F::get_size_max_byte(n) for n a multiple of F::m_BlockSize
+// (static_vbp.h:63-65, dynamic_vbp.h:106-118, delta.h:92-102, for.h:100-112).
+MS_HD uint64_t ms_leaf_size_max(uint32_t kind, uint32_t bw, uint64_t block, uint64_t page, uint64_t n)
+{
+    if (kind == MS_UNCOMPR) return 8 * n;
+    if (kind == MS_STATIC_VBP) return n * bw / 8;
+    return 8 * n + (n / block) * page;
+}
+
+MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
+{
+    ms_geom g = {};
+    g.kind = f.kind;
+    switch (f.kind) {
+    case MS_UNCOMPR:
+        g.leaf = MS_UNCOMPR; g.lanes = 1; g.bs = 1; g.inner_bs = 1; g.gap = 0;
+        break;
+    case MS_STATIC_VBP:
+        g.leaf = MS_STATIC_VBP; g.bw = f.bw; g.lanes = f.step; g.bs = 64ull * f.step; g.inner_bs = g.bs;
+        g.gap = 2 * ms_leaf_size_max(MS_STATIC_VBP, f.bw, 0, 0, g.bs);
+        break;
+    case MS_DYNAMIC_VBP:
+        g.leaf = MS_DYNAMIC_VBP; g.lanes = f.step; g.bs = f.block; g.inner_bs = f.block; g.leaf_page = f.page;
+        g.gap = 2 * ms_leaf_size_max(MS_DYNAMIC_VBP, 0, f.block, f.page, f.block);
+        break;
+    case MS_DELTA:
+    case MS_FOR: {
+        // an uncompressed inner part is indexed as one 64-bit-wide block per
+        // outer block (vbp_l<64, S> has the uncompressed layout)
+        g.leaf = f.inner_kind; g.bw = f.inner_kind == MS_UNCOMPR ? 64 : f.inner_bw;
+        g.lanes = f.inner_kind == MS_UNCOMPR ? 1 : f.inner_step;
+        g.inner_bs = f.inner_kind == MS_UNCOMPR ? f.block
+                   : f.inner_kind == MS_STATIC_VBP ? 64ull * f.inner_step : f.inner_block;
+        g.dstep = f.kind == MS_DELTA ? f.step : 0;
+        g.leaf_page = f.inner_page;
+        g.bs = f.block; g.outer_block = f.block; g.outer_page = f.page;
+        const uint64_t inner_max = ms_leaf_size_max(f.inner_kind, f.inner_bw, f.inner_block, f.inner_page, f.block);
+        if (f.kind == MS_DELTA) g.gap = 2 * inner_max;
+        else {
+            const uint64_t pages = (f.block + (uint64_t)f.page * f.block - 1) / ((uint64_t)f.page * f.block);
+            g.gap = 2 * (pages * 8ull * f.page + inner_max);
+        }
+        break;
+    }
+    default:   // MS_RLE: no block size, no rest
+        g.leaf = MS_RLE; g.lanes = 1; g.bs = 0; g.inner_bs = 0; g.gap = 0;
+        break;
+    }
+    return g;
+}
+
+// create_data_uncompr_start (column.h:182-196) relative to the 64-byte
+// aligned base of the image.
+MS_HD uint64_t ms_rest_offset(const ms_geom &g, uint64_t compr_end)
+{
+    return ms_align64(compr_end + g.gap);
+}
+
+// get_count_values_uncompr (column.h:205-214).
+MS_HD uint64_t ms_count_rest(const ms_geom &g, uint64_t size_used, uint64_t size_compr)
+{
+    if (size_compr == size_used) return 0;
+    return (size_used - ms_rest_offset(g, size_compr)) / 8;
+}
+
+// Device-side view of a column passed to kernels by value.
+struct ms_colv {
+    const uint8_t *data;
+    const ms_dev_meta *meta;
+    const uint64_t *index;   // per innermost block: (offset << 8) | bw
+    const uint64_t *refs;    // for_f: reference value per FOR block
+    ms_geom g;
+};
+
+// Derived numbers every reader needs; computed from the device meta.
+struct ms_parts {
+    uint64_t n;        // count
+    uint64_t n_rest;   // values in the uncompressed rest
+    uint64_t n_compr;  // values in the compressed part
+    uint64_t rest_off; // byte offset of the rest
+};
+
+MS_HD ms_parts ms_parts_of(const ms_geom &g, uint64_t count, uint64_t size_used, uint64_t size_compr)
+{
+    ms_parts p;
+    p.n = count;
+    if (g.kind == MS_RLE) { p.n_rest = 0; p.n_compr = count; p.rest_off = size_used; return p; }
+    p.n_rest = ms_count_rest(g, size_used, size_compr);
+    p.n_compr = count - p.n_rest;
+    p.rest_off = g.kind == MS_UNCOMPR ? 0 : ms_rest_offset(g, size_compr);
+    if (g.kind == MS_UNCOMPR) { p.n_rest = count; p.n_compr = 0; }
+    return p;
+}
+
+// Host-side handle behind ms_col*.
+struct ms_col {
+    ms_format_desc f;
+    ms_geom g;
+    uint8_t *data;           // device image, size_alloc bytes (+ slack)
+    uint64_t size_alloc;
+    ms_dev_meta *meta;       // device
+    uint64_t *index;         // device, index_cap entries (or null)
+    uint64_t index_cap;
+    uint64_t *refs;          // device, refs_cap entries (for_f) or null
+    uint64_t refs_cap;
+    uint64_t count_bound;    // host-known upper bound of count
+    hipStream_t stream;      // stream of creation (frees are ordered on it)
+};

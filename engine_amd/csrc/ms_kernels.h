@@ -1,0 +1,79 @@
+// ms_kernels.h -- launch entry points of the kernels (host side), used by
+// the C-ABI layer.  Every launcher is stream ordered and reads the counts
+// it needs from the columns' device meta data; host-side bounds only size
+// the grids.
+#pragma once
+
+#include "ms_internal.h"
+#include "ms_device.hpp"
+
+namespace ms {
+
+struct scratch;   // stream-ordered temporaries of one operator call
+
+// Thin RAII-free arena: every temporary of a call is freed at the end of
+// the call on the call's stream (hipFreeAsync keeps the ordering).
+struct scratch {
+    hipStream_t s;
+    void *ptrs[64];
+    int n = 0;
+    explicit scratch(hipStream_t st) : s(st) {}
+    void *get(size_t bytes);
+    ~scratch();
+};
+
+int colv_of(const ms_col *c, ms_colv &v);
+
+// decode side -------------------------------------------------------
+int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+// *d_result (+)= sum of x_i with pred(x_i); the result is zeroed first
+// unless `accumulate`.
+int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s);
+// writes the column's `count` values to d_out (uncompressed)
+int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
+// ascending positions with pred into d_pos (capacity >= count bound);
+// *d_cnt receives their number
+int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc);
+// exclusive prefix sums of n u64 values (in-place allowed)
+int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc);
+
+// encode side -------------------------------------------------------
+// Emits the m values d_v[0 .. *d_m) into `out` (already allocated, format
+// out->f) as `chunk`-value morph_batch calls (chunk 0: one call for the
+// whole compressed part, i.e. a column-level morph) followed by the raw
+// rest, and sets out's meta.  *d_append receives the write-iterator's
+// append offset (done()'s second element).
+int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
+                uint64_t *d_append, scratch &sc);
+// rle_f compression of d_v[0 .. *d_m)
+int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, scratch &sc);
+
+// operators ---------------------------------------------------------
+int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_np, uint64_t np_bound,
+                  uint64_t *d_out, uint64_t *d_status, scratch &sc);
+int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
+                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc);
+int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
+                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc);
+int launch_calc(uint32_t op, const uint64_t *a, const uint64_t *b, const uint64_t *d_n, uint64_t n_bound,
+                uint64_t *d_out, hipStream_t s);
+// small device-side bookkeeping kernels
+int launch_copy_u64(uint64_t *dst, const uint64_t *src, hipStream_t s);
+int launch_set_meta(ms_dev_meta *m, const uint64_t *d_count, uint64_t size_per_value, uint64_t size_compr_is_used,
+                    hipStream_t s);
+
+}  // namespace ms
+
+int ms_fail(int code, const char *fmt, ...);
+
+#define MS_HIP(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess) return ms_fail(MS_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define MS_TRY(x)                   \
+    do {                            \
+        int r_ = (x);               \
+        if (r_ != MS_OK) return r_; \
+    } while (0)

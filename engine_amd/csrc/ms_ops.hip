@@ -1,0 +1,248 @@
+// ms_ops.hip -- project's gather, intersect_sorted, merge_sorted and
+// calc_binary on gfx950.
+//
+//   gather     : out_i = data[pos_i] with random access into the compressed
+//                part (static_vbp bit extraction, indexed dynamic_vbp /
+//                for_f blocks) or the uncompressed rest
+//                (project_compr.h:44-67, vbp_format_rndaccess.h:94-638).
+//   intersect  : multiset intersection of ascending inputs, an element of
+//                the left input survives if its rank among equal left
+//                values is below the right input's count of that value
+//                (scalar/intersect_uncompr.h:69-80 emits min multiplicity).
+//   merge      : multiset union, equal pairs once (max multiplicity,
+//                scalar/merge_uncompr.h:69-96), placed by merge-path ranks.
+//   calc       : element-wise op mod 2^64 (scalar/calc_uncompr.h:98-100).
+#include "ms_kernels.h"
+
+namespace ms {
+
+static constexpr int WG = 256;
+
+static inline uint32_t grid_for(uint64_t items, uint64_t per_wg, uint32_t cap = 8192)
+{
+    uint64_t g = (items + per_wg - 1) / per_wg;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+__device__ __forceinline__ uint64_t lower_bound(const uint64_t *a, uint64_t n, uint64_t v)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t upper_bound(const uint64_t *a, uint64_t n, uint64_t v)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint64_t gather_one(const ms_colv &c, const ms_parts &pt, uint64_t p)
+{
+    if (p >= pt.n_compr) return ((const uint64_t *)(c.data + pt.rest_off))[p - pt.n_compr];
+    if (c.g.kind == MS_STATIC_VBP) {
+        const uint32_t S = c.g.lanes;
+        return vbp_read((const uint64_t *)c.data, S, (uint32_t)(p % S), p / S, c.g.bw);
+    }
+    const uint64_t b = p / c.g.inner_bs;
+    const uint64_t r = p - b * c.g.inner_bs;
+    const uint64_t e = c.index[b];
+    const uint32_t S = c.g.lanes;
+    uint64_t v = vbp_read((const uint64_t *)(c.data + (e >> 8)), S, (uint32_t)(r % S), r / S, (uint32_t)(e & 0xff));
+    if (c.g.kind == MS_FOR) v += c.refs[b / (c.g.outer_block / c.g.inner_bs)];
+    return v;
+}
+
+__global__ __launch_bounds__(WG) void k_gather(ms_colv c, const uint64_t *pos, const uint64_t *d_np, uint64_t *out,
+                                               uint64_t *status)
+{
+    const ms_parts pt = ms_parts_of(c.g, c.meta->count, c.meta->size_used, c.meta->size_compr);
+    const uint64_t np = *d_np;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < np; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t p = pos[i];
+        if (p >= pt.n) {
+            atomicCAS((unsigned long long *)status, 0ull, (unsigned long long)MS_ERR_RANGE);
+            out[i] = 0;
+            continue;
+        }
+        out[i] = gather_one(c, pt, p);
+    }
+}
+
+int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_np, uint64_t np_bound,
+                  uint64_t *d_out, uint64_t *d_status, scratch &sc)
+{
+    ms_colv v;
+    MS_TRY(colv_of(data, v));
+    if (v.g.kind == MS_DELTA || v.g.kind == MS_RLE)
+        return ms_fail(MS_ERR_INVALID, "gather: no random access into this format");
+    hipLaunchKernelGGL(k_gather, dim3(grid_for(np_bound, WG * 4ull)), dim3(WG), 0, sc.s, v, d_pos, d_np, d_out,
+                       d_status);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ intersect
+__global__ __launch_bounds__(WG) void k_isect_flags(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                    const uint64_t *d_nb, uint64_t bound, uint64_t *flag)
+{
+    const uint64_t na = *d_na, nb = *d_nb;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < bound; i += (uint64_t)gridDim.x * WG) {
+        uint64_t f = 0;
+        if (i < na) {
+            const uint64_t v = a[i];
+            const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
+            const uint64_t lb = lower_bound(b, nb, v);
+            uint64_t cnt = 0;
+            if (lb < nb && b[lb] == v) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+            f = rank < cnt;
+        }
+        flag[i] = f;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_compact_flagged(const uint64_t *a, const uint64_t *d_na, const uint64_t *flag,
+                                                        const uint64_t *off, uint64_t *out, uint64_t *d_cnt)
+{
+    const uint64_t na = *d_na;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < na; i += (uint64_t)gridDim.x * WG)
+        if (flag[i]) out[off[i]] = a[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = na ? off[na - 1] + flag[na - 1] : 0;
+}
+
+int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
+                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc)
+{
+    (void)nb_bound;
+    const uint64_t bound = na_bound + 1;
+    uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *off = (uint64_t *)sc.get(bound * 8 + 64);
+    if (!flag || !off) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
+    hipLaunchKernelGGL(k_isect_flags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, bound,
+                       flag);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(flag, off, bound, sc));
+    hipLaunchKernelGGL(k_compact_flagged, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, flag, off,
+                       d_out, d_cnt);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ merge
+__global__ __launch_bounds__(WG) void k_merge_rflags(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                     const uint64_t *d_nb, uint64_t bound, uint64_t *flag)
+{
+    const uint64_t na = *d_na, nb = *d_nb;
+    for (uint64_t j = (uint64_t)blockIdx.x * WG + threadIdx.x; j < bound; j += (uint64_t)gridDim.x * WG) {
+        uint64_t f = 0;
+        if (j < nb) {
+            const uint64_t v = b[j];
+            const uint64_t rank = (j == 0 || b[j - 1] != v) ? 0 : j - lower_bound(b, j, v);
+            const uint64_t lb = lower_bound(a, na, v);
+            uint64_t cnt = 0;
+            if (lb < na && a[lb] == v) cnt = upper_bound(a, na, v) - lb;
+            f = rank >= cnt;
+        }
+        flag[j] = f;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_merge_place(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                    const uint64_t *d_nb, const uint64_t *flag, const uint64_t *er,
+                                                    uint64_t *out, uint64_t *d_cnt)
+{
+    const uint64_t na = *d_na, nb = *d_nb;
+    const uint64_t stride = (uint64_t)gridDim.x * WG;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < na; i += stride)
+        out[i + er[lower_bound(b, nb, a[i])]] = a[i];
+    for (uint64_t j = (uint64_t)blockIdx.x * WG + threadIdx.x; j < nb; j += stride)
+        if (flag[j]) out[er[j] + upper_bound(a, na, b[j])] = b[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = na + er[nb];
+}
+
+int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
+                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc)
+{
+    const uint64_t bound = nb_bound + 1;
+    uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
+    uint64_t *er = (uint64_t *)sc.get(bound * 8 + 64);
+    if (!flag || !er) return ms_fail(MS_ERR_NOMEM, "merge scratch");
+    hipLaunchKernelGGL(k_merge_rflags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, bound,
+                       flag);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(flag, er, bound, sc));
+    hipLaunchKernelGGL(k_merge_place, dim3(grid_for(na_bound + nb_bound + 1, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na,
+                       b, d_nb, flag, er, d_out, d_cnt);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ calc
+__device__ __forceinline__ uint64_t calc_op(uint32_t op, uint64_t x, uint64_t y)
+{
+    switch (op) {
+    case MS_OP_ADD: return x + y;
+    case MS_OP_SUB: return x - y;
+    case MS_OP_MUL: return x * y;
+    case MS_OP_DIV: return y ? x / y : 0;
+    case MS_OP_MOD: return y ? x % y : 0;
+    case MS_OP_MIN: return x < y ? x : y;
+    case MS_OP_MAX: return x > y ? x : y;
+    case MS_OP_AND: return x & y;
+    case MS_OP_OR: return x | y;
+    default: return x ^ y;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_calc(uint32_t op, const uint64_t *a, const uint64_t *b, const uint64_t *d_n,
+                                             uint64_t *out)
+{
+    const uint64_t n = *d_n;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG)
+        out[i] = calc_op(op, a[i], b[i]);
+}
+
+int launch_calc(uint32_t op, const uint64_t *a, const uint64_t *b, const uint64_t *d_n, uint64_t n_bound,
+                uint64_t *d_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_calc, dim3(grid_for(n_bound, WG * 4ull)), dim3(WG), 0, s, op, a, b, d_n, d_out);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+__global__ void k_copy_u64(uint64_t *dst, const uint64_t *src) { *dst = *src; }
+
+int launch_copy_u64(uint64_t *dst, const uint64_t *src, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_copy_u64, dim3(1), dim3(1), 0, s, dst, src);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// meta = {count, count * size_per_value, size_compr_is_used ? size_used : 0}
+__global__ void k_set_meta(ms_dev_meta *m, const uint64_t *d_count, uint64_t per, uint64_t compr_is_used)
+{
+    const uint64_t c = *d_count;
+    m->count = c;
+    m->size_used = c * per;
+    m->size_compr = compr_is_used ? c * per : 0;
+}
+
+int launch_set_meta(ms_dev_meta *m, const uint64_t *d_count, uint64_t size_per_value, uint64_t size_compr_is_used,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_meta, dim3(1), dim3(1), 0, s, m, d_count, size_per_value, size_compr_is_used);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+}  // namespace ms

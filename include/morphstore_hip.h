@@ -221,8 +221,7 @@ int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val,
  * ascending columns (min multiplicity, as scalar/intersect_uncompr.h:69-80),
  * written through the selective write-iterator into out_f. */
 int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f,
-                        const ms_col *l, const ms_col *r, uint32_t lanes,
-                        void *stream);
+                        const ms_col *l, const ms_col *r, void *stream);
 
 /* merge_sorted<ve, out_f, l_f, r_f>(l, r): multiset union of two ascending
  * columns (an equal pair is emitted once, scalar/merge_uncompr.h:69-96). */

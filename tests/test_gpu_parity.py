@@ -1,0 +1,218 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle:
+identical byte images and meta data for identical format parameters, and
+bit-exact values, positions and sums."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import engine_amd as ea
+from engine_amd import datagen
+from engine_amd.formats import (MS_CMP_EQ, MS_CMP_GE, MS_CMP_GT, MS_CMP_LE, MS_CMP_LT, MS_CMP_NE, MS_OP_ADD,
+                                default_delta_dynamic_vbp_f, default_dynamic_vbp_f, default_for_dynamic_vbp_f,
+                                delta_f, dynamic_vbp_f, for_f, rle_f, static_vbp_f, uncompr_f)
+from tests import refmodel as rm
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KAT = json.load(open(os.path.join(HERE, "golden", "reference_kat.json")))
+
+FORMATS = [
+    static_vbp_f(12, 1), static_vbp_f(7, 8), static_vbp_f(33, 8), static_vbp_f(64, 8), static_vbp_f(5, 4),
+    dynamic_vbp_f(512, 2048, 8), default_dynamic_vbp_f(8), dynamic_vbp_f(1024, 64, 8),
+    default_dynamic_vbp_f(4), default_dynamic_vbp_f(1),
+    default_delta_dynamic_vbp_f(8), delta_f(1024, 8, static_vbp_f(40, 8)), delta_f(512, 4, default_dynamic_vbp_f(4)),
+    default_for_dynamic_vbp_f(8), for_f(1024, 8, uncompr_f()), for_f(2048, 16, dynamic_vbp_f(1024, 64, 8)),
+    rle_f(),
+]
+FMT_IDS = [repr(f) for f in FORMATS]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    ea.init(0)
+
+
+def data_for(f, n, seed):
+    if f.kind == 1:
+        return datagen.uniform_bits(seed, n, f.bw)
+    if f.kind == 3:
+        return datagen.sorted_geometric(seed, n)
+    if f.kind == 4:
+        v = datagen.for_base_offset(seed, n)
+        if n > 100:
+            v[77] = np.uint64(2**63 + 11)     # signed lane minimum of the 8-lane reference value
+        return v
+    if f.kind == 5:
+        return datagen.rle_runs(seed, n, mean_len=9)
+    if n >= 512:
+        return datagen.dvbp_random_bw(seed, n // 512 * 512, lo=1, hi=40).tolist() + \
+            datagen.uniform_bits(seed + 1, n % 512, 20).tolist()
+    return datagen.uniform_bits(seed, n, 20)
+
+
+def assert_same_column(gcol, ocol):
+    img, meta = gcol.image()
+    assert meta == ocol.meta(), (meta, ocol.meta())
+    oimg = ocol.image()
+    assert img.tobytes() == oimg.tobytes()
+
+
+# ---------------------------------------------------------------- morph
+@pytest.mark.parametrize("f", FORMATS, ids=FMT_IDS)
+@pytest.mark.parametrize("n", [0, 1, 100, 512, 4096 + 333, 70000])
+def test_morph_compress_byte_identical(oracle, f, n):
+    v = np.asarray(data_for(f, n, n + 7), dtype=np.uint64)
+    g = ea.morph(f, ea.upload_values(v))
+    o = oracle.morph(f, v)
+    assert_same_column(g, o)
+    assert np.array_equal(g.values(), v)
+
+
+@pytest.mark.parametrize("f", FORMATS, ids=FMT_IDS)
+def test_upload_oracle_image_then_decompress(oracle, f):
+    n = 3 * 8192 + 77
+    v = np.asarray(data_for(f, n, 3), dtype=np.uint64)
+    o = oracle.morph(f, v)
+    g = ea.upload_image(f, o.image(), *o.meta())
+    assert np.array_equal(g.values(), v)
+    assert np.array_equal(ea.morph(f, g).values(), v)      # same-format morph copies
+
+
+# ------------------------------------------------------------ select_sum
+CMPS = [MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE]
+
+
+@pytest.mark.parametrize("f", FORMATS + [uncompr_f()], ids=FMT_IDS + ["uncompr_f"])
+def test_select_sum_all_formats(oracle, f):
+    n = 5 * 4096 + 45
+    v = np.asarray(data_for(f, n, 11), dtype=np.uint64)
+    col = ea.morph(f, ea.upload_values(v)) if f.kind else ea.upload_values(v)
+    ocol = oracle.morph(f, v)
+    for val in [int(v[n // 2]), int(v[3]) + 1, 0, 2**64 - 1]:
+        for cmp in CMPS:
+            got = int(ea.select_sum(cmp, val, col).get()[0])
+            assert got == oracle.select_sum(cmp, val, ocol) == rm.masked_sum(v, cmp, val), (cmp, val)
+    s = ea.agg_sum(col)
+    o = oracle.agg_sum(ocol)
+    assert_same_column(s, o)
+
+
+# ---------------------------------------------------------------- select
+OUT_FORMATS = [uncompr_f(), default_delta_dynamic_vbp_f(8), default_dynamic_vbp_f(8), dynamic_vbp_f(512, 2048, 8),
+               default_for_dynamic_vbp_f(8), static_vbp_f(20, 8)]
+OUT_IDS = [repr(f) for f in OUT_FORMATS]
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("out_f", OUT_FORMATS, ids=OUT_IDS)
+@pytest.mark.parametrize("in_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8),
+                                  default_delta_dynamic_vbp_f(8), default_for_dynamic_vbp_f(8), rle_f()],
+                         ids=lambda f: repr(f))
+def test_select_byte_identical(oracle, lanes, out_f, in_f):
+    n = 2 * 8192 + 517                     # rest of 517 values: 512 vectorised + 5 scalar with 8 lanes
+    v = np.asarray(data_for(in_f, n, 5) if in_f.kind else datagen.uniform_bits(5, n, 12), dtype=np.uint64)
+    col = ea.morph(in_f, ea.upload_values(v)) if in_f.kind else ea.upload_values(v)
+    ocol = oracle.morph(in_f, v)
+    for cmp, val in [(MS_CMP_LT, int(np.median(v))), (MS_CMP_EQ, int(v[-2])), (MS_CMP_GT, 2**64 - 1)]:
+        g = ea.select(cmp, val, out_f, col, lanes)
+        o = oracle.select(cmp, val, out_f, ocol, lanes)
+        assert_same_column(g, o)
+        assert np.array_equal(g.values(), rm.select_positions(v, cmp, val))
+
+
+# --------------------------------------------------------------- project
+@pytest.mark.parametrize("data_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8),
+                                    default_for_dynamic_vbp_f(8), default_delta_dynamic_vbp_f(8), rle_f()],
+                         ids=lambda f: repr(f))
+@pytest.mark.parametrize("pos_f", [uncompr_f(), default_delta_dynamic_vbp_f(8)], ids=lambda f: repr(f))
+@pytest.mark.parametrize("out_f", [uncompr_f(), default_dynamic_vbp_f(8)], ids=lambda f: repr(f))
+def test_project_byte_identical(oracle, data_f, pos_f, out_f):
+    n = 8192 * 3 + 41
+    v = np.asarray(data_for(data_f, n, 9), dtype=np.uint64)
+    p = np.flatnonzero(datagen.uniform_bits(13, n, 4) < 6).astype(np.uint64)
+    dcol = ea.morph(data_f, ea.upload_values(v)) if data_f.kind else ea.upload_values(v)
+    pcol = ea.morph(pos_f, ea.upload_values(p)) if pos_f.kind else ea.upload_values(p)
+    g = ea.project(out_f, dcol, pcol, 8)
+    o = oracle.project(out_f, oracle.morph(data_f, v), oracle.morph(pos_f, p), 8)
+    assert_same_column(g, o)
+    assert np.array_equal(g.values(), v[p])
+    g.check()
+
+
+def test_project_out_of_range_reports():
+    d = ea.upload_values(np.arange(10, dtype=np.uint64))
+    p = ea.upload_values(np.array([1, 2, 10], np.uint64))
+    g = ea.project(uncompr_f(), d, p, 8)
+    with pytest.raises(ea.MsError) as e:
+        g.check()
+    assert e.value.code == 7
+
+
+# ------------------------------------------------------- set operators
+@pytest.mark.parametrize("out_f", [uncompr_f(), default_delta_dynamic_vbp_f(8)], ids=lambda f: repr(f))
+def test_intersect_merge_calc(oracle, out_f):
+    a = np.sort(datagen.uniform_bits(1, 30000, 15))
+    b = np.sort(datagen.uniform_bits(2, 25000, 15))
+    fa = default_delta_dynamic_vbp_f(8)
+    ga, gb = ea.morph(fa, ea.upload_values(a)), ea.upload_values(b)
+    oa, ob = oracle.morph(fa, a), oracle.from_values(b)
+    assert_same_column(ea.intersect_sorted(out_f, ga, gb), oracle.intersect_sorted(out_f, oa, ob))
+    assert_same_column(ea.merge_sorted(out_f, ga, gb), oracle.merge_sorted(out_f, oa, ob))
+    assert np.array_equal(ea.intersect_sorted(out_f, ga, gb).values(), rm.multiset_intersect(a, b))
+    assert np.array_equal(ea.merge_sorted(out_f, gb, ga).values(), rm.multiset_union(a, b))
+    c = datagen.uniform_bits(3, 30000, 64)
+    gc = ea.upload_values(c)
+    for op in range(10):
+        g = ea.calc_binary(op, out_f, ga, gc)
+        o = oracle.calc_binary(op, out_f, oa, oracle.from_values(c))
+        assert_same_column(g, o)
+    with pytest.raises(ea.MsError) as e:
+        ea.calc_binary(MS_OP_ADD, out_f, ga, gb)
+    assert e.value.code == 4
+
+
+# ------------------------------------------------------------------ KATs
+def test_reference_kats_on_gpu():
+    k = KAT["select_less"]
+    out = ea.select(MS_CMP_LT, k["val"], uncompr_f(), ea.upload_values(k["in"]), 1)
+    assert out.values().tolist() == k["out"]
+    k = KAT["project"]
+    out = ea.project(uncompr_f(), ea.upload_values(k["data"]), ea.upload_values(k["pos"]), 1)
+    assert out.values().tolist() == k["out"]
+    k = KAT["agg_sum"]
+    assert ea.agg_sum(ea.upload_values(k["in"])).values().tolist() == k["out"]
+    k = KAT["agg_sum_1_to_100"]
+    assert ea.agg_sum(ea.upload_values(np.arange(1, 101))).values().tolist() == k["out"]
+    k = KAT["intersect_sorted"]
+    out = ea.intersect_sorted(uncompr_f(), ea.upload_values(k["l"]), ea.upload_values(k["r"]))
+    assert out.values().tolist() == k["out"]
+    k = KAT["merge_sorted"]
+    out = ea.merge_sorted(uncompr_f(), ea.upload_values(k["l"]), ea.upload_values(k["r"]))
+    assert out.values().tolist() == k["out"]
+    k = KAT["calc_binary_plus"]
+    out = ea.calc_binary(MS_OP_ADD, uncompr_f(), ea.upload_values(k["l"]), ea.upload_values(k["r"]))
+    assert out.values().tolist() == k["out"]
+
+
+# ------------------------------------------------ full-size properties
+def test_config_sizes_round_trip_and_sums():
+    """2^24-value columns of every config-2 layout: decode(encode(x)) == x
+    and the fused select + sum equals numpy's, at a size the oracle would
+    take long for."""
+    n = 1 << 24
+    cases = [
+        (static_vbp_f(12, 1), datagen.uniform_bits(0x4D6F7270, n, 12), 1 << 11),
+        (dynamic_vbp_f(512, 2048, 8), datagen.dvbp_random_bw(0x4D6F7271, n), 1 << 16),
+        (default_delta_dynamic_vbp_f(8), datagen.sorted_geometric(0x4D6F7271, n), 5 * n),
+        (default_for_dynamic_vbp_f(8), datagen.for_base_offset(0x4D6F7271, n), 0),
+        (rle_f(), datagen.rle_runs(0x4D6F7271, n), 1 << 19),
+    ]
+    for f, v, val in cases:
+        if f.kind == 4:
+            val = int(v.min()) + (1 << 15)
+        col = ea.morph(f, ea.upload_values(v))
+        assert np.array_equal(col.values(), v), repr(f)
+        assert int(ea.select_sum(MS_CMP_LT, val, col).get()[0]) == rm.masked_sum(v, MS_CMP_LT, val), repr(f)
