@@ -213,8 +213,10 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
 
 static uint64_t out_capacity(const ms_format_desc &f, uint64_t count_bound)
 {
-    // pessimistic size plus room for raw appends behind the rest
-    return size_max_any_len(f, count_bound) + 8 * 64 + 256;
+    // pessimistic size, plus the rest gap (for short outputs the rest sits
+    // behind a gap larger than the pessimistic size), plus room for raw
+    // appends behind the rest
+    return size_max_any_len(f, count_bound) + ms_make_geom(f).gap + 8 * 64 + 256;
 }
 
 static hipStream_t S(void *stream) { return (hipStream_t)stream; }

@@ -67,6 +67,8 @@ def geometric(seed, n, p_num, p_den):
 
 def sorted_geometric(seed, n, p_num=1, p_den=10):
     g = geometric(seed, n, p_num, p_den)
+    if n == 0:
+        return g
     g[0] = 0
     return np.cumsum(g, dtype=np.uint64)
 
@@ -78,9 +80,11 @@ def for_base_offset(seed, n, offset_bits=16):
 
 def rle_runs(seed, n, mean_len=64, value_bits=20):
     out = np.empty(n, np.uint64)
+    if n == 0:
+        return out
     pos, chunk, off = 0, max(1024, n // mean_len + 16), 0
     while pos < n:
-        lens = geometric(seed ^ 0x77, chunk, 1, mean_len)
+        lens = geometric(seed ^ 0x77 ^ (off * 0x9E3779B1), chunk, 1, mean_len)
         vals = splitmix64(seed ^ 0x78, chunk, off) >> np.uint64(64 - value_bits)
         off += chunk
         rep = np.repeat(vals, lens.astype(np.int64))

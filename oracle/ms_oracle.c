@@ -500,6 +500,20 @@ int oc_col_from_values(oc_col *out, const uint64_t *v, uint64_t n)
     return MS_OK;
 }
 
+/* A column over a copy of an existing byte image (for timing the CPU
+ * baseline on the exact bytes the GPU produced). */
+int oc_col_from_image(oc_col *out, const ms_format_desc *f, const uint8_t *image, uint64_t count,
+                      uint64_t size_used, uint64_t size_compr)
+{
+    if (!valid(f)) return MS_ERR_INVALID;
+    if (col_alloc(out, f, size_used + 64)) return MS_ERR_NOMEM;
+    if (size_used) memcpy(out->data, image, size_used);
+    out->count = count;
+    out->size_used = size_used;
+    out->size_compr = size_compr;
+    return MS_OK;
+}
+
 /* column.h:205-214 */
 uint64_t oc_count_values_uncompr(const oc_col *c)
 {
@@ -580,7 +594,8 @@ int oc_wit_emit(oc_col *out, const ms_format_desc *f, const uint64_t *v, uint64_
     }
     oc_size_max_byte_any_len(f, m, &alloc);
     oc_block_size(f, &bs);
-    if (col_alloc(out, f, alloc + extra_capacity + 2 * 64)) return MS_ERR_NOMEM;
+    oc_rest_offset(f, 0, &r);                  /* the rest may sit behind a gap larger than alloc */
+    if (col_alloc(out, f, alloc + r + extra_capacity + 2 * 64)) return MS_ERR_NOMEM;
     K = round_up(bs, 2048);                                      /* m_CountBuffer */
     full = m / K;
     for (uint64_t c = 0; c < full; c++) {

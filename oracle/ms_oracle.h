@@ -51,6 +51,8 @@ int oc_morph_batch_decompress(const ms_format_desc *f, const uint8_t *in,
 
 /* Column level (morph.h:157-241, rle.h:155-297). */
 int oc_col_from_values(oc_col *out, const uint64_t *v, uint64_t n);
+int oc_col_from_image(oc_col *out, const ms_format_desc *f, const uint8_t *image, uint64_t count,
+                      uint64_t size_used, uint64_t size_compr);
 int oc_morph_compress(oc_col *out, const ms_format_desc *dst,
                       const uint64_t *in, uint64_t n);
 int oc_morph_decompress(const oc_col *in, uint64_t *out);

@@ -47,6 +47,7 @@ def lib():
             "oc_morph_batch_decompress": (ctypes.c_int, [FD, ctypes.c_void_p, U64, ctypes.c_void_p, P(U64)]),
             "oc_col_from_values": (ctypes.c_int, [C, ctypes.c_void_p, U64]),
             "oc_morph_compress": (ctypes.c_int, [C, FD, ctypes.c_void_p, U64]),
+            "oc_col_from_image": (ctypes.c_int, [C, FD, ctypes.c_void_p, U64, U64, U64]),
             "oc_morph_decompress": (ctypes.c_int, [C, ctypes.c_void_p]),
             "oc_count_values_uncompr": (U64, [C]),
             "oc_col_free": (None, [C]),
@@ -125,6 +126,14 @@ def from_values(v):
     v = _u64(v)
     c = Column()
     _check(lib().oc_col_from_values(ctypes.byref(c.c), v.ctypes.data, len(v)), "from_values")
+    return c
+
+
+def from_image(f, image, count, size_used, size_compr):
+    image = np.ascontiguousarray(image, dtype=np.uint8)
+    c = Column()
+    _check(lib().oc_col_from_image(ctypes.byref(c.c), ctypes.byref(f), image.ctypes.data, count, size_used,
+                                   size_compr), "from_image")
     return c
 
 

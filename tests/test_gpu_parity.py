@@ -123,6 +123,17 @@ def test_select_byte_identical(oracle, lanes, out_f, in_f):
         assert np.array_equal(g.values(), rm.select_positions(v, cmp, val))
 
 
+@pytest.mark.parametrize("n", [1, 5, 13, 100])
+@pytest.mark.parametrize("out_f", OUT_FORMATS, ids=OUT_IDS)
+def test_select_tiny_inputs(oracle, n, out_f):
+    v = datagen.uniform_bits(n, n, 3)
+    for lanes in (1, 8):
+        for val in (0, 3, 8):
+            g = ea.select(MS_CMP_LT, val, out_f, ea.upload_values(v), lanes)
+            o = oracle.select(MS_CMP_LT, val, out_f, oracle.from_values(v), lanes)
+            assert_same_column(g, o)
+
+
 # --------------------------------------------------------------- project
 @pytest.mark.parametrize("data_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8),
                                     default_for_dynamic_vbp_f(8), default_delta_dynamic_vbp_f(8), rle_f()],
