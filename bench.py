@@ -35,10 +35,10 @@ def masked_sum(x, val):
         return int(np.sum(np.where(x < np.uint64(val), x, np.uint64(0)), dtype=np.uint64))
 
 
-def make_layouts(n, seed):
+def make_layouts(n, seed, with_uncompr=False):
     from engine_amd import datagen
     from engine_amd.formats import (default_delta_dynamic_vbp_f, default_for_dynamic_vbp_f, dynamic_vbp_f,
-                                    rle_f)
+                                    rle_f, uncompr_f)
     x = datagen.dvbp_random_bw(seed, n)
     yield "dynamic_vbp<512,2048,8>", dynamic_vbp_f(512, 2048, 8), x, 1 << 16
     x = datagen.sorted_geometric(seed, n)
@@ -47,6 +47,9 @@ def make_layouts(n, seed):
     yield "for<1024,8,dynamic_vbp<512,64,8>>", default_for_dynamic_vbp_f(8), x, int(x.min()) + (1 << 15)
     x = datagen.rle_runs(seed, n)
     yield "rle", rle_f(), x, 1 << 19
+    if with_uncompr:
+        # reference point: the same select + sum over uncompressed values
+        yield "uncompr", uncompr_f(), datagen.uniform_bits(seed, n, 32), 1 << 31
 
 
 def cpu_baseline(col_gpu, fmt, val, threads_all):
@@ -74,6 +77,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default=None, help="only the layouts whose name starts with this (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path shards per column/page "
@@ -87,9 +91,11 @@ def main():
     n = 1 << args.log2n
     seed = datagen.BASE_SEED + 1
     cols = []
-    for name, fmt, x, val in make_layouts(n, seed):
+    for name, fmt, x, val in make_layouts(n, seed, with_uncompr=args.only == "uncompr"):
+        if args.only and not name.startswith(args.only):
+            continue
         t0 = time.perf_counter()
-        col = ea.morph(fmt, ea.upload_values(x))
+        col = ea.morph(fmt, ea.upload_values(x)) if fmt.kind else ea.upload_values(x)
         m = col.meta()
         expect = masked_sum(x, val)
         got = int(ea.select_sum(2, val, col).get()[0])

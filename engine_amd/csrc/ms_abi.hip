@@ -187,7 +187,8 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     c->count_bound = count_bound;
     c->stream = s;
     c->index_cap = idx_entries(c->g, count_bound);
-    c->refs_cap = f.kind == MS_FOR ? count_bound / f.block + 2 : 0;
+    c->refs_cap = f.kind == MS_FOR ? count_bound / f.block + 2
+                : f.kind == MS_DELTA ? (count_bound / MS_DELTA_TILE + 2) * 8 : 0;
     c->size_alloc = size_alloc;
     const uint64_t idx_bytes = (c->index_cap * 8 + 255) / 256 * 256;
     const uint64_t refs_bytes = (c->refs_cap * 8 + 255) / 256 * 256;
@@ -334,6 +335,11 @@ extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *
     if (!idx.empty()) MS_HIP(hipMemcpyAsync(c->index, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, S(stream)));
     if (!refs.empty()) MS_HIP(hipMemcpyAsync(c->refs, refs.data(), refs.size() * 8, hipMemcpyHostToDevice, S(stream)));
     MS_HIP(hipMemcpyAsync(c->meta, &dm, sizeof dm, hipMemcpyHostToDevice, S(stream)));
+    if (f->kind == MS_DELTA) {
+        scratch sc(S(stream));
+        int rc = launch_delta_carries(c, sc);
+        if (rc) { ms_col_free(c); return rc; }
+    }
     MS_HIP(hipStreamSynchronize(S(stream)));
     *out = c;
     return MS_OK;

@@ -43,18 +43,19 @@ __device__ __forceinline__ ms_parts parts_of(const ms_colv &c)
     return ms_parts_of(c.g, c.meta->count, c.meta->size_used, c.meta->size_compr);
 }
 
-// Adds the workgroup's per-thread values to *res with one atomic.
-__device__ __forceinline__ void wg_atomic_add(uint64_t acc, uint64_t *res)
+// Writes the workgroup's sum of per-thread values to part[blockIdx.x]
+// (no same-address atomics: a finishing kernel adds the partial sums).
+__device__ __forceinline__ void wg_partial(uint64_t acc, uint64_t *part)
 {
-    __shared__ uint64_t part[WAVES];
+    __shared__ uint64_t s_part[WAVES];
     acc = wave_sum_u64(acc);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t s = 0;
 #pragma unroll
-        for (int w = 0; w < WAVES; w++) s += part[w];
-        if (s) atomicAdd((unsigned long long *)res, (unsigned long long)s);
+        for (int w = 0; w < WAVES; w++) s += s_part[w];
+        part[blockIdx.x] = s;
     }
 }
 
@@ -95,6 +96,34 @@ __device__ __forceinline__ uint64_t read_compr(const ms_colv &c, uint64_t i)
 // ------------------------------------------------------------------
 // Fused select + sum over 8 codes of one thread.
 // ------------------------------------------------------------------
+// Select + sum of the 8 codes cut from preloaded words (BW <= 32).
+template <int BW>
+__device__ __forceinline__ void ss_words(const uint64_t *w, uint32_t sh, const ms_cpred &cp, uint64_t ref,
+                                         uint64_t &acc)
+{
+    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+    uint32_t c[8];
+    vbp_cut32<BW>(w, sh, c);
+    A s = 0;
+    uint32_t cnt;
+    if (cp.mode == PM_ALL) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) s += c[m];
+        cnt = 8;
+    } else {
+        const uint32_t lo = (uint32_t)cp.clo, span = (uint32_t)cp.cspan;
+        const bool out = cp.mode == PM_OUT;
+        cnt = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+            s += take ? c[m] : 0u;
+            cnt += take;
+        }
+    }
+    acc += (uint64_t)s + (uint64_t)cnt * ref;
+}
+
 template <int BW>
 __device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
                                         uint64_t ref, uint64_t &acc)
@@ -103,27 +132,7 @@ __device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32
     uint32_t sh;
     vbp_load<BW>(base, S, j, g, w, sh);
     if constexpr (BW <= 32) {
-        using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
-        uint32_t c[8];
-        vbp_cut32<BW>(w, sh, c);
-        A s = 0;
-        uint32_t cnt;
-        if (cp.mode == PM_ALL) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) s += c[m];
-            cnt = 8;
-        } else {
-            const uint32_t lo = (uint32_t)cp.clo, span = (uint32_t)cp.cspan;
-            const bool out = cp.mode == PM_OUT;
-            cnt = 0;
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
-                s += take ? c[m] : 0u;
-                cnt += take;
-            }
-        }
-        acc += (uint64_t)s + (uint64_t)cnt * ref;
+        ss_words<BW>(w, sh, cp, ref, acc);
     } else {
         uint64_t c[8];
         vbp_cut64<BW>(w, sh, c);
@@ -163,6 +172,22 @@ __device__ __forceinline__ void ss_dispatch(uint32_t bw, const uint64_t *base, u
     }
 }
 
+#define MS_BW32_CASES(X)                                                                                            \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
+    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+
+__device__ __forceinline__ void ss_words_dispatch(uint32_t bw, const uint64_t *w, uint32_t sh, const ms_cpred &cp,
+                                                  uint64_t ref, uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_words<B>(w, sh, cp, ref, acc); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
 // Decodes 8 codes as 64-bit values with a run-time bit width.
 template <int BW>
 __device__ __forceinline__ void dec_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, uint64_t (&c)[8])
@@ -190,39 +215,109 @@ __device__ __forceinline__ void dec_dispatch(uint32_t bw, const uint64_t *base, 
 
 // Indexed layouts with 8 lanes and blocks of a multiple of 512 values:
 // dynamic_vbp_f<512k, P, 8> and for_f<L, P, dynamic_vbp_f<512k, P', 8>>.
+// One wave decodes a 512-value unit; the unit's index entry for the next
+// iteration is fetched one iteration ahead.  Per unit the predicate is
+// specialised on the block's code range: no value / every value / x <= c /
+// x in [lo, hi] / x outside [lo, hi].
+enum { SM_NONE = 0, SM_ALL = 1, SM_LE = 2, SM_IN = 3, SM_OUT = 4 };
+
+template <int BW, bool CNT>
+__device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t mode,
+                                        uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
+{
+    constexpr int NW = vbp_traits<BW>::NW;
+    const uint32_t byte0 = g * BW;
+    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
+    const uint32_t sh = (byte0 & 7) * 8;
+    uint64_t w[NW];
+    // the last word may lie past this thread's bits (and past the block):
+    // column buffers are padded, and those bits are never used
+#pragma unroll
+    for (int q = 0; q < NW; q++) w[q] = p[q * 8];
+    uint32_t c[8];
+    vbp_cut32<BW>(w, sh, c);
+    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+    A s = 0;
+    uint32_t cnt = 0;
+    if (mode == SM_ALL) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) s += c[m];
+        cnt = 8;
+    } else if (mode == SM_LE) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = c[m] <= span;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    } else {
+        const bool out = mode == SM_OUT;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    }
+    acc += (uint64_t)s;
+    if (CNT) acc += (uint64_t)cnt * ref;
+}
+
+template <bool CNT>
+__device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
+                                                 uint32_t mode, uint32_t lo, uint32_t span, uint64_t ref,
+                                                 uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_fast<B, CNT>(base, j, g, mode, lo, span, ref, acc); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
 template <bool IS_FOR>
 __global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, uint64_t *res)
 {
     const ms_parts pt = parts_of(c);
     const uint64_t spb = c.g.inner_bs / 512;
-    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t n = pt.n_compr / 512;
     const uint64_t ib_per_ob = IS_FOR ? c.g.outer_block / c.g.inner_bs : 1;
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t & 7;
-    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
+    const uint32_t j = t & 7, gl = t >> 3;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
-    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
-        const uint64_t b = u / spb, sub = u - b * spb;
-        const uint64_t e = uniform64(c.index[b]);
+    uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
+    uint64_t e_next = u < n ? uniform64(c.index[u / spb]) : 0ull;
+    for (; u < n; u += W) {
+        const uint64_t e = e_next;
+        if (u + W < n) e_next = uniform64(c.index[(u + W) / spb]);
+        const uint64_t b = u / spb;
         const uint32_t bw = (uint32_t)(e & 0xff);
         const uint64_t *base = (const uint64_t *)(c.data + (e >> 8));
-        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0;
+        const uint32_t g = (uint32_t)(u - b * spb) * 8 + gl;
+        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0ull;
         const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
-        const uint64_t g = sub * 8 + (t >> 3);
         if (IS_FOR && ref + maxcode < ref) {
             // codes wrap around 2^64: evaluate on the values themselves
 #pragma unroll 1
             for (int m = 0; m < 8; m++) {
-                const uint64_t x = ref + vbp_read(base, 8, j, 8 * g + m, bw);
+                const uint64_t x = ref + vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
                 if (pred_eval(p, x)) acc += x;
             }
             continue;
         }
         const ms_cpred cp = code_pred(p, ref, maxcode);
         if (cp.mode == PM_NONE) continue;
-        ss_dispatch(bw, base, 8, j, g, cp, ref, acc);
+        if (bw > 32) {
+            ss_dispatch(bw, base, 8, j, g, cp, ref, acc);
+            continue;
+        }
+        uint32_t mode = cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT : cp.clo == 0 ? SM_LE : SM_IN;
+        ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, (uint32_t)cp.clo, (uint32_t)cp.cspan, ref, acc);
     }
-    wg_atomic_add(acc, res);
+    wg_partial(acc, res);
 }
 
 // static_vbp_f<vbp_l<BW, S>> with 64 % S == 0.
@@ -246,7 +341,7 @@ __global__ __launch_bounds__(WG) void k_select_sum_static(ms_colv c, ms_pred p, 
             if (g < ngroups) ss_unit<BW>((const uint64_t *)c.data, S, j, g, cp, 0, acc);
         }
     }
-    wg_atomic_add(acc, res);
+    wg_partial(acc, res);
 }
 
 // Uncompressed values: the whole of an uncompr_f column, or the rest part.
@@ -267,7 +362,7 @@ __global__ __launch_bounds__(WG) void k_select_sum_uncompr(ms_colv c, ms_pred p,
         const uint64_t v = x[2 * i];
         acc += pred_eval(p, v) ? v : 0;
     }
-    wg_atomic_add(acc, res);
+    wg_partial(acc, res);
 }
 
 // Any layout readable by read_compr, one value per thread.
@@ -279,7 +374,7 @@ __global__ __launch_bounds__(WG) void k_select_sum_generic(ms_colv c, ms_pred p,
         const uint64_t x = read_compr(c, i);
         if (pred_eval(p, x)) acc += x;
     }
-    wg_atomic_add(acc, res);
+    wg_partial(acc, res);
 }
 
 // rle_f: a run of `len` copies of v contributes len * v when v passes.
@@ -292,143 +387,130 @@ __global__ __launch_bounds__(WG) void k_select_sum_rle(ms_colv c, ms_pred p, uin
         const u64x2 vl = *((const u64x2 *)w + r);
         if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
     }
-    wg_atomic_add(acc, res);
+    wg_partial(acc, res);
 }
 
 // ------------------------------------------------------------------
 // delta_f: x = lane-wise running sum of the decoded deltas (delta.h:
-// 229-261), one pass with a decoupled look-back over 2048-value tiles.
-// MODE 0 writes x, MODE 1 folds select + sum.
+// 229-261).  The column's derived index holds, per 2048-value tile, the
+// running sums of the S lanes at the tile's start (its "carries"), so every
+// tile decodes independently: a workgroup (4 waves x 512 values) scans its
+// tile locally and adds the carries.  SRC 0 decodes 8-lane indexed blocks,
+// SRC 1 reads deltas already decoded into `arr`.
 // ------------------------------------------------------------------
-struct tile_status {
-    uint64_t flag;       // 0: empty, 1: aggregate ready, 2: inclusive ready
-    uint64_t agg[8];
-    uint64_t incl[8];
-    uint64_t pad[7];
-};
+static constexpr uint64_t DELTA_TILE = MS_DELTA_TILE;
+static_assert(MS_DELTA_TILE == WAVES * 512, "a delta tile is one workgroup's 4 units");
 
-__device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p)
+template <int SRC>
+__device__ __forceinline__ void delta_load(const ms_colv &c, const uint64_t *arr, uint64_t u, uint64_t nunits,
+                                           uint32_t S, uint32_t j, uint32_t gl, uint64_t (&d)[8])
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t ld_relaxed(const uint64_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_relaxed(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(uint64_t *p, uint64_t v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// SRC 0: indexed 8-lane blocks (dynamic_vbp inner with 512k-value blocks)
-// SRC 1: deltas already decoded into `arr` (any inner format)
-template <int MODE, int SRC>
-__global__ __launch_bounds__(WG) void k_delta_scan(ms_colv c, const uint64_t *arr, tile_status *st, uint64_t *ticket,
-                                                   uint64_t *out, ms_pred p, uint64_t *res)
-{
-    __shared__ uint64_t s_tile;
-    __shared__ uint64_t s_wtot[WAVES][8];
-    __shared__ uint64_t s_excl[8];
-    const ms_parts pt = parts_of(c);
-    const uint32_t S = c.g.dstep;                      // delta step
-    const uint64_t nunits = pt.n_compr / 512;
-    const uint64_t ntiles = (nunits + WAVES - 1) / WAVES;
-    if (threadIdx.x == 0) s_tile = atomicAdd((unsigned long long *)ticket, 1ull);
-    __syncthreads();
-    const uint64_t tile = s_tile;
-    if (tile >= ntiles) return;
-    const uint32_t wv = wave_index();
-    const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t % S;
-    const uint32_t gl = t / S;
-    const uint64_t u = tile * WAVES + wv;
-    uint64_t d[8];
-    if (u < nunits) {
-        if (SRC == 0) {
-            const uint64_t spb = c.g.inner_bs / 512;
-            const uint64_t b = u / spb, sub = u - b * spb;
-            const uint64_t e = uniform64(c.index[b]);
-            dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
-        } else {
-            const uint64_t *a = arr + u * 512;
-#pragma unroll
-            for (int m = 0; m < 8; m++) d[m] = a[S * (8 * gl + m) + j];
-        }
-    } else {
+    if (u >= nunits) {
 #pragma unroll
         for (int m = 0; m < 8; m++) d[m] = 0;
+        return;
     }
-    // thread-local inclusive prefix of its 8 codes
+    if (SRC == 0) {
+        const uint64_t spb = c.g.inner_bs / 512;
+        const uint64_t b = u / spb, sub = u - b * spb;
+        const uint64_t e = uniform64(c.index[b]);
+        dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
+    } else {
+        const uint64_t *a = arr + u * 512;
+#pragma unroll
+        for (int m = 0; m < 8; m++) d[m] = a[S * (8 * gl + m) + j];
+    }
+}
+
+// Lane-wise scan of one 2048-value tile: returns, per thread, the
+// exclusive prefix of its 8 values inside the tile (d[] becomes the
+// inclusive prefix within the thread).  tile_tot receives lane j's total.
+__device__ __forceinline__ uint64_t delta_tile_scan(uint64_t (&d)[8], uint32_t S, uint32_t j, uint32_t t,
+                                                    uint32_t wv, uint64_t (*s_wtot)[8], uint64_t &tile_tot)
+{
 #pragma unroll
     for (int m = 1; m < 8; m++) d[m] += d[m - 1];
-    // exclusive scan over the threads of lane j inside the wave (stride S)
-    uint64_t tot = d[7], incl = tot;
+    const uint64_t tot = d[7];
+    uint64_t incl = tot;
     for (uint32_t o = S; o < 64; o <<= 1) {
         const uint64_t v = __shfl_up(incl, o, MS_WAVE);
         if (t >= o) incl += v;
     }
-    uint64_t excl = incl - tot;
-    // wave totals of each lane: held by the last group's threads
     if (t >= 64 - S) s_wtot[wv][j] = incl;
     __syncthreads();
-    uint64_t wpre = 0, tile_tot = 0;
+    uint64_t wpre = 0;
+    tile_tot = 0;
     for (uint32_t w = 0; w < WAVES; w++) {
         const uint64_t v = s_wtot[w][j];
         if (w < wv) wpre += v;
         tile_tot += v;
     }
-    // publish the aggregate, then look back for the exclusive prefix
-    if (threadIdx.x < S) {
-        st_relaxed(&st[tile].agg[threadIdx.x], tile_tot);
-    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        if (tile == 0) {
-            for (uint32_t l = 0; l < S; l++) st_relaxed(&st[0].incl[l], s_wtot[0][l] + s_wtot[1][l] + s_wtot[2][l] + s_wtot[3][l]);
-            st_release(&st[0].flag, 2);
-        } else {
-            st_release(&st[tile].flag, 1);
-        }
-    }
-    if (wv == 0) {
-        uint64_t e = 0;
-        if (tile > 0) {
-            int64_t q = (int64_t)tile - 1;
-            while (q >= 0) {
-                uint64_t f;
-                do {
-                    f = ld_acquire(&st[q].flag);
-                } while (f == 0);
-                if (t < S) e += (f == 2) ? ld_relaxed(&st[q].incl[t]) : ld_relaxed(&st[q].agg[t]);
-                if (f == 2) break;
-                q--;
+    return wpre + incl - tot;
+}
+
+// MODE 0 writes x, MODE 1 folds select + sum into per-workgroup partials.
+template <int MODE, int SRC>
+__global__ __launch_bounds__(WG) void k_delta_tiles(ms_colv c, const uint64_t *arr, uint64_t *out, ms_pred p,
+                                                    uint64_t *part)
+{
+    __shared__ uint64_t s_wtot[WAVES][8];
+    const ms_parts pt = parts_of(c);
+    const uint32_t S = c.g.dstep;
+    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t ntiles = (nunits + WAVES - 1) / WAVES;
+    const uint32_t wv = wave_index();
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t % S, gl = t / S;
+    uint64_t acc = 0;
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t u = tile * WAVES + wv;
+        uint64_t d[8];
+        delta_load<SRC>(c, arr, u, nunits, S, j, gl, d);
+        uint64_t tile_tot;
+        const uint64_t base = c.refs[tile * 8 + j] + delta_tile_scan(d, S, j, t, wv, s_wtot, tile_tot);
+        if (u < nunits) {
+            if (MODE == 0) {
+                uint64_t *o = out + u * 512;
+#pragma unroll
+                for (int m = 0; m < 8; m++) o[S * (8 * gl + m) + j] = base + d[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; m++) {
+                    const uint64_t x = base + d[m];
+                    acc += pred_eval(p, x) ? x : 0;
+                }
             }
-            if (t < S) st_relaxed(&st[tile].incl[t], e + tile_tot);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            if (t == 0) st_release(&st[tile].flag, 2);
         }
-        if (t < S) s_excl[t] = e;
     }
-    __syncthreads();
-    const uint64_t base = s_excl[j] + wpre + excl;
-    if (u >= nunits) return;
-    if (MODE == 0) {
-        uint64_t *o = out + u * 512;
-#pragma unroll
-        for (int m = 0; m < 8; m++) o[S * (8 * gl + m) + j] = base + d[m];
-    } else {
-        uint64_t acc = 0;
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const uint64_t x = base + d[m];
-            acc += pred_eval(p, x) ? x : 0;
-        }
-        acc = wave_sum_u64(acc);
-        if (t == 0 && acc) atomicAdd((unsigned long long *)res, (unsigned long long)acc);
+    if (MODE == 1) wg_partial(acc, part);
+}
+
+// Per-tile lane totals, lane-major: sums[j * ntb + tile].
+template <int SRC>
+__global__ __launch_bounds__(WG) void k_delta_tile_sums(ms_colv c, const uint64_t *arr, uint64_t *sums, uint64_t ntb)
+{
+    __shared__ uint64_t s_wtot[WAVES][8];
+    const ms_parts pt = parts_of(c);
+    const uint32_t S = c.g.dstep;
+    const uint64_t nunits = pt.n_compr / 512;
+    const uint32_t wv = wave_index();
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t % S, gl = t / S;
+    for (uint64_t tile = blockIdx.x; tile < ntb; tile += gridDim.x) {
+        uint64_t d[8];
+        delta_load<SRC>(c, arr, tile * WAVES + wv, nunits, S, j, gl, d);
+        uint64_t tile_tot;
+        (void)delta_tile_scan(d, S, j, t, wv, s_wtot, tile_tot);
+        if (threadIdx.x < S) sums[(uint64_t)threadIdx.x * ntb + tile] = tile_tot;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_delta_carries(const uint64_t *scan, uint64_t ntb, uint32_t S, uint64_t *refs)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < ntb * 8; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t tile = i / 8, j = i % 8;
+        refs[i] = j < S ? scan[j * ntb + tile] : 0;
     }
 }
 
@@ -607,78 +689,125 @@ static static_dec_t static_dec_kernel(uint32_t bw)
     }
 }
 
-static int delta_scan(const ms_col *in, const ms_colv &v, int mode, uint64_t *out, const ms_pred &p, uint64_t *res,
-                      hipStream_t s)
+// Sums the per-workgroup partials and the uncompressed rest (select + sum
+// over the rest part, the vectorised and scalar loops of
+// agg_sum_compr.h:110-146 folded into one) and writes the result.
+__global__ __launch_bounds__(WG) void k_sum_finish(ms_colv c, ms_pred p, const uint64_t *part, uint32_t nparts,
+                                                   int with_rest, uint64_t *result)
 {
-    scratch sc(s);
-    const uint64_t nb = in->count_bound;
-    const uint64_t ntiles = (nb / 512 + WAVES - 1) / WAVES + 1;
-    tile_status *st = (tile_status *)sc.get(ntiles * sizeof(tile_status) + 64);
-    uint64_t *ticket = (uint64_t *)sc.get(64);
-    if (!st || !ticket) return ms_fail(MS_ERR_NOMEM, "delta scan scratch");
-    MS_TRY(launch_fill_u64((uint64_t *)st, ntiles * sizeof(tile_status) / 8, 0, s));
-    MS_TRY(launch_fill_u64(ticket, 1, 0, s));
-    const uint64_t *arr = nullptr;
-    int src = 0;
-    if (!fast_blocks(v) || v.g.dstep != 8) {
-        // decode the deltas per value first
-        uint64_t *tmp = (uint64_t *)sc.get(nb * 8 + 64);
-        if (!tmp) return ms_fail(MS_ERR_NOMEM, "delta scratch");
-        hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, tmp);
-        arr = tmp;
-        src = 1;
+    uint64_t acc = 0;
+    for (uint32_t i = threadIdx.x; i < nparts; i += WG) acc += part[i];
+    if (with_rest) {
+        const ms_parts pt = parts_of(c);
+        const uint64_t *x = (const uint64_t *)(c.data + pt.rest_off);
+        for (uint64_t i = threadIdx.x; i < pt.n_rest; i += WG) acc += pred_eval(p, x[i]) ? x[i] : 0;
     }
-    const dim3 grid((uint32_t)ntiles);
-    if (mode == 0) {
-        if (src == 0) hipLaunchKernelGGL((k_delta_scan<0, 0>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
-        else hipLaunchKernelGGL((k_delta_scan<0, 1>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
-    } else {
-        if (src == 0) hipLaunchKernelGGL((k_delta_scan<1, 0>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
-        else hipLaunchKernelGGL((k_delta_scan<1, 1>), grid, dim3(WG), 0, s, v, arr, st, ticket, out, p, res);
+    __shared__ uint64_t s_part[WAVES];
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t sum = 0;
+        for (int w = 0; w < WAVES; w++) sum += s_part[w];
+        *result = sum;
     }
+}
+
+static bool delta_fast(const ms_colv &v) { return fast_blocks(v) && v.g.dstep == 8; }
+
+// Deltas of a delta_f column whose inner layout has no fast decoder,
+// decoded per value into a temporary.
+static int delta_src(const ms_col *in, const ms_colv &v, scratch &sc, const uint64_t **arr)
+{
+    *arr = nullptr;
+    if (delta_fast(v)) return MS_OK;
+    uint64_t *tmp = (uint64_t *)sc.get(in->count_bound * 8 + 64);
+    if (!tmp) return ms_fail(MS_ERR_NOMEM, "delta scratch");
+    hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(in->count_bound, WG * 8ull)), dim3(WG), 0, sc.s, v, tmp);
+    MS_HIP(hipGetLastError());
+    *arr = tmp;
+    return MS_OK;
+}
+
+int launch_delta_carries(const ms_col *in, scratch &sc)
+{
+    ms_colv v;
+    MS_TRY(colv_of(in, v));
+    const uint64_t ntb = in->count_bound / DELTA_TILE + 1;
+    if (!in->refs || in->refs_cap < ntb * 8) return ms_fail(MS_ERR_CAPACITY, "delta carries capacity");
+    const uint64_t *arr;
+    MS_TRY(delta_src(in, v, sc, &arr));
+    uint64_t *sums = (uint64_t *)sc.get(ntb * 8 * 8 + 64);
+    uint64_t *scan = (uint64_t *)sc.get(ntb * 8 * 8 + 64);
+    if (!sums || !scan) return ms_fail(MS_ERR_NOMEM, "delta carries scratch");
+    const uint32_t S = in->g.dstep;
+    const dim3 grid(grid_for(ntb, 1, 65536));
+    if (arr) hipLaunchKernelGGL(k_delta_tile_sums<1>, grid, dim3(WG), 0, sc.s, v, arr, sums, ntb);
+    else hipLaunchKernelGGL(k_delta_tile_sums<0>, grid, dim3(WG), 0, sc.s, v, arr, sums, ntb);
+    MS_HIP(hipGetLastError());
+    for (uint32_t j = 0; j < S; j++) MS_TRY(launch_exclusive_scan(sums + j * ntb, scan + j * ntb, ntb, sc));
+    hipLaunchKernelGGL(k_delta_carries, dim3(grid_for(ntb * 8, WG)), dim3(WG), 0, sc.s, scan, ntb, S, in->refs);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
 
+static uint32_t units_grid(uint64_t count_bound) { return grid_for(count_bound / 512 + 1, WAVES * 8ull, 2048); }
+
 int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s)
 {
+    (void)accumulate;
     ms_colv v;
     MS_TRY(colv_of(in, v));
-    if (!accumulate) MS_TRY(launch_fill_u64(d_result, 1, 0, s));
+    scratch sc(s);
     const uint64_t nb = in->count_bound;
-    if (nb == 0) return MS_OK;
-    const uint32_t grid_units = grid_for(nb / 512 + 1, WAVES * 4ull, 4096);
+    uint32_t grid = 1;
+    int with_rest = 1;
+    uint64_t *part = (uint64_t *)sc.get(8192 * 8);
+    if (!part) return ms_fail(MS_ERR_NOMEM, "select_sum scratch");
     switch (v.g.kind) {
     case MS_UNCOMPR:
-        hipLaunchKernelGGL(k_select_sum_uncompr, dim3(grid_for(nb, WG * 16ull, 4096)), dim3(WG), 0, s, v, p, d_result);
-        MS_HIP(hipGetLastError());
-        return MS_OK;
+        grid = grid_for(nb, WG * 32ull, 2048);
+        hipLaunchKernelGGL(k_select_sum_uncompr, dim3(grid), dim3(WG), 0, s, v, p, part);
+        with_rest = 0;
+        break;
     case MS_RLE:
-        hipLaunchKernelGGL(k_select_sum_rle, dim3(grid_for(nb, WG * 8ull, 4096)), dim3(WG), 0, s, v, p, d_result);
-        MS_HIP(hipGetLastError());
-        return MS_OK;
+        grid = grid_for(nb / 64 + 1, WG * 4ull, 2048);
+        hipLaunchKernelGGL(k_select_sum_rle, dim3(grid), dim3(WG), 0, s, v, p, part);
+        with_rest = 0;
+        break;
     case MS_STATIC_VBP:
-        if (fast_static(v)) hipLaunchKernelGGL(static_sum_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
-        else hipLaunchKernelGGL(k_select_sum_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, p, d_result);
+        if (fast_static(v)) {
+            grid = units_grid(nb);
+            hipLaunchKernelGGL(static_sum_kernel(v.g.bw), dim3(grid), dim3(WG), 0, s, v, p, part);
+        } else {
+            grid = grid_for(nb, WG * 16ull, 2048);
+            hipLaunchKernelGGL(k_select_sum_generic, dim3(grid), dim3(WG), 0, s, v, p, part);
+        }
         break;
     case MS_DYNAMIC_VBP:
     case MS_FOR:
         if (fast_blocks(v)) {
-            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
-            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid_units), dim3(WG), 0, s, v, p, d_result);
+            grid = units_grid(nb);
+            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid), dim3(WG), 0, s, v, p, part);
+            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid), dim3(WG), 0, s, v, p, part);
         } else {
-            hipLaunchKernelGGL(k_select_sum_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, p, d_result);
+            grid = grid_for(nb, WG * 16ull, 2048);
+            hipLaunchKernelGGL(k_select_sum_generic, dim3(grid), dim3(WG), 0, s, v, p, part);
         }
         break;
-    case MS_DELTA:
-        MS_TRY(delta_scan(in, v, 1, nullptr, p, d_result, s));
+    case MS_DELTA: {
+        const uint64_t *arr;
+        MS_TRY(delta_src(in, v, sc, &arr));
+        grid = grid_for(nb / DELTA_TILE + 1, 2, 2048);
+        if (arr) hipLaunchKernelGGL((k_delta_tiles<1, 1>), dim3(grid), dim3(WG), 0, s, v, arr, nullptr, p, part);
+        else hipLaunchKernelGGL((k_delta_tiles<1, 0>), dim3(grid), dim3(WG), 0, s, v, arr, nullptr, p, part);
         break;
+    }
     default:
         return ms_fail(MS_ERR_INVALID, "select_sum: unsupported format");
     }
     MS_HIP(hipGetLastError());
-    // the uncompressed rest part
-    hipLaunchKernelGGL(k_select_sum_uncompr, dim3(grid_for(in->g.bs + 1, WG * 2ull, 64)), dim3(WG), 0, s, v, p, d_result);
+    hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(WG), 0, s, v, p, (const uint64_t *)part, grid, with_rest, d_result);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
@@ -718,9 +847,15 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
             hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
         }
         break;
-    case MS_DELTA:
-        MS_TRY(delta_scan(in, v, 0, d_out, ms_pred{}, nullptr, s));
+    case MS_DELTA: {
+        scratch sc(s);
+        const uint64_t *arr;
+        MS_TRY(delta_src(in, v, sc, &arr));
+        const dim3 grid(grid_for(nb / DELTA_TILE + 1, 1, 65536));
+        if (arr) hipLaunchKernelGGL((k_delta_tiles<0, 1>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);
+        else hipLaunchKernelGGL((k_delta_tiles<0, 0>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);
         break;
+    }
     default:
         return ms_fail(MS_ERR_INVALID, "decompress: unsupported format");
     }

@@ -46,14 +46,14 @@ __device__ __forceinline__ void vbp_load(const uint64_t *__restrict__ base, uint
     const uint32_t need = sh + 8 * BW;
 #pragma unroll
     for (int q = 0; q < NW; q++) {
-        if (BW % 8 == 0 || (uint32_t)(q * 64) < need) w[q] = __builtin_nontemporal_load(base + (w0 + q) * S + j);
+        if (BW % 8 == 0 || (uint32_t)(q * 64) < need) w[q] = base[(w0 + q) * S + j];
         else w[q] = 0;
     }
 }
 
 // Cuts the 8 codes (BW <= 32) out of the loaded words.
 template <int BW>
-__device__ __forceinline__ void vbp_cut32(const uint64_t (&w)[vbp_traits<BW>::NW], uint32_t sh, uint32_t (&c)[8])
+__device__ __forceinline__ void vbp_cut32(const uint64_t *w, uint32_t sh, uint32_t (&c)[8])
 {
     static_assert(BW <= 32, "32-bit cut");
     constexpr int NW = vbp_traits<BW>::NW;
@@ -92,7 +92,7 @@ __device__ __forceinline__ void vbp_cut32(const uint64_t (&w)[vbp_traits<BW>::NW
 
 // Cuts the 8 codes for any BW (1..64) as 64-bit values.
 template <int BW>
-__device__ __forceinline__ void vbp_cut64(const uint64_t (&w)[vbp_traits<BW>::NW], uint32_t sh, uint64_t (&c)[8])
+__device__ __forceinline__ void vbp_cut64(const uint64_t *w, uint32_t sh, uint64_t (&c)[8])
 {
     constexpr int NW = vbp_traits<BW>::NW;
     constexpr int NL = vbp_traits<BW>::NL;
@@ -117,6 +117,27 @@ __device__ __forceinline__ void vbp_cut64(const uint64_t (&w)[vbp_traits<BW>::NW
         if (b + BW > 64) v |= L[a + 1] << (64 - b);
         c[m] = v & MASK;
     }
+}
+
+// Loads, for a run-time bit width bw <= 32, the (at most 5) words holding
+// codes 8g..8g+7 of lane j; words past the thread's bit string stay 0.
+__device__ __forceinline__ void vbp_load_rt(const uint64_t *__restrict__ base, uint32_t S, uint32_t j, uint64_t g,
+                                            uint32_t bw, uint64_t (&w)[5], uint32_t &sh)
+{
+    const uint64_t byte0 = g * (uint64_t)bw;
+    const uint64_t w0 = byte0 >> 3;
+    sh = (uint32_t)(byte0 & 7) * 8;
+    const uint32_t need = sh + 8 * bw;
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        w[q] = (uint32_t)(q * 64) < need ? base[(w0 + q) * S + j] : 0ull;
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t lane)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)x, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(x >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Random read of code k of lane j with a run-time bit width.

@@ -247,6 +247,18 @@ __global__ __launch_bounds__(WG) void k_pack_static(emit_layout l, const uint64_
     }
 }
 
+// delta_f tile carries: the S values preceding each tile's first value.
+__global__ __launch_bounds__(WG) void k_delta_carries_values(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                             uint64_t *refs)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint64_t ntiles = (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < ntiles * 8; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t tile = i / 8, j = i % 8;
+        refs[i] = (tile == 0 || j >= l.dstep) ? 0ull : v[tile * MS_DELTA_TILE - l.dstep + j];
+    }
+}
+
 __global__ void k_emit_finish(emit_layout l, const uint64_t *d_m, const uint64_t *start, const uint32_t *bws,
                               ms_dev_meta *meta, uint64_t *d_append)
 {
@@ -335,6 +347,12 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         MS_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)start,
                            (const uint32_t *)bws, out->meta, d_append);
+        if (f.kind == MS_DELTA) {
+            if (out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
+                return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
+            hipLaunchKernelGGL(k_delta_carries_values, dim3(grid_for(m_bound / MS_DELTA_TILE * 8 + 8, WG)),
+                               dim3(WG), 0, s, l, d_v, d_m, out->refs);
+        }
     }
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,

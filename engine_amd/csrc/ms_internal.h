@@ -7,7 +7,9 @@
 //   * a block index: one 64-bit entry per innermost vertical-bit-packed
 //                    block of the compressed part, (payload byte offset << 8)
 //                    | bit width.  It is derived data, not part of the image;
-//   * for for_f     : a copy of each FOR block's reference value.
+//   * for for_f     : a copy of each FOR block's reference value;
+//   * for delta_f   : per MS_DELTA_TILE values, the lane sums at the tile's
+//                     start (8 slots per tile).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,6 +18,10 @@
 #include "../../include/morphstore_hip.h"
 
 #define MS_HD __host__ __device__ __forceinline__
+
+// delta_f columns keep, per tile of this many values of the compressed
+// part, the running sums of the delta lanes at the tile's start.
+#define MS_DELTA_TILE 2048ull
 
 struct ms_dev_meta {
     uint64_t count;
@@ -113,7 +119,7 @@ struct ms_colv {
     const uint8_t *data;
     const ms_dev_meta *meta;
     const uint64_t *index;   // per innermost block: (offset << 8) | bw
-    const uint64_t *refs;    // for_f: reference value per FOR block
+    const uint64_t *refs;    // for_f: reference per FOR block; delta_f: tile carries
     ms_geom g;
 };
 

@@ -34,6 +34,8 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
 // ascending positions with pred into d_pos (capacity >= count bound);
 // *d_cnt receives their number
 int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc);
+// rebuilds a delta_f column's tile carries from its image and index
+int launch_delta_carries(const ms_col *in, scratch &sc);
 // exclusive prefix sums of n u64 values (in-place allowed)
 int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc);
 
