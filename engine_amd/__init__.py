@@ -13,7 +13,8 @@ import numpy as np
 from .formats import ColMeta, FormatDesc  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmorphstore_hip.so")
+# MS_LIB_PATH selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("MS_LIB_PATH") or os.path.join(_HERE, "libmorphstore_hip.so")
 
 _lib = None
 _inited = False

@@ -86,8 +86,8 @@ static bool valid_format(const ms_format_desc *fp)
         const uint64_t ibs = in.kind == MS_UNCOMPR ? 1 : in.kind == MS_STATIC_VBP ? 64ull * in.step : in.block;
         if (f.block % ibs) return false;
         if (f.kind == MS_DELTA)
-            return f.step >= 1 && f.step <= 64 && f.block % f.step == 0 &&
-                   (in.kind == MS_UNCOMPR || in.step == f.step) && (64 % f.step == 0);
+            return f.step >= 1 && f.step <= 8 && f.block % f.step == 0 &&
+                   (in.kind == MS_UNCOMPR || in.step == f.step) && (8 % f.step == 0);
         const uint32_t E = f.step ? f.step : (in.kind != MS_UNCOMPR ? in.step : 1);
         return f.page > 0 && f.page % E == 0 && f.block % E == 0 && (in.kind == MS_UNCOMPR || in.step == E) &&
                (64 % E == 0);

@@ -10,54 +10,9 @@
 // k = 8g .. 8g+7 with g = unit*(64/S) + t/S (values S*k + j).
 #include <hipcub/hipcub.hpp>
 
-#include <type_traits>
-
-#include "ms_kernels.h"
+#include "ms_units.hpp"
 
 namespace ms {
-
-typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
-
-static constexpr int WG = 256;
-static constexpr int WAVES = WG / MS_WAVE;
-
-static inline uint32_t grid_for(uint64_t items, uint64_t per_wg, uint32_t cap = 8192)
-{
-    uint64_t g = (items + per_wg - 1) / per_wg;
-    if (g < 1) g = 1;
-    if (g > cap) g = cap;
-    return (uint32_t)g;
-}
-
-__device__ __forceinline__ uint32_t wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-
-__device__ __forceinline__ uint64_t uniform64(uint64_t x)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ ms_parts parts_of(const ms_colv &c)
-{
-    return ms_parts_of(c.g, c.meta->count, c.meta->size_used, c.meta->size_compr);
-}
-
-// Writes the workgroup's sum of per-thread values to part[blockIdx.x]
-// (no same-address atomics: a finishing kernel adds the partial sums).
-__device__ __forceinline__ void wg_partial(uint64_t acc, uint64_t *part)
-{
-    __shared__ uint64_t s_part[WAVES];
-    acc = wave_sum_u64(acc);
-    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t s = 0;
-#pragma unroll
-        for (int w = 0; w < WAVES; w++) s += s_part[w];
-        part[blockIdx.x] = s;
-    }
-}
 
 __global__ void k_fill_u64(uint64_t *p, uint64_t n, uint64_t v)
 {
@@ -73,249 +28,66 @@ int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s)
     return MS_OK;
 }
 
-// ------------------------------------------------------------------
-// Random read of value i of the compressed part (static_vbp, or any
-// indexed layout; for_f adds the block's reference value).
-// ------------------------------------------------------------------
-__device__ __forceinline__ uint64_t read_compr(const ms_colv &c, uint64_t i)
-{
-    if (c.g.kind == MS_STATIC_VBP) {
-        const uint32_t S = c.g.lanes;
-        return vbp_read((const uint64_t *)c.data, S, (uint32_t)(i % S), i / S, c.g.bw);
-    }
-    const uint64_t b = i / c.g.inner_bs;
-    const uint64_t r = i - b * c.g.inner_bs;
-    const uint64_t e = c.index[b];
-    const uint32_t bw = (uint32_t)(e & 0xff);
-    const uint32_t S = c.g.lanes;
-    uint64_t v = vbp_read((const uint64_t *)(c.data + (e >> 8)), S, (uint32_t)(r % S), r / S, bw);
-    if (c.g.kind == MS_FOR) v += c.refs[b / (c.g.outer_block / c.g.inner_bs)];
-    return v;
-}
-
-// ------------------------------------------------------------------
-// Fused select + sum over 8 codes of one thread.
-// ------------------------------------------------------------------
-// Select + sum of the 8 codes cut from preloaded words (BW <= 32).
-template <int BW>
-__device__ __forceinline__ void ss_words(const uint64_t *w, uint32_t sh, const ms_cpred &cp, uint64_t ref,
-                                         uint64_t &acc)
-{
-    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
-    uint32_t c[8];
-    vbp_cut32<BW>(w, sh, c);
-    A s = 0;
-    uint32_t cnt;
-    if (cp.mode == PM_ALL) {
-#pragma unroll
-        for (int m = 0; m < 8; m++) s += c[m];
-        cnt = 8;
-    } else {
-        const uint32_t lo = (uint32_t)cp.clo, span = (uint32_t)cp.cspan;
-        const bool out = cp.mode == PM_OUT;
-        cnt = 0;
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
-            s += take ? c[m] : 0u;
-            cnt += take;
-        }
-    }
-    acc += (uint64_t)s + (uint64_t)cnt * ref;
-}
-
-template <int BW>
-__device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
-                                        uint64_t ref, uint64_t &acc)
-{
-    uint64_t w[vbp_traits<BW>::NW];
-    uint32_t sh;
-    vbp_load<BW>(base, S, j, g, w, sh);
-    if constexpr (BW <= 32) {
-        ss_words<BW>(w, sh, cp, ref, acc);
-    } else {
-        uint64_t c[8];
-        vbp_cut64<BW>(w, sh, c);
-        uint64_t s = 0, cnt = 0;
-        if (cp.mode == PM_ALL) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) s += c[m];
-            cnt = 8;
-        } else {
-            const bool out = cp.mode == PM_OUT;
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const bool take = ((c[m] - cp.clo) <= cp.cspan) != out;
-                s += take ? c[m] : 0ull;
-                cnt += take;
-            }
-        }
-        acc += s + cnt * ref;
-    }
-}
-
-#define MS_BW_CASES(X)                                                                                              \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
-    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)  \
-    X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58)  \
-    X(59) X(60) X(61) X(62) X(63) X(64)
-
-__device__ __forceinline__ void ss_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
-                                            const ms_cpred &cp, uint64_t ref, uint64_t &acc)
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: ss_unit<B>(base, S, j, g, cp, ref, acc); break;
-        MS_BW_CASES(MS_CASE)
-#undef MS_CASE
-    default: break;
-    }
-}
-
-#define MS_BW32_CASES(X)                                                                                            \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
-    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
-
-__device__ __forceinline__ void ss_words_dispatch(uint32_t bw, const uint64_t *w, uint32_t sh, const ms_cpred &cp,
-                                                  uint64_t ref, uint64_t &acc)
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: ss_words<B>(w, sh, cp, ref, acc); break;
-        MS_BW32_CASES(MS_CASE)
-#undef MS_CASE
-    default: break;
-    }
-}
-
-// Decodes 8 codes as 64-bit values with a run-time bit width.
-template <int BW>
-__device__ __forceinline__ void dec_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, uint64_t (&c)[8])
-{
-    uint64_t w[vbp_traits<BW>::NW];
-    uint32_t sh;
-    vbp_load<BW>(base, S, j, g, w, sh);
-    vbp_cut64<BW>(w, sh, c);
-}
-
-__device__ __forceinline__ void dec_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
-                                             uint64_t (&c)[8])
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: dec_unit<B>(base, S, j, g, c); break;
-        MS_BW_CASES(MS_CASE)
-#undef MS_CASE
-    default:
-#pragma unroll
-        for (int m = 0; m < 8; m++) c[m] = 0;
-        break;
-    }
-}
-
-// Indexed layouts with 8 lanes and blocks of a multiple of 512 values:
-// dynamic_vbp_f<512k, P, 8> and for_f<L, P, dynamic_vbp_f<512k, P', 8>>.
-// One wave decodes a 512-value unit; the unit's index entry for the next
-// iteration is fetched one iteration ahead.  Per unit the predicate is
-// specialised on the block's code range: no value / every value / x <= c /
-// x in [lo, hi] / x outside [lo, hi].
-enum { SM_NONE = 0, SM_ALL = 1, SM_LE = 2, SM_IN = 3, SM_OUT = 4 };
-
-template <int BW, bool CNT>
-__device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t mode,
-                                        uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
-{
-    constexpr int NW = vbp_traits<BW>::NW;
-    const uint32_t byte0 = g * BW;
-    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
-    const uint32_t sh = (byte0 & 7) * 8;
-    uint64_t w[NW];
-    // the last word may lie past this thread's bits (and past the block):
-    // column buffers are padded, and those bits are never used
-#pragma unroll
-    for (int q = 0; q < NW; q++) w[q] = p[q * 8];
-    uint32_t c[8];
-    vbp_cut32<BW>(w, sh, c);
-    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
-    A s = 0;
-    uint32_t cnt = 0;
-    if (mode == SM_ALL) {
-#pragma unroll
-        for (int m = 0; m < 8; m++) s += c[m];
-        cnt = 8;
-    } else if (mode == SM_LE) {
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const bool take = c[m] <= span;
-            s += take ? c[m] : 0u;
-            if (CNT) cnt += take;
-        }
-    } else {
-        const bool out = mode == SM_OUT;
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
-            s += take ? c[m] : 0u;
-            if (CNT) cnt += take;
-        }
-    }
-    acc += (uint64_t)s;
-    if (CNT) acc += (uint64_t)cnt * ref;
-}
-
-template <bool CNT>
-__device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
-                                                 uint32_t mode, uint32_t lo, uint32_t span, uint64_t ref,
-                                                 uint64_t &acc)
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: ss_fast<B, CNT>(base, j, g, mode, lo, span, ref, acc); break;
-        MS_BW32_CASES(MS_CASE)
-#undef MS_CASE
-    default: break;
-    }
-}
-
+// One wave decodes one 512-value unit per iteration (grid-stride); the
+// next unit's index entry is fetched one iteration ahead.  Unit -> block
+// and block -> for_f block are shifts (the host only takes this path when
+// the ratios are powers of two).  dynamic_vbp takes its per-bit-width
+// predicate from a table built on the host; for_f specialises it per block
+// around the block's reference value.
 template <bool IS_FOR>
-__global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, uint64_t *res)
+__global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, ms_bwtab tab, uint64_t *res)
 {
     const ms_parts pt = parts_of(c);
-    const uint64_t spb = c.g.inner_bs / 512;
-    const uint64_t n = pt.n_compr / 512;
-    const uint64_t ib_per_ob = IS_FOR ? c.g.outer_block / c.g.inner_bs : 1;
+    const uint32_t lg_spb = c.g.lg_spb;
+    const uint32_t lg_ipo = c.g.lg_ipo;
+    const uint64_t n = pt.n_compr >> 9;
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & 7, gl = t >> 3;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
     uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
-    uint64_t e_next = u < n ? uniform64(c.index[u / spb]) : 0ull;
+    uint64_t e_next = u < n ? uniform64(c.index[u >> lg_spb]) : 0ull;
     for (; u < n; u += W) {
         const uint64_t e = e_next;
-        if (u + W < n) e_next = uniform64(c.index[(u + W) / spb]);
-        const uint64_t b = u / spb;
+        if (u + W < n) e_next = uniform64(c.index[(u + W) >> lg_spb]);
+        const uint64_t b = u >> lg_spb;
         const uint32_t bw = (uint32_t)(e & 0xff);
         const uint64_t *base = (const uint64_t *)(c.data + (e >> 8));
-        const uint32_t g = (uint32_t)(u - b * spb) * 8 + gl;
-        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0ull;
-        const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
-        if (IS_FOR && ref + maxcode < ref) {
-            // codes wrap around 2^64: evaluate on the values themselves
+        const uint32_t g = (uint32_t)(u & ((1ull << lg_spb) - 1)) * 8 + gl;
+        uint64_t ref = 0;
+        uint32_t mode, lo, span;
+        if (IS_FOR) {
+            ref = uniform64(c.refs[b >> lg_ipo]);
+            const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
+            if (bw > 32 || ref + maxcode < ref) {
+                // wide or wrapping codes (rare): value by value on x itself
 #pragma unroll 1
-            for (int m = 0; m < 8; m++) {
-                const uint64_t x = ref + vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
-                if (pred_eval(p, x)) acc += x;
+                for (int m = 0; m < 8; m++) {
+                    const uint64_t x = ref + vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
+                    if (pred_eval(p, x)) acc += x;
+                }
+                continue;
             }
-            continue;
+            const ms_cpred cp = code_pred(p, ref, maxcode);
+            mode = cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
+                 : cp.clo == 0 ? SM_LE : SM_IN;
+            lo = (uint32_t)cp.clo;
+            span = (uint32_t)cp.cspan;
+        } else {
+            if (bw > 32) {
+#pragma unroll 1
+                for (int m = 0; m < 8; m++) {
+                    const uint64_t x = vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
+                    if (pred_eval(p, x)) acc += x;
+                }
+                continue;
+            }
+            mode = tab.mode[bw];
+            lo = tab.lo[bw];
+            span = tab.span[bw];
         }
-        const ms_cpred cp = code_pred(p, ref, maxcode);
-        if (cp.mode == PM_NONE) continue;
-        if (bw > 32) {
-            ss_dispatch(bw, base, 8, j, g, cp, ref, acc);
-            continue;
-        }
-        uint32_t mode = cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT : cp.clo == 0 ? SM_LE : SM_IN;
-        ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, (uint32_t)cp.clo, (uint32_t)cp.cspan, ref, acc);
+        if (mode == SM_NONE) continue;
+        ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, lo, span, ref, acc);
     }
     wg_partial(acc, res);
 }
@@ -325,19 +97,19 @@ template <int BW>
 __global__ __launch_bounds__(WG) void k_select_sum_static(ms_colv c, ms_pred p, uint64_t *res)
 {
     const ms_parts pt = parts_of(c);
-    const uint32_t S = c.g.lanes;
-    const uint64_t gpu = 64 / S;
-    const uint64_t ngroups = pt.n_compr / S / 8;
-    const uint64_t nunits = (ngroups + gpu - 1) / gpu;
+    const uint32_t S = c.g.lanes, lgS = c.g.lg_lanes;
+    const uint64_t gpu = 64 >> lgS;
+    const uint64_t ngroups = pt.n_compr >> (lgS + 3);
+    const uint64_t nunits = (ngroups + gpu - 1) >> (6 - lgS);
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t % S;
+    const uint32_t j = t & (S - 1);
     const uint64_t maxcode = BW >= 64 ? ~0ull : (1ull << BW) - 1ull;
     const ms_cpred cp = code_pred(p, 0, maxcode);
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
     if (cp.mode != PM_NONE) {
         for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
-            const uint64_t g = u * gpu + t / S;
+            const uint64_t g = u * gpu + (t >> lgS);
             if (g < ngroups) ss_unit<BW>((const uint64_t *)c.data, S, j, g, cp, 0, acc);
         }
     }
@@ -411,8 +183,7 @@ __device__ __forceinline__ void delta_load(const ms_colv &c, const uint64_t *arr
         return;
     }
     if (SRC == 0) {
-        const uint64_t spb = c.g.inner_bs / 512;
-        const uint64_t b = u / spb, sub = u - b * spb;
+        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
         const uint64_t e = uniform64(c.index[b]);
         dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
     } else {
@@ -457,11 +228,11 @@ __global__ __launch_bounds__(WG) void k_delta_tiles(ms_colv c, const uint64_t *a
     __shared__ uint64_t s_wtot[WAVES][8];
     const ms_parts pt = parts_of(c);
     const uint32_t S = c.g.dstep;
-    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t nunits = pt.n_compr >> 9;
     const uint64_t ntiles = (nunits + WAVES - 1) / WAVES;
     const uint32_t wv = wave_index();
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t % S, gl = t / S;
+    const uint32_t j = t & (S - 1), gl = t >> c.g.lg_dstep;
     uint64_t acc = 0;
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t u = tile * WAVES + wv;
@@ -493,10 +264,10 @@ __global__ __launch_bounds__(WG) void k_delta_tile_sums(ms_colv c, const uint64_
     __shared__ uint64_t s_wtot[WAVES][8];
     const ms_parts pt = parts_of(c);
     const uint32_t S = c.g.dstep;
-    const uint64_t nunits = pt.n_compr / 512;
+    const uint64_t nunits = pt.n_compr >> 9;
     const uint32_t wv = wave_index();
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t % S, gl = t / S;
+    const uint32_t j = t & (S - 1), gl = t >> c.g.lg_dstep;
     for (uint64_t tile = blockIdx.x; tile < ntb; tile += gridDim.x) {
         uint64_t d[8];
         delta_load<SRC>(c, arr, tile * WAVES + wv, nunits, S, j, gl, d);
@@ -521,16 +292,14 @@ template <bool IS_FOR>
 __global__ __launch_bounds__(WG) void k_decomp_blocks(ms_colv c, uint64_t *out)
 {
     const ms_parts pt = parts_of(c);
-    const uint64_t spb = c.g.inner_bs / 512;
-    const uint64_t nunits = pt.n_compr / 512;
-    const uint64_t ib_per_ob = IS_FOR ? c.g.outer_block / c.g.inner_bs : 1;
+    const uint64_t nunits = pt.n_compr >> 9;
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & 7, gl = t >> 3;
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
     for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
-        const uint64_t b = u / spb, sub = u - b * spb;
+        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
         const uint64_t e = uniform64(c.index[b]);
-        const uint64_t ref = IS_FOR ? uniform64(c.refs[b / ib_per_ob]) : 0;
+        const uint64_t ref = IS_FOR ? uniform64(c.refs[b >> c.g.lg_ipo]) : 0;
         uint64_t d[8];
         dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
         uint64_t *o = out + u * 512;
@@ -543,15 +312,15 @@ template <int BW>
 __global__ __launch_bounds__(WG) void k_decomp_static(ms_colv c, uint64_t *out)
 {
     const ms_parts pt = parts_of(c);
-    const uint32_t S = c.g.lanes;
-    const uint64_t gpu = 64 / S;
-    const uint64_t ngroups = pt.n_compr / S / 8;
-    const uint64_t nunits = (ngroups + gpu - 1) / gpu;
+    const uint32_t S = c.g.lanes, lgS = c.g.lg_lanes;
+    const uint64_t gpu = 64 >> lgS;
+    const uint64_t ngroups = pt.n_compr >> (lgS + 3);
+    const uint64_t nunits = (ngroups + gpu - 1) >> (6 - lgS);
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t % S;
+    const uint32_t j = t & (S - 1);
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
     for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
-        const uint64_t g = u * gpu + t / S;
+        const uint64_t g = u * gpu + (t >> lgS);
         if (g >= ngroups) continue;
         uint64_t d[8];
         dec_unit<BW>((const uint64_t *)c.data, S, j, g, d);
@@ -657,12 +426,38 @@ __global__ __launch_bounds__(WG) void k_sel_compact(const uint64_t *x, const uin
 // ------------------------------------------------------------------
 static bool fast_blocks(const ms_colv &v)
 {
-    // 8-lane indexed layouts whose blocks hold a multiple of 512 values
+    // 8-lane indexed layouts whose blocks hold a power-of-two multiple of
+    // 512 values (and, for cascades, a power-of-two number of blocks per
+    // outer block)
     return (v.g.kind == MS_DYNAMIC_VBP || v.g.kind == MS_FOR || v.g.kind == MS_DELTA) &&
-           v.g.leaf == MS_DYNAMIC_VBP && v.g.lanes == 8 && v.g.inner_bs % 512 == 0;
+           v.g.leaf == MS_DYNAMIC_VBP && v.g.lanes == 8 && v.g.lg_spb != 0xff &&
+           (v.g.kind == MS_DYNAMIC_VBP || v.g.lg_ipo != 0xff);
 }
 
-static bool fast_static(const ms_colv &v) { return v.g.kind == MS_STATIC_VBP && 64 % v.g.lanes == 0; }
+// ms_bwtab for codes that are the values (ref 0), bit widths 1..32.
+static ms_bwtab make_bwtab(const ms_pred &p)
+{
+    ms_bwtab t = {};
+    for (uint32_t bw = 1; bw <= 32; bw++) {
+        const uint64_t maxcode = (1ull << bw) - 1ull;
+        uint64_t lo = p.lo, hi = p.hi;
+        const bool in_empty = p.empty || lo > maxcode;
+        if (!in_empty && hi > maxcode) hi = maxcode;
+        const bool in_all = !in_empty && lo == 0 && hi == maxcode;
+        uint32_t mode;
+        if (!p.neg) mode = in_empty ? SM_NONE : in_all ? SM_ALL : lo == 0 ? SM_LE : SM_IN;
+        else mode = in_empty ? SM_ALL : in_all ? SM_NONE : SM_OUT;
+        t.mode[bw] = mode;
+        t.lo[bw] = in_empty ? 0 : (uint32_t)lo;
+        t.span[bw] = in_empty ? 0 : (uint32_t)(hi - lo);
+    }
+    return t;
+}
+
+static bool fast_static(const ms_colv &v)
+{
+    return v.g.kind == MS_STATIC_VBP && v.g.lg_lanes != 0xff && v.g.lanes <= 64;
+}
 
 typedef void (*static_kernel_t)(ms_colv, ms_pred, uint64_t *);
 typedef void (*static_dec_t)(ms_colv, uint64_t *);
@@ -714,6 +509,7 @@ __global__ __launch_bounds__(WG) void k_sum_finish(ms_colv c, ms_pred p, const u
 }
 
 static bool delta_fast(const ms_colv &v) { return fast_blocks(v) && v.g.dstep == 8; }
+static bool delta_shiftable(const ms_colv &v) { return v.g.lg_dstep != 0xff && v.g.dstep <= 8; }
 
 // Deltas of a delta_f column whose inner layout has no fast decoder,
 // decoded per value into a temporary.
@@ -788,8 +584,9 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     case MS_FOR:
         if (fast_blocks(v)) {
             grid = units_grid(nb);
-            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid), dim3(WG), 0, s, v, p, part);
-            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid), dim3(WG), 0, s, v, p, part);
+            const ms_bwtab tab = make_bwtab(p);
+            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid), dim3(WG), 0, s, v, p, tab, part);
+            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid), dim3(WG), 0, s, v, p, tab, part);
         } else {
             grid = grid_for(nb, WG * 16ull, 2048);
             hipLaunchKernelGGL(k_select_sum_generic, dim3(grid), dim3(WG), 0, s, v, p, part);

@@ -44,8 +44,20 @@ struct ms_geom {
     uint64_t outer_page;  // for: blocks per page
     uint64_t gap;         // 2 * F::get_size_max_byte(F::m_BlockSize)
     uint32_t dstep;       // delta_f: step of the differences
-    uint32_t pad;
+    // log2 of ratios the kernels divide by (0xff: not a power of two)
+    uint8_t lg_spb;       // 512-value units per innermost block
+    uint8_t lg_ipo;       // innermost blocks per outer (delta/for) block
+    uint8_t lg_lanes;     // innermost lanes
+    uint8_t lg_dstep;     // delta step
 };
+
+MS_HD uint8_t ms_lg2(uint64_t x)
+{
+    if (x == 0 || (x & (x - 1))) return 0xff;
+    uint8_t l = 0;
+    while ((1ull << l) < x) l++;
+    return l;
+}
 
 MS_HD uint64_t ms_align64(uint64_t x) { return (x + 63ull) & ~63ull; }
 
@@ -97,6 +109,11 @@ MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
         g.leaf = MS_RLE; g.lanes = 1; g.bs = 0; g.inner_bs = 0; g.gap = 0;
         break;
     }
+    g.lg_spb = (g.inner_bs && g.inner_bs % 512 == 0) ? ms_lg2(g.inner_bs / 512) : 0xff;
+    g.lg_ipo = (g.outer_block && g.inner_bs && g.outer_block % g.inner_bs == 0) ? ms_lg2(g.outer_block / g.inner_bs)
+                                                                               : 0xff;
+    g.lg_lanes = ms_lg2(g.lanes);
+    g.lg_dstep = ms_lg2(g.dstep);
     return g;
 }
 

@@ -1,0 +1,365 @@
+// ms_units.hpp -- per-unit decode routines shared by the kernels of one
+// column (ms_decode.hip) and the batched persistent kernel (ms_batch.hip).
+// A unit is 512 values decoded by one wave (see ms_device.hpp for the
+// thread mapping of the vertical bit-packed layouts).
+#pragma once
+
+#include <type_traits>
+
+#include "ms_kernels.h"
+
+namespace ms {
+
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+constexpr int WG = 256;
+constexpr int WAVES = WG / MS_WAVE;
+
+inline uint32_t grid_for(uint64_t items, uint64_t per_wg, uint32_t cap = 8192)
+{
+    uint64_t g = (items + per_wg - 1) / per_wg;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+__device__ __forceinline__ uint32_t wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ ms_parts parts_of(const ms_colv &c)
+{
+    return ms_parts_of(c.g, c.meta->count, c.meta->size_used, c.meta->size_compr);
+}
+
+// Writes the workgroup's sum of per-thread values to part[blockIdx.x]
+// (no same-address atomics: a finishing kernel adds the partial sums).
+__device__ __forceinline__ void wg_partial(uint64_t acc, uint64_t *part)
+{
+    __shared__ uint64_t s_part[WAVES];
+    acc = wave_sum_u64(acc);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+#pragma unroll
+        for (int w = 0; w < WAVES; w++) s += s_part[w];
+        part[blockIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------------
+// Random read of value i of the compressed part (static_vbp, or any
+// indexed layout; for_f adds the block's reference value).
+// ------------------------------------------------------------------
+__device__ __forceinline__ uint64_t read_compr(const ms_colv &c, uint64_t i)
+{
+    if (c.g.kind == MS_STATIC_VBP) {
+        const uint32_t S = c.g.lanes;
+        return vbp_read((const uint64_t *)c.data, S, (uint32_t)(i % S), i / S, c.g.bw);
+    }
+    const uint64_t b = i / c.g.inner_bs;
+    const uint64_t r = i - b * c.g.inner_bs;
+    const uint64_t e = c.index[b];
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    const uint32_t S = c.g.lanes;
+    uint64_t v = vbp_read((const uint64_t *)(c.data + (e >> 8)), S, (uint32_t)(r % S), r / S, bw);
+    if (c.g.kind == MS_FOR) v += c.refs[b / (c.g.outer_block / c.g.inner_bs)];
+    return v;
+}
+
+// ------------------------------------------------------------------
+// Fused select + sum over 8 codes of one thread.
+// ------------------------------------------------------------------
+// Select + sum of the 8 codes cut from preloaded words (BW <= 32).
+template <int BW>
+__device__ __forceinline__ void ss_words(const uint64_t *w, uint32_t sh, const ms_cpred &cp, uint64_t ref,
+                                         uint64_t &acc)
+{
+    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+    uint32_t c[8];
+    vbp_cut32<BW>(w, sh, c);
+    A s = 0;
+    uint32_t cnt;
+    if (cp.mode == PM_ALL) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) s += c[m];
+        cnt = 8;
+    } else {
+        const uint32_t lo = (uint32_t)cp.clo, span = (uint32_t)cp.cspan;
+        const bool out = cp.mode == PM_OUT;
+        cnt = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+            s += take ? c[m] : 0u;
+            cnt += take;
+        }
+    }
+    acc += (uint64_t)s + (uint64_t)cnt * ref;
+}
+
+template <int BW>
+__device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
+                                        uint64_t ref, uint64_t &acc)
+{
+    uint64_t w[vbp_traits<BW>::NW];
+    uint32_t sh;
+    vbp_load<BW>(base, S, j, g, w, sh);
+    if constexpr (BW <= 32) {
+        ss_words<BW>(w, sh, cp, ref, acc);
+    } else {
+        uint64_t c[8];
+        vbp_cut64<BW>(w, sh, c);
+        uint64_t s = 0, cnt = 0;
+        if (cp.mode == PM_ALL) {
+#pragma unroll
+            for (int m = 0; m < 8; m++) s += c[m];
+            cnt = 8;
+        } else {
+            const bool out = cp.mode == PM_OUT;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const bool take = ((c[m] - cp.clo) <= cp.cspan) != out;
+                s += take ? c[m] : 0ull;
+                cnt += take;
+            }
+        }
+        acc += s + cnt * ref;
+    }
+}
+
+#define MS_BW_CASES(X)                                                                                              \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
+    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)  \
+    X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58)  \
+    X(59) X(60) X(61) X(62) X(63) X(64)
+
+__device__ __forceinline__ void ss_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
+                                            const ms_cpred &cp, uint64_t ref, uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_unit<B>(base, S, j, g, cp, ref, acc); break;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+#define MS_BW32_CASES(X)                                                                                            \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
+    X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32)
+
+__device__ __forceinline__ void ss_words_dispatch(uint32_t bw, const uint64_t *w, uint32_t sh, const ms_cpred &cp,
+                                                  uint64_t ref, uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_words<B>(w, sh, cp, ref, acc); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+// Decodes 8 codes as 64-bit values with a run-time bit width.
+template <int BW>
+__device__ __forceinline__ void dec_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, uint64_t (&c)[8])
+{
+    uint64_t w[vbp_traits<BW>::NW];
+    uint32_t sh;
+    vbp_load<BW>(base, S, j, g, w, sh);
+    vbp_cut64<BW>(w, sh, c);
+}
+
+__device__ __forceinline__ void dec_dispatch(uint32_t bw, const uint64_t *base, uint32_t S, uint32_t j, uint64_t g,
+                                             uint64_t (&c)[8])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: dec_unit<B>(base, S, j, g, c); break;
+        MS_BW_CASES(MS_CASE)
+#undef MS_CASE
+    default:
+#pragma unroll
+        for (int m = 0; m < 8; m++) c[m] = 0;
+        break;
+    }
+}
+
+// Indexed layouts with 8 lanes and blocks of a multiple of 512 values:
+// dynamic_vbp_f<512k, P, 8> and for_f<L, P, dynamic_vbp_f<512k, P', 8>>.
+// One wave decodes a 512-value unit; the unit's index entry for the next
+// iteration is fetched one iteration ahead.  Per unit the predicate is
+// specialised on the block's code range: no value / every value / x <= c /
+// x in [lo, hi] / x outside [lo, hi].
+enum { SM_NONE = 0, SM_ALL = 1, SM_LE = 2, SM_IN = 3, SM_OUT = 4 };
+
+template <int BW, bool CNT>
+__device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t mode,
+                                        uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
+{
+    constexpr int NW = vbp_traits<BW>::NW;
+    const uint32_t byte0 = g * BW;
+    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
+    const uint32_t sh = (byte0 & 7) * 8;
+    uint64_t w[NW];
+    // the last word may lie past this thread's bits (and past the block):
+    // column buffers are padded, and those bits are never used
+#pragma unroll
+    for (int q = 0; q < NW; q++) w[q] = p[q * 8];
+    uint32_t c[8];
+    vbp_cut32<BW>(w, sh, c);
+    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+    A s = 0;
+    uint32_t cnt = 0;
+    if (mode == SM_ALL) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) s += c[m];
+        cnt = 8;
+    } else if (mode == SM_LE) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = c[m] <= span;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    } else {
+        const bool out = mode == SM_OUT;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    }
+    acc += (uint64_t)s;
+    if (CNT) acc += (uint64_t)cnt * ref;
+}
+
+// Split load / compute of one unit with a compile-time bit width: the load
+// switch fills w[0 .. NW(BW)) of a 5-word register array, so a wave can
+// issue the loads of several units before it decodes the first.
+template <int BW>
+__device__ __forceinline__ void ss_load(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint64_t (&w)[5])
+{
+    constexpr int NW = vbp_traits<BW>::NW;
+    const uint32_t byte0 = g * BW;
+    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
+#pragma unroll
+    for (int q = 0; q < NW; q++) w[q] = p[q * 8];   // may over-read one word; buffers are padded
+}
+
+template <int BW, bool CNT>
+__device__ __forceinline__ void ss_comp(const uint64_t (&w)[5], uint32_t g, uint32_t mode, uint32_t lo,
+                                        uint32_t span, uint64_t ref, uint64_t &acc)
+{
+    const uint32_t sh = ((g * BW) & 7) * 8;
+    uint32_t c[8];
+    vbp_cut32<BW>(w, sh, c);
+    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
+    A s = 0;
+    uint32_t cnt = 0;
+    if (mode == SM_ALL) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) s += c[m];
+        cnt = 8;
+    } else if (mode == SM_LE) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = c[m] <= span;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    } else {
+        const bool out = mode == SM_OUT;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
+            s += take ? c[m] : 0u;
+            if (CNT) cnt += take;
+        }
+    }
+    acc += (uint64_t)s;
+    if (CNT) acc += (uint64_t)cnt * ref;
+}
+
+__device__ __forceinline__ void ss_load_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
+                                                 uint64_t (&w)[5])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_load<B>(base, j, g, w); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+template <bool CNT>
+__device__ __forceinline__ void ss_comp_dispatch(uint32_t bw, const uint64_t (&w)[5], uint32_t g, uint32_t mode,
+                                                 uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_comp<B, CNT>(w, g, mode, lo, span, ref, acc); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+// Per-bit-width specialisation of a predicate for columns whose values are
+// their codes (no reference value): built on the host once per call.
+struct ms_bwtab {
+    uint32_t mode[33];
+    uint32_t lo[33];
+    uint32_t span[33];
+};
+
+// A unit's payload (64*bw bytes, bw <= 32) fetched with contiguous 16-byte
+// loads per lane -- one KiB per wave instruction -- into registers, then
+// staged in the wave's LDS slice where the lane-interleaved words are cut.
+struct unit_regs {
+    u64x2 r[2];
+};
+
+__device__ __forceinline__ void unit_fetch(const uint8_t *payload, uint32_t bytes, uint32_t t, unit_regs &R)
+{
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if ((uint32_t)k * 1024 + t * 16 < bytes)
+            R.r[k] = __builtin_nontemporal_load((const u64x2 *)(payload + k * 1024 + t * 16));
+}
+
+__device__ __forceinline__ void unit_stage(uint64_t *lw, uint32_t bytes, uint32_t t, const unit_regs &R)
+{
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if ((uint32_t)k * 1024 + t * 16 < bytes) *(u64x2 *)(lw + k * 128 + 2 * t) = R.r[k];
+    // LDS instructions of one wave execute in order: the reads of the other
+    // lanes below see these writes; only compiler reordering must be stopped
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool CNT>
+__device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
+                                                 uint32_t mode, uint32_t lo, uint32_t span, uint64_t ref,
+                                                 uint64_t &acc)
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: ss_fast<B, CNT>(base, j, g, mode, lo, span, ref, acc); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+}  // namespace ms
