@@ -47,6 +47,12 @@ ABI = {
     "ms_intersect_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_merge_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_calc_binary": (ctypes.c_int, [_COLP, U32, _FD, VP, VP, VP]),
+    "ms_plan_create": (ctypes.c_int, [_COLP, U64]),
+    "ms_plan_begin": (ctypes.c_int, [VP, VP]),
+    "ms_plan_end": (ctypes.c_int, [VP, VP]),
+    "ms_plan_launch": (ctypes.c_int, [VP, VP]),
+    "ms_plan_arena_bytes": (U64, [VP]),
+    "ms_plan_free": (ctypes.c_int, [VP]),
 }
 
 
@@ -204,3 +210,46 @@ def select_sum(cmp, val, col, result=None, stream=None):
     r = result or DeviceU64()
     check(lib().ms_select_sum(r.p, cmp, val, col.h, stream))
     return r
+
+
+class Plan:
+    """A query plan recorded on `stream` and replayed as a hipGraph.
+
+    with plan.record(stream): ...operators on `stream`...   (first: measure)
+    with plan.record(stream): ...same operators...          (then: capture)
+    plan.launch(stream)
+    """
+
+    def __init__(self, arena_bytes=0):
+        init()
+        self.h = VP()
+        check(lib().ms_plan_create(ctypes.byref(self.h), arena_bytes))
+
+    def record(self, stream):
+        plan = self
+
+        class _Rec:
+            def __enter__(self):
+                check(lib().ms_plan_begin(plan.h, stream))
+
+            def __exit__(self, *exc):
+                check(lib().ms_plan_end(plan.h, stream))
+                return False
+        return _Rec()
+
+    def launch(self, stream):
+        check(lib().ms_plan_launch(self.h, stream))
+
+    @property
+    def arena_bytes(self):
+        return lib().ms_plan_arena_bytes(self.h)
+
+    def free(self):
+        if self.h:
+            lib().ms_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.ms_plan_free(self.h)
+            self.h = None

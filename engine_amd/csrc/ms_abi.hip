@@ -24,20 +24,71 @@ int ms_fail(int code, const char *fmt, ...)
     return code;
 }
 
+// ------------------------------------------------------------------ plans
+// While a plan records (ms_plan_begin .. ms_plan_end) every device
+// allocation of the operators on the calling thread comes from the plan's
+// arena, so the captured hipGraph holds kernels and copies only and can be
+// replayed.  A plan created with arena_bytes == 0 first runs in measuring
+// mode: the operators run eagerly and the plan records the bytes they
+// allocate; the next ms_plan_begin captures with an arena of that size.
+struct ms_plan {
+    int mode;              // 0: measure, 1: capture
+    uint8_t *arena;
+    uint64_t arena_bytes;
+    uint64_t used;
+    uint64_t measured;
+    hipStream_t stream;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    bool recording;
+    std::vector<void *> deferred;   // frees of eager memory requested while capturing
+};
+
+static thread_local ms_plan *g_rec = nullptr;
+
+static uint64_t round256(uint64_t x) { return (x + 255) / 256 * 256; }
+
 namespace ms {
+
+void *dev_alloc(size_t bytes, hipStream_t s)
+{
+    bytes = round256(bytes ? bytes : 64);
+    if (g_rec && g_rec->recording && g_rec->mode == 1) {
+        if (g_rec->used + bytes > g_rec->arena_bytes) return nullptr;
+        void *p = g_rec->arena + g_rec->used;
+        g_rec->used += bytes;
+        return p;
+    }
+    void *p = nullptr;
+    if (hipMallocAsync(&p, bytes, s) != hipSuccess) return nullptr;
+    if (g_rec && g_rec->recording && g_rec->mode == 0) g_rec->measured += bytes;
+    return p;
+}
+
+static bool in_arena(const void *p)
+{
+    return g_rec && g_rec->arena && (const uint8_t *)p >= g_rec->arena &&
+           (const uint8_t *)p < g_rec->arena + g_rec->arena_bytes;
+}
+
+void dev_free(void *p, hipStream_t s)
+{
+    if (!p || in_arena(p)) return;
+    (void)hipFreeAsync(p, s);
+}
 
 void *scratch::get(size_t bytes)
 {
     if (n >= 64) return nullptr;
-    void *p = nullptr;
-    if (hipMallocAsync(&p, bytes ? bytes : 64, s) != hipSuccess) return nullptr;
+    void *p = dev_alloc(bytes, s);
+    if (!p) return nullptr;
     ptrs[n++] = p;
     return p;
 }
 
 scratch::~scratch()
 {
-    for (int i = 0; i < n; i++) (void)hipFreeAsync(ptrs[i], s);
+    for (int i = 0; i < n; i++) dev_free(ptrs[i], s);
 }
 
 int colv_of(const ms_col *c, ms_colv &v)
@@ -164,7 +215,7 @@ extern "C" int ms_init(int device)
     return MS_OK;
 }
 
-static int need_init()
+int ms_need_init()
 {
     if (g_device < 0) return ms_fail(MS_ERR_NODEVICE, "ms_init was not called");
     return MS_OK;
@@ -193,18 +244,19 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     const uint64_t idx_bytes = (c->index_cap * 8 + 255) / 256 * 256;
     const uint64_t refs_bytes = (c->refs_cap * 8 + 255) / 256 * 256;
     const uint64_t data_bytes = (size_alloc + 1024 + 255) / 256 * 256;
-    uint8_t *base = nullptr;
-    if (hipMallocAsync((void **)&base, 256 + idx_bytes + refs_bytes + data_bytes, s) != hipSuccess) {
+    uint8_t *base = (uint8_t *)dev_alloc(256 + idx_bytes + refs_bytes + data_bytes, s);
+    if (!base) {
         delete c;
         return ms_fail(MS_ERR_NOMEM, "device allocation of %llu bytes", (unsigned long long)(size_alloc + 1024));
     }
+    c->arena = g_rec && g_rec->recording && g_rec->mode == 1 ? g_rec : nullptr;
     c->meta = (ms_dev_meta *)base;
     c->index = c->index_cap ? (uint64_t *)(base + 256) : nullptr;
     c->refs = c->refs_cap ? (uint64_t *)(base + 256 + idx_bytes) : nullptr;
     c->data = base + 256 + idx_bytes + refs_bytes;
     int rc = launch_fill_u64((uint64_t *)base, 256 / 8, 0, s);
     if (rc) {
-        (void)hipFreeAsync(base, s);
+        dev_free(base, s);
         delete c;
         return rc;
     }
@@ -312,7 +364,7 @@ static int check_meta(const ms_format_desc &f, const ms_col_meta &m)
 extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *image, const ms_col_meta *meta,
                              void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !meta || (!image && meta->size_used)) return ms_fail(MS_ERR_INVALID, "null argument");
     if (!valid_format(f)) return ms_fail(MS_ERR_INVALID, "invalid format");
     MS_TRY(check_meta(*f, *meta));
@@ -414,8 +466,99 @@ extern "C" uint64_t ms_col_count_bound(const ms_col *c) { return c ? c->count_bo
 extern "C" int ms_col_free(ms_col *c)
 {
     if (!c) return MS_OK;
-    (void)hipFreeAsync((void *)c->meta, c->stream);
+    if (c->arena) {
+        // arena columns die with their plan
+    } else if (g_rec && g_rec->recording && g_rec->mode == 1) {
+        g_rec->deferred.push_back((void *)c->meta);   // a free now would be captured into the graph
+    } else {
+        (void)hipFreeAsync((void *)c->meta, c->stream);
+    }
     delete c;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ plans
+extern "C" int ms_plan_create(ms_plan **out, uint64_t arena_bytes)
+{
+    MS_TRY(ms_need_init());
+    if (!out) return ms_fail(MS_ERR_INVALID, "null argument");
+    ms_plan *p = new (std::nothrow) ms_plan();
+    if (!p) return ms_fail(MS_ERR_NOMEM, "host allocation");
+    p->arena_bytes = arena_bytes;
+    p->mode = arena_bytes ? 1 : 0;
+    if (arena_bytes && hipMalloc((void **)&p->arena, arena_bytes) != hipSuccess) {
+        delete p;
+        return ms_fail(MS_ERR_NOMEM, "plan arena of %llu bytes", (unsigned long long)arena_bytes);
+    }
+    *out = p;
+    return MS_OK;
+}
+
+extern "C" int ms_plan_begin(ms_plan *p, void *stream)
+{
+    if (!p || !stream) return ms_fail(MS_ERR_INVALID, "a plan records on a created (non-default) stream");
+    if (g_rec) return ms_fail(MS_ERR_INVALID, "another plan is recording on this thread");
+    if (p->mode == 0 && p->measured) {
+        // the measuring pass is done: allocate the arena and capture
+        p->arena_bytes = p->measured + (1u << 20);
+        if (hipMalloc((void **)&p->arena, p->arena_bytes) != hipSuccess)
+            return ms_fail(MS_ERR_NOMEM, "plan arena of %llu bytes", (unsigned long long)p->arena_bytes);
+        p->mode = 1;
+    }
+    if (p->exec) return ms_fail(MS_ERR_INVALID, "plan already captured");
+    p->stream = S(stream);
+    p->used = 0;
+    p->recording = true;
+    g_rec = p;
+    if (p->mode == 1) {
+        MS_HIP(hipDeviceSynchronize());
+        hipError_t e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            g_rec = nullptr;
+            p->recording = false;
+            return ms_fail(MS_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+        }
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_plan_end(ms_plan *p, void *stream)
+{
+    if (!p || g_rec != p || S(stream) != p->stream) return ms_fail(MS_ERR_INVALID, "plan is not recording here");
+    g_rec = nullptr;
+    p->recording = false;
+    if (p->mode == 0) {
+        MS_HIP(hipStreamSynchronize(p->stream));
+        return MS_OK;
+    }
+    hipError_t e = hipStreamEndCapture(p->stream, &p->graph);
+    for (void *q : p->deferred) (void)hipFreeAsync(q, p->stream);
+    p->deferred.clear();
+    if (e != hipSuccess) return ms_fail(MS_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    MS_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return MS_OK;
+}
+
+extern "C" int ms_plan_launch(ms_plan *p, void *stream)
+{
+    if (!p || !p->exec) return ms_fail(MS_ERR_INVALID, "plan has no captured graph");
+    MS_HIP(hipGraphLaunch(p->exec, S(stream)));
+    return MS_OK;
+}
+
+extern "C" uint64_t ms_plan_arena_bytes(const ms_plan *p) { return p ? (p->mode ? p->arena_bytes : p->measured) : 0; }
+
+extern "C" int ms_plan_free(ms_plan *p)
+{
+    if (!p) return MS_OK;
+    if (g_rec == p) g_rec = nullptr;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->arena) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(p->arena);
+    }
+    delete p;
     return MS_OK;
 }
 
@@ -558,7 +701,7 @@ static ms_format_desc uncompr_desc()
 // ------------------------------------------------------------------ morph
 extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !in) return ms_fail(MS_ERR_INVALID, "null argument");
     if (!valid_format(dst)) return ms_fail(MS_ERR_INVALID, "invalid destination format");
     scratch sc(S(stream));
@@ -597,7 +740,7 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
 extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_format_desc *out_f, const ms_col *in,
                          uint32_t lanes, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !in || cmp > MS_CMP_GE || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     scratch sc(S(stream));
@@ -622,7 +765,7 @@ extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_form
 
 extern "C" int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val, const ms_col *in, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!d_result || !in || cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "bad argument");
     return launch_select_sum(in, make_pred(cmp, val), d_result, false, S(stream));
 }
@@ -631,7 +774,7 @@ extern "C" int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val, con
 extern "C" int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_col *data, const ms_col *pos,
                           uint32_t lanes, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !data || !pos || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     scratch sc(S(stream));
@@ -683,7 +826,7 @@ __global__ void k_agg_meta(ms_dev_meta *m, uint64_t compr)
 
 extern "C" int ms_agg_sum(ms_col **out, const ms_col *in, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !in) return ms_fail(MS_ERR_INVALID, "bad argument");
     ms_col *c;
     MS_TRY(new_col(uncompr_desc(), 1, 64, S(stream), &c));
@@ -703,7 +846,7 @@ extern "C" int ms_agg_sum(ms_col **out, const ms_col *in, void *stream)
 extern "C" int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f, const ms_col *l, const ms_col *r,
                                    void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     scratch sc(S(stream));
@@ -722,7 +865,7 @@ extern "C" int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f, co
 extern "C" int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f, const ms_col *l, const ms_col *r,
                                void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     scratch sc(S(stream));
@@ -741,7 +884,7 @@ extern "C" int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f, const 
 extern "C" int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f, const ms_col *l,
                               const ms_col *r, void *stream)
 {
-    MS_TRY(need_init());
+    MS_TRY(ms_need_init());
     if (!out || !l || !r || op > MS_OP_XOR) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     ms_col_meta ml, mr;

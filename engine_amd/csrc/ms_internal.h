@@ -173,4 +173,5 @@ struct ms_col {
     uint64_t refs_cap;
     uint64_t count_bound;    // host-known upper bound of count
     hipStream_t stream;      // stream of creation (frees are ordered on it)
+    struct ms_plan *arena;   // non-null: memory belongs to this plan's arena
 };

@@ -24,6 +24,10 @@ struct scratch {
 
 int colv_of(const ms_col *c, ms_colv &v);
 
+// device memory of the operators (from a recording plan's arena if any)
+void *dev_alloc(size_t bytes, hipStream_t s);
+void dev_free(void *p, hipStream_t s);
+
 // decode side -------------------------------------------------------
 int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
 // *d_result (+)= sum of x_i with pred(x_i); the result is zeroed first
@@ -67,6 +71,7 @@ int launch_set_meta(ms_dev_meta *m, const uint64_t *d_count, uint64_t size_per_v
 }  // namespace ms
 
 int ms_fail(int code, const char *fmt, ...);
+int ms_need_init();   // MS_OK once ms_init succeeded
 
 #define MS_HIP(x)                                                                        \
     do {                                                                                 \

@@ -38,3 +38,22 @@ class Event:
 
 def synchronize():
     _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A created HIP stream (default priority)."""
+
+    def __init__(self):
+        self.s = ctypes.c_void_p()
+        _ok(hip().hipStreamCreate(ctypes.byref(self.s)), "hipStreamCreate")
+
+    def synchronize(self):
+        _ok(hip().hipStreamSynchronize(self.s), "hipStreamSynchronize")
+
+    @property
+    def handle(self):
+        return self.s
+
+    def __del__(self):
+        if self.s and _hip is not None:
+            _hip.hipStreamDestroy(self.s)

@@ -227,3 +227,46 @@ def test_config_sizes_round_trip_and_sums():
         col = ea.morph(f, ea.upload_values(v))
         assert np.array_equal(col.values(), v), repr(f)
         assert int(ea.select_sum(MS_CMP_LT, val, col).get()[0]) == rm.masked_sum(v, MS_CMP_LT, val), repr(f)
+
+
+# ---------------------------------------------------- plans (hipGraphs)
+def run_two_column_query(flag, key, measure, stream):
+    """select(flag==3) -> delta+dvbp positions; intersect with
+    select(key<2^18); project(measure); agg_sum."""
+    pos_f = default_delta_dynamic_vbp_f(8)
+    p1 = ea.select(MS_CMP_EQ, 3, pos_f, flag, 8, stream)
+    p2 = ea.select(MS_CMP_LT, 1 << 18, pos_f, key, 8, stream)
+    i = ea.intersect_sorted(pos_f, p1, p2, stream)
+    m = ea.project(uncompr_f(), measure, i, 8, stream)
+    return ea.agg_sum(m, stream), (p1, p2, i, m)
+
+
+def test_plan_graph_replays_match_eager(oracle):
+    from engine_amd import hip
+    n = 3 * 65536 + 1234
+    fl = datagen.zipf_flags(1, n)
+    ky = datagen.uniform_bits(2, n, 20)
+    ms = datagen.uniform_bits(3, n, 16)
+    f = dynamic_vbp_f(512, 2048, 8)
+    st = hip.Stream()
+    flag, key, measure = (ea.morph(f, ea.upload_values(x), st.handle) for x in (fl, ky, ms))
+    eager, _ = run_two_column_query(flag, key, measure, st.handle)
+    want = eager.values(st.handle).tolist()
+    sel = (fl == 3) & (ky < (1 << 18))
+    with np.errstate(over="ignore"):
+        assert want == [int(np.sum(ms[sel], dtype=np.uint64))]
+    plan = ea.Plan()
+    with plan.record(st.handle):                 # measuring pass
+        r0, keep0 = run_two_column_query(flag, key, measure, st.handle)
+    assert plan.arena_bytes > 0
+    with plan.record(st.handle):                 # capture
+        res, keep = run_two_column_query(flag, key, measure, st.handle)
+    for rep in range(4):
+        plan.launch(st.handle)
+        st.synchronize()
+        assert res.values(st.handle).tolist() == want, rep
+        # eager work between replays
+        other = int(ea.select_sum(MS_CMP_LT, 100, measure).get()[0])
+        assert other == rm.masked_sum(ms, MS_CMP_LT, 100)
+    del keep, res
+    plan.free()
