@@ -9,9 +9,19 @@ compressed on the GPU and resident in HBM before timing starts:
   (c) for_f<1024,8,dynamic_vbp_f<512,64,8>>, base U[2^40,2^41) + offsets < 2^16
   (d) rle_f, run lengths geometric(mean 64), run values U[0,2^20)
 A step runs SELECT SUM(x) WHERE x < c on each column (c gives about 50%
-selectivity) through the fused kernels (ms_select_sum).  `value` is the
-compressed bytes read per step (sum of sizeUsed) divided by the step time.
-Every result is checked bit-exactly against numpy before timing.
+selectivity).  Two executions of the same step are timed:
+  plain    one fused call per column (ms_select_sum: 2 kernels per column)
+  batched  the four columns as one batch (ms_batch_run: one persistent
+           kernel over a device work queue + a partial-sum fold)
+`value` is the batched step: compressed bytes read per step (sum of
+sizeUsed + 8 B per result) divided by the step time.
+
+Second workload (uneven columns, --c4-cols, default 4096): columns of
+log-uniform length in [10^3, 10^6], format uniform over static_vbp<b,8>,
+dynamic_vbp, delta+dynamic_vbp, for+dynamic_vbp and rle, one select+sum per
+column; timed per-column (eager, and replayed as one hipGraph) against one
+autotuned batch.  Every result of every path is checked bit-exactly
+against numpy before it is timed.
 """
 import argparse
 import json
@@ -24,6 +34,7 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "fused select+sum GB/s over compressed cols, %HBM peak; batched vs plain"
+CMP_LT = 2
 
 
 def log(*a):
@@ -60,14 +71,117 @@ def cpu_baseline(col_gpu, fmt, val, threads_all):
     ocol = orc.from_image(fmt, img, *meta)
     out = {}
     for threads in (1, threads_all):
-        orc.select_sum_stream(2, val, ocol, threads)
+        orc.select_sum_stream(CMP_LT, val, ocol, threads)
         ts = []
         for _ in range(5):
             t0 = time.perf_counter()
-            r = orc.select_sum_stream(2, val, ocol, threads)
+            r = orc.select_sum_stream(CMP_LT, val, ocol, threads)
             ts.append(time.perf_counter() - t0)
         out[threads] = (statistics.median(ts), r)
     return out, meta
+
+
+def timed(fn, steps, warmup, hip):
+    """Average ms of fn() over `steps` calls, bracketed by HIP events on the
+    default stream (every call below issues its work there)."""
+    for _ in range(warmup):
+        fn()
+    hip.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    return e0.elapsed_ms(e1) / steps
+
+
+def c4_workload(args, ea, hip, datagen):
+    """Uneven mixed-format columns: per-column calls vs one tuned batch."""
+    t0 = time.perf_counter()
+    cols, expect, bytes_used, nvals, kinds = [], [], 0, 0, {}
+    for kind, fmt, x, val in datagen.mixed_columns(datagen.BASE_SEED + 3, args.c4_cols):
+        col = ea.morph(fmt, ea.upload_values(x))
+        cols.append((col, val))
+        expect.append(masked_sum(x, val))
+        m = col.meta()
+        bytes_used += m.size_used
+        nvals += len(x)
+        kinds[kind] = kinds.get(kind, 0) + 1
+    nc = len(cols)
+    log(f"c4: {nc} columns, {nvals} values, {bytes_used} compressed bytes, prepared in "
+        f"{time.perf_counter() - t0:.1f}s")
+    res = ea.DeviceU64(nc)
+    slots = [res.at(k) for k in range(nc)]
+
+    def plain():
+        for k, (col, val) in enumerate(cols):
+            ea.select_sum(CMP_LT, val, col, slots[k])
+
+    def verify(tag):
+        got = res.get().tolist()
+        bad = [k for k in range(nc) if got[k] != expect[k]]
+        if bad:
+            log(f"c4 MISMATCH ({tag}) at entries {bad[:5]}")
+            sys.exit(1)
+
+    plain()
+    verify("plain")
+    ms_plain = timed(plain, max(2, args.steps // 10), 1, hip)
+    # the same per-column calls recorded once and replayed as one hipGraph
+    st = hip.Stream()
+    plan = ea.Plan()
+
+    def plain_on(stream):
+        for k, (col, val) in enumerate(cols):
+            ea.select_sum(CMP_LT, val, col, slots[k], stream)
+
+    with plan.record(st.handle):
+        plain_on(st.handle)
+    with plan.record(st.handle):
+        plain_on(st.handle)
+    res.zero()                 # the replay must write every result
+    plan.launch(st.handle)
+    st.synchronize()
+    verify("plain graph")
+    for _ in range(2):
+        plan.launch(st.handle)
+    st.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    reps = max(2, args.steps // 4)
+    e0.record(st.handle)
+    for _ in range(reps):
+        plan.launch(st.handle)
+    e1.record(st.handle)
+    ms_graph = e0.elapsed_ms(e1) / reps
+    plan.free()
+    # one batch, autotuned
+    b = ea.Batch()
+    for col, val in cols:
+        b.enqueue(col, CMP_LT, val)
+    res.zero()
+    b.run(res)
+    verify("batched default")
+    t1 = time.perf_counter()
+    cfg = b.tune(res, reps=3)
+    tune_s = time.perf_counter() - t1
+    b.run(res)
+    verify("batched tuned")
+    ms_batch = timed(lambda: b.run(res), args.steps, args.warmup, hip)
+    _, items, _ = b.stats()
+    gb = lambda ms: round((bytes_used + 8 * nc) / ms / 1e6, 1)  # noqa: E731
+    log(f"c4: plain {ms_plain:.3f} ms ({gb(ms_plain)} GB/s), plain graph {ms_graph:.3f} ms ({gb(ms_graph)} GB/s), "
+        f"batched {ms_batch:.3f} ms ({gb(ms_batch)} GB/s), tuned {cfg.as_dict()} in {tune_s:.1f}s")
+    return {
+        "workload": f"{nc} uint64 columns, lengths log-uniform in [10^3, 10^6], formats {kinds}, "
+                    f"select(x<c)+agg_sum per column",
+        "values": nvals, "compressed_bytes": bytes_used,
+        "plain_ms": round(ms_plain, 4), "plain_GBps": gb(ms_plain),
+        "plain_graph_ms": round(ms_graph, 4), "plain_graph_GBps": gb(ms_graph),
+        "batched_ms": round(ms_batch, 4), "batched_GBps": gb(ms_batch),
+        "batched_pct_hbm_peak": round(100 * gb(ms_batch) / HBM_PEAK_GBS, 2),
+        "speedup_vs_plain": round(ms_plain / ms_batch, 2), "speedup_vs_plain_graph": round(ms_graph / ms_batch, 2),
+        "work_items": items, "tuned_config": cfg.as_dict(), "tune_seconds": round(tune_s, 1),
+    }
 
 
 def main():
@@ -76,6 +190,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=27)
+    ap.add_argument("--c4-cols", type=int, default=4096, help="columns of the uneven workload (0: skip it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="only the layouts whose name starts with this (profiling)")
     args = ap.parse_args()
@@ -98,7 +213,7 @@ def main():
         col = ea.morph(fmt, ea.upload_values(x)) if fmt.kind else ea.upload_values(x)
         m = col.meta()
         expect = masked_sum(x, val)
-        got = int(ea.select_sum(2, val, col).get()[0])
+        got = int(ea.select_sum(CMP_LT, val, col).get()[0])
         if got != expect:
             log(f"MISMATCH {name}: gpu {got} numpy {expect}")
             sys.exit(1)
@@ -107,63 +222,72 @@ def main():
         log(f"{name}: n={n} sizeUsed={m.size_used} ({8 * n / m.size_used:.2f}x) prepared in "
             f"{time.perf_counter() - t0:.1f}s, sum verified")
         del x
-    res = ea.DeviceU64()
-    # per layout: average duration of one fused call
+    nc = len(cols)
+    res = ea.DeviceU64(nc)
+    slots = [res.at(k) for k in range(nc)]
+    # per layout: one fused call
     per = {}
-    for c in cols:
-        for _ in range(args.warmup):
-            ea.select_sum(2, c["val"], c["col"], res)
-        e0, e1 = hip.Event(), hip.Event()
-        e0.record()
-        for _ in range(args.steps):
-            ea.select_sum(2, c["val"], c["col"], res)
-        e1.record()
-        ms = e0.elapsed_ms(e1) / args.steps
+    for k, c in enumerate(cols):
+        ms = timed(lambda: ea.select_sum(CMP_LT, c["val"], c["col"], slots[k]), args.steps, args.warmup, hip)
         per[c["name"]] = ms
         log(f"{c['name']}: {ms * 1e3:.1f} us/call, {c['size_used'] / ms / 1e6:.1f} GB/s compressed, "
             f"{8 * n / ms / 1e6:.1f} GB/s uncompressed-equivalent")
-    # whole step: all four columns
-    for _ in range(args.warmup):
-        for c in cols:
-            ea.select_sum(2, c["val"], c["col"], res)
-    hip.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
+
+    def plain_step():
+        for k, c in enumerate(cols):
+            ea.select_sum(CMP_LT, c["val"], c["col"], slots[k])
+
+    ms_plain = timed(plain_step, args.steps, args.warmup, hip)
+    # batched: the same four select+sums as one persistent launch
+    batch = ea.Batch()
+    for c in cols:
+        batch.enqueue(c["col"], CMP_LT, c["val"])
+    batch.run(res)
+    if res.get().tolist() != [c["sum"] for c in cols]:
+        log("MISMATCH batched step")
+        sys.exit(1)
+    cfg = batch.tune(res, reps=5)
+    batch.run(res)
+    if res.get().tolist() != [c["sum"] for c in cols]:
+        log("MISMATCH batched step (tuned)")
+        sys.exit(1)
     wall0 = time.perf_counter()
-    e0.record()
-    for _ in range(args.steps):
-        for c in cols:
-            ea.select_sum(2, c["val"], c["col"], res)
-    e1.record()
-    ms_step = e0.elapsed_ms(e1) / args.steps
+    ms_step = timed(lambda: batch.run(res), args.steps, args.warmup, hip)
     wall = time.perf_counter() - wall0
     total_bytes = sum(c["size_used"] + 8 for c in cols)
     value = total_bytes / ms_step / 1e6
-    dom = cols[0]
-    dom_ms = per[dom["name"]]
-    achieved = (dom["size_used"] + 8) / dom_ms / 1e6
+    value_plain = total_bytes / ms_plain / 1e6
+    log(f"step: plain {ms_plain * 1e3:.1f} us ({value_plain:.1f} GB/s), batched {ms_step * 1e3:.1f} us "
+        f"({value:.1f} GB/s), tuned {cfg.as_dict()}")
     traffic = None
-    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_select_sum_dvbp.json")
+    tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_batch_select_sum.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("bytes_per_launch")
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": "format cascade sweep, fused select(x<c)+agg_sum, 4 uint64 columns of 2^%d "
-                               "values resident in HBM" % args.log2n,
+        "config": {"workload": "format cascade sweep, fused select(x<c)+agg_sum over %d uint64 columns of 2^%d "
+                               "values resident in HBM, one batched persistent launch per step" % (nc, args.log2n),
                    "columns": {c["name"]: {"size_used": c["size_used"], "ratio": round(8 * n / c["size_used"], 3),
-                                           "us_per_call": round(per[c["name"]] * 1e3, 2),
-                                           "GBps_compressed": round(c["size_used"] / per[c["name"]] / 1e6, 1),
-                                           "GBps_uncompr_equiv": round(8 * n / per[c["name"]] / 1e6, 1),
-                                           "input_hash": c["hash"]} for c in cols}},
+                                           "plain_us_per_call": round(per[c["name"]] * 1e3, 2),
+                                           "plain_GBps_compressed": round(c["size_used"] / per[c["name"]] / 1e6, 1),
+                                           "input_hash": c["hash"]} for c in cols},
+                   "batch_config": cfg.as_dict()},
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
-        "uncompr_equiv_GBps": round(4 * 8 * n / ms_step / 1e6, 1),
+        "plain": {"ms_per_step": round(ms_plain, 4), "value": round(value_plain, 1),
+                  "pct_hbm_peak": round(100 * value_plain / HBM_PEAK_GBS, 2)},
+        "batched_vs_plain": round(ms_plain / ms_step, 3),
+        "uncompr_equiv_GBps": round(nc * 8 * n / ms_step / 1e6, 1),
         "wall_s_timed": round(wall, 3),
-        "roofline": {"bound": "hbm", "kernel": "k_select_sum_blocks (dynamic_vbp<512,2048,8>)",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "roofline": {"bound": "hbm", "kernel": "k_batch_select_sum (+ k_batch_finish)",
+                     "achieved": round(value, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
-    if not args.no_cpu:
+    if args.c4_cols > 0 and not args.only:
+        out["uneven_columns"] = c4_workload(args, ea, hip, datagen)
+    if not args.no_cpu and cols:
+        dom = cols[0]
         threads_all = len(os.sched_getaffinity(0))
         r, meta = cpu_baseline(dom["col"], dom["fmt"], dom["val"], threads_all)
         assert r[1][1] == dom["sum"] and r[threads_all][1] == dom["sum"], "CPU baseline result mismatch"

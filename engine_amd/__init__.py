@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-from .formats import ColMeta, FormatDesc  # noqa: F401
+from .formats import BatchConfig, ColMeta, FormatDesc  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # MS_LIB_PATH selects another build of the same library (e.g. a tuning variant)
@@ -53,6 +53,26 @@ ABI = {
     "ms_plan_launch": (ctypes.c_int, [VP, VP]),
     "ms_plan_arena_bytes": (U64, [VP]),
     "ms_plan_free": (ctypes.c_int, [VP]),
+    "ms_batch_create": (ctypes.c_int, [_COLP]),
+    "ms_batch_enqueue": (ctypes.c_int, [VP, VP, U32, U64, P(U64)]),
+    "ms_batch_run": (ctypes.c_int, [VP, VP, VP]),
+    "ms_batch_tune": (ctypes.c_int, [VP, VP, U32, P(BatchConfig), VP]),
+    "ms_batch_get_config": (ctypes.c_int, [VP, P(BatchConfig)]),
+    "ms_batch_set_config": (ctypes.c_int, [VP, P(BatchConfig)]),
+    "ms_batch_stats": (ctypes.c_int, [VP, P(U64), P(U64), P(U64)]),
+    "ms_batch_free": (ctypes.c_int, [VP]),
+    "ms_gen_chunks": (ctypes.c_int, [VP, VP, U64, VP, U32, VP]),
+    "ms_chunk_stats": (ctypes.c_int, [VP, U64, VP, VP, VP, VP]),
+    "ms_chunk_features": (ctypes.c_int, [VP, U64, VP, VP, VP]),
+    "ms_candidate_format": (ctypes.c_int, [U32, U32, _FD]),
+    "ms_mlp_create": (ctypes.c_int, [_COLP, U64]),
+    "ms_mlp_get_params": (ctypes.c_int, [VP, VP]),
+    "ms_mlp_set_params": (ctypes.c_int, [VP, VP]),
+    "ms_mlp_loss_grad": (ctypes.c_int, [VP, VP, VP, VP, U64, VP, VP, VP]),
+    "ms_mlp_train": (ctypes.c_int, [VP, VP, VP, VP, U64, U32, U32, ctypes.c_float, U64, VP, VP]),
+    "ms_mlp_predict": (ctypes.c_int, [VP, VP, U64, VP, VP]),
+    "ms_mlp_free": (ctypes.c_int, [VP]),
+    "ms_select_formats": (ctypes.c_int, [VP, VP, U64, VP, VP, VP]),
 }
 
 
@@ -188,9 +208,22 @@ class DeviceU64:
             DeviceU64._hip = ctypes.CDLL("libamdhip64.so")
         self.n = n
         self.p = VP()
+        self._owner = None
         rc = DeviceU64._hip.hipMalloc(ctypes.byref(self.p), ctypes.c_size_t(8 * n))
         if rc != 0:
             raise MsError(3, f"hipMalloc failed ({rc})")
+
+    def at(self, k):
+        """A view of element k (for ms_select_sum's result argument)."""
+        v = object.__new__(DeviceU64)
+        v.n, v.p, v._owner = 1, VP(self.p.value + 8 * k), self
+        return v
+
+    def zero(self):
+        DeviceU64._hip.hipDeviceSynchronize()
+        rc = DeviceU64._hip.hipMemset(self.p, 0, ctypes.c_size_t(8 * self.n))
+        if rc != 0:
+            raise MsError(3, f"hipMemset failed ({rc})")
 
     def get(self):
         out = np.zeros(self.n, np.uint64)
@@ -201,7 +234,7 @@ class DeviceU64:
         return out
 
     def __del__(self):
-        if self.p and DeviceU64._hip is not None:
+        if self.p and DeviceU64._hip is not None and getattr(self, "_owner", None) is None:
             DeviceU64._hip.hipFree(self.p)
 
 
@@ -252,4 +285,57 @@ class Plan:
     def __del__(self):
         if self.h and _lib is not None:
             _lib.ms_plan_free(self.h)
+            self.h = None
+
+
+class Batch:
+    """Fused select + sum over many columns in one persistent launch
+    (ms_batch_*).  Keeps the enqueued columns alive."""
+
+    def __init__(self):
+        init()
+        self.h = VP()
+        check(lib().ms_batch_create(ctypes.byref(self.h)))
+        self.cols = []
+
+    def enqueue(self, col, cmp, val):
+        e = U64()
+        check(lib().ms_batch_enqueue(self.h, col.h, cmp, val, ctypes.byref(e)))
+        self.cols.append(col)
+        return e.value
+
+    def run(self, results, stream=None):
+        check(lib().ms_batch_run(self.h, results.p, stream))
+
+    def tune(self, results, reps=5, stream=None):
+        c = BatchConfig()
+        check(lib().ms_batch_tune(self.h, results.p, reps, ctypes.byref(c), stream))
+        return c
+
+    @property
+    def config(self):
+        c = BatchConfig()
+        check(lib().ms_batch_get_config(self.h, ctypes.byref(c)))
+        return c
+
+    @config.setter
+    def config(self, c):
+        check(lib().ms_batch_set_config(self.h, ctypes.byref(c)))
+
+    def stats(self):
+        e, i, b = U64(), U64(), U64()
+        check(lib().ms_batch_stats(self.h, ctypes.byref(e), ctypes.byref(i), ctypes.byref(b)))
+        return e.value, i.value, b.value
+
+    def __len__(self):
+        return len(self.cols)
+
+    def free(self):
+        if self.h:
+            lib().ms_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        if self.h and _lib is not None:
+            _lib.ms_batch_free(self.h)
             self.h = None

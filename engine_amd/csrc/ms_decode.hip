@@ -47,17 +47,19 @@ __global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, 
     uint64_t acc = 0;
     uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
     uint64_t e_next = u < n ? uniform64(c.index[u >> lg_spb]) : 0ull;
+    uint64_t r_next = IS_FOR && u < n ? uniform64(c.refs[(u >> lg_spb) >> lg_ipo]) : 0ull;
     for (; u < n; u += W) {
         const uint64_t e = e_next;
-        if (u + W < n) e_next = uniform64(c.index[(u + W) >> lg_spb]);
-        const uint64_t b = u >> lg_spb;
+        uint64_t ref = r_next;
+        if (u + W < n) {
+            e_next = uniform64(c.index[(u + W) >> lg_spb]);
+            if (IS_FOR) r_next = uniform64(c.refs[((u + W) >> lg_spb) >> lg_ipo]);
+        }
         const uint32_t bw = (uint32_t)(e & 0xff);
         const uint64_t *base = (const uint64_t *)(c.data + (e >> 8));
         const uint32_t g = (uint32_t)(u & ((1ull << lg_spb) - 1)) * 8 + gl;
-        uint64_t ref = 0;
         uint32_t mode, lo, span;
         if (IS_FOR) {
-            ref = uniform64(c.refs[b >> lg_ipo]);
             const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
             if (bw > 32 || ref + maxcode < ref) {
                 // wide or wrapping codes (rare): value by value on x itself
@@ -255,6 +257,22 @@ __global__ __launch_bounds__(WG) void k_delta_tiles(ms_colv c, const uint64_t *a
         }
     }
     if (MODE == 1) wg_partial(acc, part);
+}
+
+// Select + sum over a delta_f column with 8 lanes and indexed 8-lane
+// inner blocks: one wave per tile, no workgroup synchronisation.
+__global__ __launch_bounds__(WG) void k_delta_select_sum(ms_colv c, ms_pred p, uint64_t *part)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t nunits = pt.n_compr >> 9;
+    const uint64_t ntiles = (nunits + 3) >> 2;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t & 7, gl = t >> 3;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    uint64_t acc = 0;
+    for (uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index(); tile < ntiles; tile += W)
+        delta_tile_ss(c, p, tile, nunits, t, j, gl, acc);
+    wg_partial(acc, part);
 }
 
 // Per-tile lane totals, lane-major: sums[j * ntb + tile].
@@ -595,6 +613,11 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     case MS_DELTA: {
         const uint64_t *arr;
         MS_TRY(delta_src(in, v, sc, &arr));
+        if (!arr && v.g.lg_spb != 0xff) {
+            grid = grid_for(nb / DELTA_TILE + 1, WAVES * 2ull, 2048);
+            hipLaunchKernelGGL(k_delta_select_sum, dim3(grid), dim3(WG), 0, s, v, p, part);
+            break;
+        }
         grid = grid_for(nb / DELTA_TILE + 1, 2, 2048);
         if (arr) hipLaunchKernelGGL((k_delta_tiles<1, 1>), dim3(grid), dim3(WG), 0, s, v, arr, nullptr, p, part);
         else hipLaunchKernelGGL((k_delta_tiles<1, 0>), dim3(grid), dim3(WG), 0, s, v, arr, nullptr, p, part);

@@ -243,78 +243,6 @@ __device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint3
     if (CNT) acc += (uint64_t)cnt * ref;
 }
 
-// Split load / compute of one unit with a compile-time bit width: the load
-// switch fills w[0 .. NW(BW)) of a 5-word register array, so a wave can
-// issue the loads of several units before it decodes the first.
-template <int BW>
-__device__ __forceinline__ void ss_load(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint64_t (&w)[5])
-{
-    constexpr int NW = vbp_traits<BW>::NW;
-    const uint32_t byte0 = g * BW;
-    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
-#pragma unroll
-    for (int q = 0; q < NW; q++) w[q] = p[q * 8];   // may over-read one word; buffers are padded
-}
-
-template <int BW, bool CNT>
-__device__ __forceinline__ void ss_comp(const uint64_t (&w)[5], uint32_t g, uint32_t mode, uint32_t lo,
-                                        uint32_t span, uint64_t ref, uint64_t &acc)
-{
-    const uint32_t sh = ((g * BW) & 7) * 8;
-    uint32_t c[8];
-    vbp_cut32<BW>(w, sh, c);
-    using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
-    A s = 0;
-    uint32_t cnt = 0;
-    if (mode == SM_ALL) {
-#pragma unroll
-        for (int m = 0; m < 8; m++) s += c[m];
-        cnt = 8;
-    } else if (mode == SM_LE) {
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const bool take = c[m] <= span;
-            s += take ? c[m] : 0u;
-            if (CNT) cnt += take;
-        }
-    } else {
-        const bool out = mode == SM_OUT;
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-            const bool take = ((uint32_t)(c[m] - lo) <= span) != out;
-            s += take ? c[m] : 0u;
-            if (CNT) cnt += take;
-        }
-    }
-    acc += (uint64_t)s;
-    if (CNT) acc += (uint64_t)cnt * ref;
-}
-
-__device__ __forceinline__ void ss_load_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
-                                                 uint64_t (&w)[5])
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: ss_load<B>(base, j, g, w); break;
-        MS_BW32_CASES(MS_CASE)
-#undef MS_CASE
-    default: break;
-    }
-}
-
-template <bool CNT>
-__device__ __forceinline__ void ss_comp_dispatch(uint32_t bw, const uint64_t (&w)[5], uint32_t g, uint32_t mode,
-                                                 uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
-{
-    switch (bw) {
-#define MS_CASE(B) \
-    case B: ss_comp<B, CNT>(w, g, mode, lo, span, ref, acc); break;
-        MS_BW32_CASES(MS_CASE)
-#undef MS_CASE
-    default: break;
-    }
-}
-
 // Per-bit-width specialisation of a predicate for columns whose values are
 // their codes (no reference value): built on the host once per call.
 struct ms_bwtab {
@@ -322,31 +250,6 @@ struct ms_bwtab {
     uint32_t lo[33];
     uint32_t span[33];
 };
-
-// A unit's payload (64*bw bytes, bw <= 32) fetched with contiguous 16-byte
-// loads per lane -- one KiB per wave instruction -- into registers, then
-// staged in the wave's LDS slice where the lane-interleaved words are cut.
-struct unit_regs {
-    u64x2 r[2];
-};
-
-__device__ __forceinline__ void unit_fetch(const uint8_t *payload, uint32_t bytes, uint32_t t, unit_regs &R)
-{
-#pragma unroll
-    for (int k = 0; k < 2; k++)
-        if ((uint32_t)k * 1024 + t * 16 < bytes)
-            R.r[k] = __builtin_nontemporal_load((const u64x2 *)(payload + k * 1024 + t * 16));
-}
-
-__device__ __forceinline__ void unit_stage(uint64_t *lw, uint32_t bytes, uint32_t t, const unit_regs &R)
-{
-#pragma unroll
-    for (int k = 0; k < 2; k++)
-        if ((uint32_t)k * 1024 + t * 16 < bytes) *(u64x2 *)(lw + k * 128 + 2 * t) = R.r[k];
-    // LDS instructions of one wave execute in order: the reads of the other
-    // lanes below see these writes; only compiler reordering must be stopped
-    __builtin_amdgcn_wave_barrier();
-}
 
 template <bool CNT>
 __device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
@@ -359,6 +262,142 @@ __device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *ba
         MS_BW32_CASES(MS_CASE)
 #undef MS_CASE
     default: break;
+    }
+}
+
+// ------------------------------------------------------------------
+// delta_f select + sum, one wave per unit of deltas with the lanes'
+// running sums ("carries") at the unit's start.  Deltas of up to 26 bits
+// are decoded and scanned in 32-bit arithmetic (64 codes of a lane sum to
+// less than 2^32); the predicate is specialised per thread around the
+// thread's first running sum, so the 8 values cost 32-bit compares only.
+// ------------------------------------------------------------------
+template <int BW>
+__device__ __forceinline__ void dec32_unit(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t (&c)[8])
+{
+    constexpr int NW = vbp_traits<BW>::NW;
+    const uint32_t byte0 = g * BW;
+    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
+    uint64_t w[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++) w[q] = p[q * 8];   // may over-read one word; buffers are padded
+    vbp_cut32<BW>(w, (byte0 & 7) * 8, c);
+}
+
+#define MS_BW26_CASES(X)                                                                                            \
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
+    X(21) X(22) X(23) X(24) X(25) X(26)
+
+__device__ __forceinline__ void dec32_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
+                                               uint32_t (&c)[8])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: dec32_unit<B>(base, j, g, c); break;
+        MS_BW26_CASES(MS_CASE)
+#undef MS_CASE
+    default:
+#pragma unroll
+        for (int m = 0; m < 8; m++) c[m] = 0;
+        break;
+    }
+}
+
+// Deltas of more than 26 bits, in 64-bit arithmetic: rare (unsorted
+// data), kept out of line so it does not set the callers' register budget.
+// Returns {select + sum of the thread's 8 values, the lane's new carry}.
+__device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide(const ms_pred p, const uint64_t *base, uint32_t bw,
+                                                                  uint32_t j, uint32_t g, uint32_t t, uint64_t carry)
+{
+    uint64_t d[8];
+    dec_dispatch(bw, base, 8, j, g, d);
+#pragma unroll
+    for (int m = 1; m < 8; m++) d[m] += d[m - 1];
+    const uint64_t tot = d[7];
+    uint64_t incl = tot;
+#pragma unroll
+    for (uint32_t o = 8; o < 64; o <<= 1) {
+        const uint64_t v = __shfl_up(incl, o, MS_WAVE);
+        if (t >= o) incl += v;
+    }
+    const uint64_t bt = carry + incl - tot;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const uint64_t x = bt + d[m];
+        acc += pred_eval(p, x) ? x : 0ull;
+    }
+    u64x2 r;
+    r.x = acc;
+    r.y = carry + __shfl(incl, 56 + j, MS_WAVE);
+    return r;
+}
+
+// One 512-value unit of an 8-lane delta_f column: `base` is the unit's
+// block, g = the thread's code group in it, t = lane id in the wave
+// (t = 8 * (g % 8) + j).  carry holds lane j's running sum before the unit
+// and is advanced past it.
+__device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *base, uint32_t bw, uint32_t j,
+                                              uint32_t g, uint32_t t, uint64_t &carry, uint64_t &acc)
+{
+    if (bw <= 26) {
+        uint32_t d[8];
+        dec32_dispatch(bw, base, j, g, d);
+#pragma unroll
+        for (int m = 1; m < 8; m++) d[m] += d[m - 1];
+        const uint32_t tot = d[7];
+        uint32_t incl = tot;
+#pragma unroll
+        for (uint32_t o = 8; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, MS_WAVE);
+            if (t >= o) incl += v;
+        }
+        const uint64_t bt = carry + (incl - tot);   // x_m = bt + d[m]
+        if (bt + tot >= bt) {
+            const ms_cpred cp = code_pred(p, bt, tot);
+            uint32_t lo = 0, span = 0xffffffffu;
+            bool out = false;
+            if (cp.mode == PM_NONE) { lo = 0xffffffffu; span = 0; }
+            else if (cp.mode != PM_ALL) { lo = (uint32_t)cp.clo; span = (uint32_t)cp.cspan; out = cp.mode == PM_OUT; }
+            uint32_t s = 0, cnt = 0;
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const bool take = ((d[m] - lo) <= span) != out;
+                s += take ? d[m] : 0u;
+                cnt += take;
+            }
+            acc += (uint64_t)s + (uint64_t)cnt * bt;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                const uint64_t x = bt + d[m];
+                acc += pred_eval(p, x) ? x : 0ull;
+            }
+        }
+        carry += __shfl(incl, 56 + j, MS_WAVE);
+        return;
+    }
+    const u64x2 r = delta_unit_wide(p, base, bw, j, g, t, carry);
+    acc += r.x;
+    carry = r.y;
+}
+
+// A delta tile (MS_DELTA_TILE = 4 units) of a column with 512-value-
+// multiple inner blocks; units at or past `nunits` are skipped.
+__device__ __forceinline__ void delta_tile_ss(const ms_colv &c, const ms_pred &p, uint64_t tile, uint64_t nunits,
+                                              uint32_t t, uint32_t j, uint32_t gl, uint64_t &acc)
+{
+    uint64_t carry = c.refs[tile * 8 + j];
+    const uint32_t lg_spb = c.g.lg_spb;
+    const uint64_t mask = (1ull << lg_spb) - 1ull;
+    const uint64_t u_end = nunits < tile * 4 + 4 ? nunits : tile * 4 + 4;
+    uint64_t e_next = tile * 4 < u_end ? uniform64(c.index[(tile * 4) >> lg_spb]) : 0ull;
+#pragma unroll 1
+    for (uint64_t u = tile * 4; u < u_end; u++) {
+        const uint64_t e = e_next;
+        if (u + 1 < u_end) e_next = uniform64(c.index[(u + 1) >> lg_spb]);
+        delta_unit_ss(p, (const uint64_t *)(c.data + (e >> 8)), (uint32_t)(e & 0xff), j,
+                      (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
     }
 }
 

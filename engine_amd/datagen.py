@@ -13,6 +13,8 @@ Workloads (uint64 columns):
   sorted_geometric   x_0 = 0, x_i = x_{i-1} + g, g ~ geometric(p = 0.1) >= 1
   for_base_offset    x = B + (r >> 48), B ~ U[2^40, 2^41) drawn once
   rle_runs           run lengths ~ geometric(mean 64), values U[0, 2^20)
+  mixed_columns      columns of log-uniform length, each in one of five
+                     formats with the matching value distribution above
 """
 import hashlib
 
@@ -100,3 +102,38 @@ def zipf_flags(seed, n, k=8, s=1.1):
     cdf = np.cumsum(w / w.sum())
     u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     return np.searchsorted(cdf, u, side="right").clip(0, k - 1).astype(np.uint64)
+
+
+MIXED_KINDS = ("static_vbp", "dynamic_vbp", "delta", "for", "rle")
+
+
+def mixed_columns(seed, ncols, lo_exp=3.0, hi_exp=6.0):
+    """Yields (kind, format, values, select value) for `ncols` columns:
+    length floor(10^(lo_exp + (hi_exp - lo_exp) u)), format uniform over
+    static_vbp<b, 8> (b ~ U{1..32}), dynamic_vbp<512,2048,8>,
+    delta<1024,8,dynamic_vbp<512,64,8>>, for<1024,8,dynamic_vbp<512,64,8>>
+    and rle, values drawn from that format's generator; the select value
+    (for x < value) passes about half of the values."""
+    from .formats import (default_delta_dynamic_vbp_f, default_for_dynamic_vbp_f, dynamic_vbp_f, rle_f,
+                          static_vbp_f)
+    r = splitmix64(seed ^ 0xC4, 3 * ncols).reshape(ncols, 3)
+    for c in range(ncols):
+        u = float(r[c, 0] >> np.uint64(11)) / 9007199254740992.0
+        n = int(10.0 ** (lo_exp + (hi_exp - lo_exp) * u))
+        kind = MIXED_KINDS[int(r[c, 1] % np.uint64(5))]
+        s = seed + 0x1000 * (c + 1)
+        if kind == "static_vbp":
+            b = int(r[c, 2] % np.uint64(32)) + 1
+            yield kind, static_vbp_f(b, 8), uniform_bits(s, n, b), 1 << (b - 1)
+        elif kind == "dynamic_vbp":
+            nb = n // 512 * 512
+            x = np.concatenate([dvbp_random_bw(s, nb), uniform_bits(s + 1, n - nb, 16)])
+            yield kind, dynamic_vbp_f(512, 2048, 8), x, 1 << 16
+        elif kind == "delta":
+            x = sorted_geometric(s, n)
+            yield kind, default_delta_dynamic_vbp_f(8), x, int(x[n // 2]) if n else 1
+        elif kind == "for":
+            x = for_base_offset(s, n)
+            yield kind, default_for_dynamic_vbp_f(8), x, (int(x.min()) if n else 0) + (1 << 15)
+        else:
+            yield kind, rle_f(), rle_runs(s, n), 1 << 19

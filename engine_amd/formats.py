@@ -59,6 +59,15 @@ class ColMeta(ctypes.Structure):
                 ("size_compr", ctypes.c_uint64), ("size_alloc", ctypes.c_uint64)]
 
 
+class BatchConfig(ctypes.Structure):
+    """ms_batch_config."""
+    _fields_ = [(n, ctypes.c_uint32) for n in ("wg_per_cu", "units_per_item", "tiles_per_item", "runs_per_item",
+                                                "values_per_item", "unit_cost")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def uncompr_f():
     return FormatDesc(MS_UNCOMPR)
 

@@ -233,6 +233,179 @@ int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f,
 int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f,
                    const ms_col *l, const ms_col *r, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* Query plans as hipGraphs (no reference counterpart: a query in the   */
+/* reference is a chain of operator calls, select_sum_benchmark.cpp:    */
+/* 78-118; here such a chain is recorded once and replayed)            */
+/* ------------------------------------------------------------------ */
+typedef struct ms_plan ms_plan;
+
+/* A plan with an arena of `arena_bytes` device bytes; with 0 the plan first
+ * measures: the operators issued between the first ms_plan_begin/end pair
+ * run eagerly and the plan records the device memory they allocate; the
+ * next begin/end pair then captures them into a hipGraph. */
+int ms_plan_create(ms_plan **out, uint64_t arena_bytes);
+
+/* Starts recording the operators issued on `stream` (a created stream, not
+ * the default one) by the calling thread.  While capturing, their output
+ * columns and temporaries live in the plan's arena (ms_col_free of such a
+ * column releases only the handle) and no call may synchronise (ms_col_*
+ * meta/download calls and ms_calc_binary are not allowed). */
+int ms_plan_begin(ms_plan *p, void *stream);
+
+/* Ends recording; after a capturing pass, instantiates the graph. */
+int ms_plan_end(ms_plan *p, void *stream);
+
+/* Replays the captured operators on `stream`; the output columns created
+ * while capturing hold the results of the latest replay. */
+int ms_plan_launch(ms_plan *p, void *stream);
+
+/* Arena size (bytes measured so far in measuring mode). */
+uint64_t ms_plan_arena_bytes(const ms_plan *p);
+
+/* Releases the graph and the arena (and with it the plan's columns). */
+int ms_plan_free(ms_plan *p);
+
+/* ------------------------------------------------------------------ */
+/* Batched select + sum over many columns in one persistent launch (no  */
+/* reference counterpart: the reference issues one operator call per    */
+/* column and query, select_sum_benchmark.cpp:78-118)                   */
+/* ------------------------------------------------------------------ */
+typedef struct ms_batch ms_batch;
+
+/* Launch and work-queue shape of a batch; picked by ms_batch_tune. */
+typedef struct ms_batch_config {
+    uint32_t wg_per_cu;        /* persistent workgroups of 256 threads per CU    */
+    uint32_t units_per_item;   /* 512-value units per work item (bit-packed)     */
+    uint32_t tiles_per_item;   /* 2048-value tiles per work item (delta_f)       */
+    uint32_t runs_per_item;    /* runs per work item (rle_f)                     */
+    uint32_t values_per_item;  /* values per work item (uncompressed parts)      */
+    uint32_t unit_cost;        /* modelled fixed cost of a unit, in bytes        */
+} ms_batch_config;
+
+int ms_batch_create(ms_batch **out);
+
+/* Adds SELECT SUM(x) WHERE cmp(x, val) over `in` (any format; delta_f needs
+ * step 8 over an 8-lane dynamic_vbp inner format) as entry *entry, the
+ * index of its result.  `in` must stay alive while the batch is used. */
+int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uint64_t val, uint64_t *entry);
+
+/* d_results[e] = the sum of entry e, for every entry: one persistent kernel
+ * walks the device work queue, a second one folds the per-item partial
+ * sums.  The first run after an enqueue builds the queue (it reads the
+ * columns' meta data and synchronises `stream` once); later runs only
+ * launch the two kernels and may be recorded into a plan. */
+int ms_batch_run(ms_batch *b, uint64_t *d_results, void *stream);
+
+/* Autotuner: times `reps` runs per candidate shape (coordinate descent over
+ * the fields of ms_batch_config), keeps the fastest and returns it. */
+int ms_batch_tune(ms_batch *b, uint64_t *d_results, uint32_t reps, ms_batch_config *best, void *stream);
+
+int ms_batch_get_config(const ms_batch *b, ms_batch_config *cfg);
+int ms_batch_set_config(ms_batch *b, const ms_batch_config *cfg);
+
+/* Entries, work items of the current queue and the bytes one run reads
+ * (the sum of sizeUsed over the entries). */
+int ms_batch_stats(const ms_batch *b, uint64_t *entries, uint64_t *items, uint64_t *bytes);
+
+int ms_batch_free(ms_batch *b);
+
+/* ------------------------------------------------------------------ */
+/* Learned per-chunk format selection (replaces the calibration-driven  */
+/* choice of src/calibration and the exhaustive size search of          */
+/* exhaustive_size_format_selector.h:35-141 for chunks of 2^16 values)  */
+/* ------------------------------------------------------------------ */
+#define MS_CHUNK_VALUES 65536u   /* values per chunk                           */
+#define MS_GEO_LEN 1024u         /* entries of a geometric threshold table     */
+#define MS_NCAND 5               /* candidate formats                          */
+#define MS_NFEAT 24              /* features per chunk                         */
+#define MS_MLP_IN 24
+#define MS_MLP_HIDDEN 64
+#define MS_MLP_OUT 10            /* log2 size and log2 decode time per candidate */
+#define MS_MLP_NPARAM (24 * 64 + 64 + 64 * 64 + 64 + 64 * 10 + 10)
+
+/* Candidates: static_vbp<max bit width, 8>, dynamic_vbp<512, 2048, 8>,
+ * delta<1024, 8, dynamic_vbp<512, 64, 8>>, for<1024, 8, dynamic_vbp<512,
+ * 64, 8>> and rle. */
+enum ms_candidate { MS_CAND_STATIC = 0, MS_CAND_DVBP = 1, MS_CAND_DELTA = 2, MS_CAND_FOR = 3, MS_CAND_RLE = 4 };
+
+/* Synthetic chunk kinds of the label generator. */
+enum ms_gen_kind {
+    MS_GEN_UNIFORM = 0,   /* top a bits of a random word                                    */
+    MS_GEN_DVBP = 1,      /* per 512 values a bit width in [a, b], top bit forced once       */
+    MS_GEN_SORTED = 2,    /* base + running sum of geometric gaps (table), first gap 0        */
+    MS_GEN_FOR = 3,       /* base + top a bits                                               */
+    MS_GEN_RLE = 4        /* runs of geometric length (table), values = top b bits           */
+};
+
+typedef struct ms_chunk_spec {
+    uint32_t kind, a, b, table;   /* table: index into the geometric tables */
+    uint64_t seed, base;
+} ms_chunk_spec;
+
+/* Integer statistics per chunk (MS_ST_COUNT uint64 each). */
+enum ms_stat {
+    MS_ST_SUM_BW = 0, MS_ST_MAX_BW, MS_ST_SUM_DBW, MS_ST_MAX_DBW, MS_ST_SUM_FBW, MS_ST_MAX_FBW, MS_ST_RUNS,
+    MS_ST_NONDEC, MS_ST_ZEROS, MS_ST_MIN, MS_ST_MAX, MS_ST_SUM_BW2, MS_ST_DLESS, MS_ST_FLESS, MS_ST_HIST,
+    MS_ST_RESERVED, MS_ST_COUNT
+};
+
+/* Writes nchunks chunks of MS_CHUNK_VALUES values to d_out (device) from
+ * host specs; geo_tables (host) holds ntables threshold tables. */
+int ms_gen_chunks(uint64_t *d_out, const ms_chunk_spec *specs, uint64_t nchunks,
+                  const uint64_t *geo_tables, uint32_t ntables, void *stream);
+
+/* Per chunk of d_values (device, nchunks * MS_CHUNK_VALUES values): the
+ * MS_ST_* statistics into d_stats; with a host cost table (MS_NCAND x 65
+ * floats: ns per 512-value unit by bit width for static/dvbp/delta/for;
+ * rle: [0] fixed ns, [1] ns per run) also decode-time labels (MS_NCAND
+ * floats per chunk) into d_time_labels. */
+int ms_chunk_stats(const uint64_t *d_values, uint64_t nchunks, uint64_t *d_stats,
+                   const float *cost_table, float *d_time_labels, void *stream);
+
+/* From the statistics: MS_NFEAT float features and the exact size in bytes
+ * of every candidate (MS_NCAND uint64) per chunk; either output may be
+ * NULL. */
+int ms_chunk_features(const uint64_t *d_stats, uint64_t nchunks, float *d_features,
+                      uint64_t *d_sizes, void *stream);
+
+/* The format descriptor of candidate `cand` (static_bw for MS_CAND_STATIC). */
+int ms_candidate_format(uint32_t cand, uint32_t static_bw, ms_format_desc *out);
+
+typedef struct ms_mlp ms_mlp;
+
+/* A network with Glorot-uniform weights drawn from splitmix64(seed). */
+int ms_mlp_create(ms_mlp **out, uint64_t seed);
+int ms_mlp_get_params(const ms_mlp *m, float *host_params);        /* MS_MLP_NPARAM floats */
+int ms_mlp_set_params(ms_mlp *m, const float *host_params);        /* resets Adam state    */
+
+/* Loss (1/n) sum_s sum_o w_o (y_so - t_so)^2 over n samples (d_x: n x 24,
+ * d_y: n x 10, device) and its gradient (MS_MLP_NPARAM floats, device). */
+int ms_mlp_loss_grad(ms_mlp *m, const float *d_x, const float *d_y, const float *out_weights,
+                     uint64_t n, float *d_grad, float *d_loss, void *stream);
+
+/* Adam training (lr, 0.9, 0.999, 1e-8) on shuffled mini-batches of `batch`
+ * samples (a multiple of 64); epoch_loss (host, `epochs` floats) receives
+ * the mean loss of each epoch. */
+int ms_mlp_train(ms_mlp *m, const float *d_x, const float *d_y, const float *out_weights, uint64_t n,
+                 uint32_t epochs, uint32_t batch, float lr, uint64_t seed, float *epoch_loss, void *stream);
+
+/* Predictions (n x 10, device). */
+int ms_mlp_predict(const ms_mlp *m, const float *d_x, uint64_t n, float *d_y, void *stream);
+int ms_mlp_free(ms_mlp *m);
+
+typedef struct ms_select_params {
+    float size_off, size_scale;   /* predicted log2 bytes = y * size_scale + size_off */
+    float time_off, time_scale;   /* predicted log2 ns    = y * time_scale + time_off */
+    float size_tol;               /* candidates within (1 + size_tol) x the smallest prediction */
+    float time_weight;            /* pick the least  size + time_weight * time among them */
+    float exact_tol;              /* fallback to the exact smallest beyond (1 + exact_tol) x it */
+} ms_select_params;
+
+/* Per chunk a candidate (bit 7 set: the exact-size fallback was taken). */
+int ms_select_formats(const float *d_pred, const uint64_t *d_sizes, uint64_t n,
+                      const ms_select_params *sp, uint8_t *d_choice, void *stream);
+
 /* Reads the column's device status word: MS_OK, or the first error a kernel
  * recorded while producing it (e.g. MS_ERR_RANGE for an out-of-range
  * position in ms_project).  Synchronises `stream`. */
