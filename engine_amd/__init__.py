@@ -32,6 +32,7 @@ ABI = {
     "ms_block_size": (ctypes.c_int, [_FD, P(U64)]),
     "ms_col_upload": (ctypes.c_int, [_COLP, _FD, VP, _CM, VP]),
     "ms_col_upload_values": (ctypes.c_int, [_COLP, VP, U64, VP]),
+    "ms_col_from_device": (ctypes.c_int, [_COLP, VP, U64, VP]),
     "ms_col_get_meta": (ctypes.c_int, [VP, _CM, VP]),
     "ms_col_download": (ctypes.c_int, [VP, VP, U64, _CM, VP]),
     "ms_col_format": (ctypes.c_int, [VP, _FD]),
@@ -160,6 +161,12 @@ def upload_values(values, stream=None):
     init()
     v = np.ascontiguousarray(values, dtype=np.uint64)
     return _new(lib().ms_col_upload_values, v.ctypes.data, len(v), stream)
+
+
+def from_device(d_values, count, stream=None):
+    """An uncompr_f column copied from `count` device values (a pointer)."""
+    init()
+    return _new(lib().ms_col_from_device, d_values, count, stream)
 
 
 def upload_image(fmt, image, count, size_used, size_compr, stream=None):

@@ -405,6 +405,24 @@ extern "C" int ms_col_upload_values(ms_col **out, const uint64_t *values, uint64
     return ms_col_upload(out, &f, values, &m, stream);
 }
 
+extern "C" int ms_col_from_device(ms_col **out, const uint64_t *d_values, uint64_t count, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out || (!d_values && count)) return ms_fail(MS_ERR_INVALID, "null argument");
+    ms_format_desc f = {};
+    f.kind = MS_UNCOMPR;
+    ms_col *c;
+    MS_TRY(new_col(f, count, 8 * count, S(stream), &c));
+    if (count) MS_HIP(hipMemcpyAsync(c->data, d_values, 8 * count, hipMemcpyDeviceToDevice, S(stream)));
+    ms_dev_meta dm = {};
+    dm.count = count;
+    dm.size_used = 8 * count;
+    MS_HIP(hipMemcpyAsync(c->meta, &dm, sizeof dm, hipMemcpyHostToDevice, S(stream)));
+    MS_HIP(hipStreamSynchronize(S(stream)));   // dm lives on this stack frame
+    *out = c;
+    return MS_OK;
+}
+
 extern "C" int ms_col_get_meta(const ms_col *c, ms_col_meta *meta, void *stream)
 {
     if (!c || !meta) return ms_fail(MS_ERR_INVALID, "null argument");

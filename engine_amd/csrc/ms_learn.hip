@@ -400,7 +400,9 @@ __global__ __launch_bounds__(WG) void k_mlp_fwd_bwd(const float *__restrict__ pr
         }
         gp[OW2 + p] = a;
     }
-    // dH1 = dH2 W2^T masked by ReLU (into sH2, no longer needed)
+    // dH1 = dH2 W2^T masked by ReLU, into sH2 once every thread is done
+    // reading it (dW3 above)
+    __syncthreads();
     for (int u = 0; u < 16; u++) {
         const int i = q * 16 + u;
         float a = 0.f;

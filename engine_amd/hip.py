@@ -57,3 +57,47 @@ class Stream:
     def __del__(self):
         if self.s and _hip is not None:
             _hip.hipStreamDestroy(self.s)
+
+
+class DeviceArray:
+    """A device buffer of n elements of a numpy dtype (hipMalloc)."""
+
+    def __init__(self, n, dtype):
+        import numpy as np
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.p = ctypes.c_void_p()
+        _ok(hip().hipMalloc(ctypes.byref(self.p), ctypes.c_size_t(max(1, self.n * self.dtype.itemsize))),
+            "hipMalloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a, offset=0):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert offset + a.size <= self.n
+        _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        _ok(hip().hipMemcpy(ctypes.c_void_p(self.p.value + offset * self.dtype.itemsize),
+                            ctypes.c_void_p(a.ctypes.data), ctypes.c_size_t(a.nbytes), 1), "hipMemcpy")
+
+    def numpy(self, count=None):
+        import numpy as np
+        count = self.n if count is None else count
+        out = np.zeros(count, self.dtype)
+        _ok(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        _ok(hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), self.p, ctypes.c_size_t(out.nbytes), 2), "hipMemcpy")
+        return out
+
+    def ptr(self, offset=0):
+        return ctypes.c_void_p(self.p.value + offset * self.dtype.itemsize)
+
+    def __del__(self):
+        if self.p and _hip is not None:
+            _hip.hipFree(self.p)
+            self.p = None

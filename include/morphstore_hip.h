@@ -147,6 +147,10 @@ int ms_block_size(const ms_format_desc *f, uint64_t *out);
 int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *image,
                   const ms_col_meta *meta, void *stream);
 
+/* An uncompr_f column holding a copy of `count` values already in device
+ * memory (no reference counterpart: device-resident producers). */
+int ms_col_from_device(ms_col **out, const uint64_t *d_values, uint64_t count, void *stream);
+
 /* Uploads `count` uncompressed values as an uncompr_f column
  * (meta = {count, 8*count, 0}), as the reference's generators produce. */
 int ms_col_upload_values(ms_col **out, const uint64_t *values, uint64_t count,

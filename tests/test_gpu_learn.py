@@ -1,0 +1,154 @@
+"""Learned format selection on the GPU: the chunk generator, statistics,
+features and exact sizes against the CPU restatement (bit-exact / float
+tolerance), the MLP gradient against the float64 oracle, training, and the
+selection's footprint and bit-exactness."""
+import numpy as np
+import pytest
+
+import engine_amd as ea
+from engine_amd import hip, learn
+from oracle import chunks as och
+from oracle import mlp as omlp
+
+pytestmark = pytest.mark.gpu
+SEED = 0x4D6F7270 + 6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    ea.init(0)
+
+
+def edge_specs(tables):
+    t2, t10, t1, t64 = tables(2), tables(10), tables(1), tables(64)
+    return [
+        (learn.GEN_UNIFORM, 0, 0, 0, SEED, 0), (learn.GEN_UNIFORM, 13, 0, 0, SEED + 1, 0),
+        (learn.GEN_UNIFORM, 64, 0, 0, SEED + 2, 0),
+        (learn.GEN_DVBP, 1, 32, 0, SEED + 3, 0), (learn.GEN_DVBP, 64, 64, 0, SEED + 4, 0),
+        (learn.GEN_DVBP, 1, 1, 0, SEED + 5, 0),
+        (learn.GEN_SORTED, 10, 0, t10, SEED + 6, 12345), (learn.GEN_SORTED, 2, 0, t2, SEED + 7, 2**64 - 50000),
+        (learn.GEN_FOR, 16, 0, 0, SEED + 8, (1 << 40) + 3), (learn.GEN_FOR, 64, 0, 0, SEED + 9, 2**63),
+        (learn.GEN_RLE, 64, 20, t64, SEED + 10, 0), (learn.GEN_RLE, 1, 64, t1, SEED + 11, 0),
+        (learn.GEN_RLE, 2, 1, t2, SEED + 12, 0),
+    ]
+
+
+def oracle_chunk(spec, tables):
+    kind, a, b, tab, seed, base = spec
+    thr = tables.tabs[tab] if kind in (learn.GEN_SORTED, learn.GEN_RLE) else None
+    return och.gen_chunk(kind, a, b, seed, base, thr)
+
+
+def test_generator_stats_features_sizes_match_oracle():
+    tables = learn.GeoTables()
+    specs = edge_specs(tables)
+    n = len(specs)
+    d_vals = hip.DeviceArray(n * learn.CHUNK, np.uint64)
+    learn.gen_chunks(specs, tables, d_vals)
+    got = d_vals.numpy().reshape(n, learn.CHUNK)
+    cost = np.abs(np.sin(np.arange(5 * 65, dtype=np.float64))).astype(np.float32).reshape(5, 65) + 0.1
+    d_st = hip.DeviceArray(n * learn.NSTAT, np.uint64)
+    d_tl = hip.DeviceArray(n * learn.NCAND, np.float32)
+    learn.chunk_stats(d_vals, n, d_st, cost, d_tl)
+    d_f = hip.DeviceArray(n * learn.NFEAT, np.float32)
+    d_sz = hip.DeviceArray(n * learn.NCAND, np.uint64)
+    learn.chunk_features(d_st, n, d_f, d_sz)
+    st = d_st.numpy().reshape(n, -1)
+    feats = d_f.numpy().reshape(n, -1)
+    sz = d_sz.numpy().reshape(n, -1)
+    tl = d_tl.numpy().reshape(n, -1)
+    for i, sp in enumerate(specs):
+        x = oracle_chunk(sp, tables)
+        assert np.array_equal(got[i], x), (i, sp)
+        ost, bw, dbw, fbw = och.stats(x)
+        assert np.array_equal(st[i], ost), (i, sp, st[i], ost)
+        assert np.array_equal(sz[i], och.sizes(ost)), i
+        assert np.allclose(feats[i], och.features(ost), atol=2e-6), (i, feats[i], och.features(ost))
+        assert np.allclose(tl[i], och.time_labels(ost, bw, dbw, fbw, cost), rtol=1e-5), i
+
+
+def test_mlp_gradient_matches_float64_oracle():
+    rng = np.random.default_rng(3)
+    n = 1000
+    X = rng.normal(0, 1, (n, 24)).astype(np.float32)
+    Y = rng.normal(0, 1, (n, 10)).astype(np.float32)
+    w = np.linspace(0.5, 2.0, 10).astype(np.float32)
+    m = learn.MLP(seed=99)
+    p = m.params
+    L, g = m.loss_grad(hip.DeviceArray.from_numpy(X.reshape(-1)), hip.DeviceArray.from_numpy(Y.reshape(-1)), n, w)
+    Lo, go = omlp.loss_grad(p.astype(np.float64), X.astype(np.float64), Y.astype(np.float64), w.astype(np.float64))
+    assert abs(L - Lo) <= 1e-4 * abs(Lo)
+    scale = np.abs(go).max()
+    assert np.max(np.abs(g - go)) <= 2e-4 * scale, np.max(np.abs(g - go)) / scale
+    # a parameter change moves the GPU loss as the oracle predicts
+    p2 = p + 1e-3 * np.sign(go).astype(np.float32)
+    m.params = p2
+    L2, _ = m.loss_grad(hip.DeviceArray.from_numpy(X.reshape(-1)), hip.DeviceArray.from_numpy(Y.reshape(-1)), n, w)
+    assert abs(L2 - omlp.loss(p2.astype(np.float64), X, Y, w)) <= 1e-4 * abs(L2)
+
+
+def synthetic_cost():
+    """A fixed decode-cost table (ns per unit by bit width) for tests that
+    must not depend on timing."""
+    bw = np.arange(65, dtype=np.float32)
+    cost = np.zeros((5, 65), np.float32)
+    cost[0] = 0.10 + 0.010 * bw
+    cost[1] = 0.14 + 0.010 * bw
+    cost[2] = 0.30 + 0.012 * bw
+    cost[3] = 0.20 + 0.011 * bw
+    cost[4, 0], cost[4, 1] = 5.0, 0.02
+    return cost
+
+
+def test_training_and_selection():
+    tables = learn.GeoTables()
+    cost = synthetic_cost()
+    ntrain = 8192
+    d_x, d_y, sizes, times = learn.label(learn.training_specs(SEED, ntrain, tables), tables, cost)
+    m = learn.MLP(seed=5)
+    losses = m.train(d_x, d_y, ntrain, np.ones(10, np.float32), epochs=40, batch=1024, lr=3e-3)
+    assert losses[-1] < 0.1 * losses[0], losses
+    # held-out chunks of the selection workload
+    specs = [s for c in range(8) for s in learn.mixture_specs(SEED + 1, c, 64, tables)]
+    n = len(specs)
+    d_f, _, esz, etm = learn.label(specs, tables, cost)
+    d_sz = hip.DeviceArray.from_numpy(esz.reshape(-1))
+    pred = m.predict(d_f, n)
+    choice, fell_back = learn.select_formats(pred, d_sz, n)
+    best = esz.min(axis=1).astype(np.float64)
+    picked = esz[np.arange(n), choice].astype(np.float64)
+    assert picked.sum() <= 1.03 * best.sum()
+    assert np.all(picked <= 1.03 * best + 1e-9)
+    assert fell_back.mean() < 0.25, fell_back.mean()
+    # the network's gain over exact sizing alone: no slower decode in total
+    exact_choice = esz.argmin(axis=1)
+    t_learned = etm[np.arange(n), choice].sum()
+    t_exact = etm[np.arange(n), exact_choice].sum()
+    assert t_learned <= t_exact * 1.02, (t_learned, t_exact)
+
+
+def test_selected_chunks_are_bit_exact():
+    tables = learn.GeoTables()
+    specs = learn.mixture_specs(SEED + 2, 0, 40, tables)
+    n = len(specs)
+    d_vals = hip.DeviceArray(n * learn.CHUNK, np.uint64)
+    learn.gen_chunks(specs, tables, d_vals)
+    host = d_vals.numpy().reshape(n, learn.CHUNK)
+    d_st = hip.DeviceArray(n * learn.NSTAT, np.uint64)
+    learn.chunk_stats(d_vals, n, d_st)
+    st = d_st.numpy().reshape(n, -1)
+    b = ea.Batch()
+    want = []
+    for i in range(n):
+        for cand in range(5):
+            f = learn.candidate_format(cand, int(st[i, learn.ST_MAX_BW]))
+            col = ea.morph(f, ea.from_device(d_vals.ptr(i * learn.CHUNK), learn.CHUNK))
+            if i % 8 == 0:
+                assert np.array_equal(col.values(), host[i])
+            val = int(np.median(host[i]))
+            b.enqueue(col, 2, val)
+            with np.errstate(over="ignore"):
+                want.append(int(np.sum(host[i][host[i] < np.uint64(val)], dtype=np.uint64)))
+    res = ea.DeviceU64(len(want))
+    b.run(res)
+    assert res.get().tolist() == want
