@@ -191,6 +191,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--c4-cols", type=int, default=4096, help="columns of the uneven workload (0: skip it)")
+    ap.add_argument("--c5-cols", type=int, default=1024,
+                    help="columns of 64 x 2^16 values of the learned-selection workload (0: skip it)")
+    ap.add_argument("--c5-train", type=int, default=1_000_000, help="labelled training chunks of the selector")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="only the layouts whose name starts with this (profiling)")
     args = ap.parse_args()
@@ -286,6 +289,14 @@ def main():
     }
     if args.c4_cols > 0 and not args.only:
         out["uneven_columns"] = c4_workload(args, ea, hip, datagen)
+    if args.c5_cols > 0 and not args.only:
+        from engine_amd import learn
+        try:
+            out["learned_selection"] = learn.run_selection_workload(ncols=args.c5_cols, ntrain=args.c5_train,
+                                                                    steps=max(2, args.steps // 2), log=log)
+        except Exception as e:     # the sweep's line above stands on its own
+            log(f"learned-selection workload failed: {e!r}")
+            out["learned_selection"] = {"error": repr(e)}
     if not args.no_cpu and cols:
         dom = cols[0]
         threads_all = len(os.sched_getaffinity(0))

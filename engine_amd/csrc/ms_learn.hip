@@ -485,32 +485,31 @@ __global__ __launch_bounds__(WG) void k_mlp_predict(const float *__restrict__ pr
     for (int o = 0; o < OUT; o++) Yp[s * OUT + o] = y[o];
 }
 
-// Per chunk: among the candidates whose predicted size is within size_tol
-// of the smallest prediction, the one with the smallest predicted time; if
-// that pick is more than exact_tol larger than the exact smallest size, the
-// exact smallest instead (bit 7 of the choice marks such a fallback).
-// Predictions are (log2 bytes - size_off) / size_scale and
-// (log2 ns - time_off) / time_scale.
+// Per chunk: the candidate with the smallest predicted size, unless another
+// candidate predicted within size_tol of that size is predicted faster by
+// more than time_margin (then the fastest such one); if the pick is more
+// than exact_tol larger than the exact smallest size, the exact smallest
+// instead (bit 7 of the choice marks such a fallback).  Predictions are
+// (log2 bytes - size_off) / size_scale and (log2 ns - time_off) / time_scale.
 __global__ __launch_bounds__(WG) void k_select_formats(const float *__restrict__ pred,
                                                        const uint64_t *__restrict__ sizes, uint64_t n,
-                                                       float size_tol, float exact_tol, float time_weight,
+                                                       float size_tol, float exact_tol, float time_margin,
                                                        float4 norm, uint8_t *__restrict__ choice)
 {
     const uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x;
     if (c >= n) return;
     const float *p = pred + c * OUT;
-    float ps[MS_NCAND], pt[MS_NCAND], best = 3.4e38f;
+    float ps[MS_NCAND], pt[MS_NCAND];
+    int small = 0;
     for (int f = 0; f < MS_NCAND; f++) {
         ps[f] = exp2f(p[f] * norm.y + norm.x);
         pt[f] = exp2f(p[MS_NCAND + f] * norm.w + norm.z);
-        best = ps[f] < best ? ps[f] : best;
+        small = ps[f] < ps[small] ? f : small;
     }
-    int pick = 0;
-    float pick_cost = 3.4e38f;
+    int pick = small;
     for (int f = 0; f < MS_NCAND; f++) {
-        if (ps[f] > (1.f + size_tol) * best) continue;
-        const float cost = ps[f] + time_weight * pt[f];
-        if (cost < pick_cost) { pick_cost = cost; pick = f; }
+        if (ps[f] > (1.f + size_tol) * ps[small]) continue;
+        if (pt[f] * (1.f + time_margin) < pt[small] && pt[f] < pt[pick]) pick = f;
     }
     const uint64_t *z = sizes + c * MS_NCAND;
     int ex = 0;
@@ -793,7 +792,7 @@ extern "C" int ms_select_formats(const float *d_pred, const uint64_t *d_sizes, u
     hipStream_t s = (hipStream_t)stream;
     const float4 norm = make_float4(sp->size_off, sp->size_scale, sp->time_off, sp->time_scale);
     hipLaunchKernelGGL(k_select_formats, dim3((n + WG - 1) / WG), dim3(WG), 0, s, d_pred, d_sizes, n, sp->size_tol,
-                       sp->exact_tol, sp->time_weight, norm, d_choice);
+                       sp->exact_tol, sp->time_margin, norm, d_choice);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

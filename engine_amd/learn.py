@@ -14,10 +14,10 @@ engine
      time (the calibration table applied to the chunk's bit widths);
   3. trains the FP32 MLP on the device (24 features -> log2 size and log2
      time of the 5 candidates);
-  4. selects per chunk: among candidates predicted within `size_tol` of the
-     smallest prediction, the fastest predicted; exact sizes judge the pick
-     and replace it by the exact smallest when it is more than `exact_tol`
-     larger.
+  4. selects per chunk: the smallest predicted candidate, or one predicted
+     within `size_tol` of it and faster by more than `time_margin`; exact
+     sizes judge the pick and replace it by the exact smallest when it is
+     more than `exact_tol` larger.
 Whatever is picked, the data stays bit-exact: only the format changes.
 """
 import ctypes
@@ -50,7 +50,7 @@ class ChunkSpec(ctypes.Structure):
 
 class SelectParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("size_off", "size_scale", "time_off", "time_scale", "size_tol",
-                                               "time_weight", "exact_tol")]
+                                               "time_margin", "exact_tol")]
 
 
 def geo_table(a):
@@ -78,11 +78,23 @@ class GeoTables:
         return np.concatenate(self.tabs) if self.tabs else np.zeros(GEO_LEN, np.uint64)
 
 
+SPEC_DTYPE = np.dtype([("kind", "<u4"), ("a", "<u4"), ("b", "<u4"), ("table", "<u4"), ("seed", "<u8"),
+                       ("base", "<u8")])
+assert SPEC_DTYPE.itemsize == ctypes.sizeof(ChunkSpec)
+
+
+def spec_array(specs):
+    """A ms_chunk_spec array from a list of tuples (or a spec array)."""
+    if isinstance(specs, np.ndarray) and specs.dtype == SPEC_DTYPE:
+        return specs
+    return np.array([tuple(int(v) for v in sp) for sp in specs], dtype=SPEC_DTYPE)
+
+
 def gen_chunks(specs, tables, d_out, offset_chunks=0, stream=None):
     """Writes len(specs) chunks into d_out (a hip.DeviceArray of uint64)."""
-    arr = (ChunkSpec * len(specs))(*[ChunkSpec(*s) for s in specs])
+    arr = np.ascontiguousarray(spec_array(specs))
     tab = np.ascontiguousarray(tables.array())
-    ea.check(ea.lib().ms_gen_chunks(d_out.ptr(offset_chunks * CHUNK), arr, len(specs), tab.ctypes.data,
+    ea.check(ea.lib().ms_gen_chunks(d_out.ptr(offset_chunks * CHUNK), arr.ctypes.data, len(arr), tab.ctypes.data,
                                     len(tables.tabs), stream))
 
 
@@ -152,9 +164,9 @@ class MLP:
             self.h = None
 
 
-def select_formats(d_pred, d_sizes, n, size_tol=0.02, time_weight=1e6, exact_tol=0.03, stream=None):
+def select_formats(d_pred, d_sizes, n, size_tol=0.02, time_margin=0.15, exact_tol=0.03, stream=None):
     """Per chunk: candidate index, and whether the exact-size fallback fired."""
-    sp = SelectParams(SIZE_OFF, SIZE_SCALE, TIME_OFF, TIME_SCALE, size_tol, time_weight, exact_tol)
+    sp = SelectParams(SIZE_OFF, SIZE_SCALE, TIME_OFF, TIME_SCALE, size_tol, time_margin, exact_tol)
     d_c = hip.DeviceArray(n, np.uint8)
     ea.check(ea.lib().ms_select_formats(d_pred.ptr(), d_sizes.ptr(), n, ctypes.byref(sp), d_c.ptr(), stream))
     c = d_c.numpy()
@@ -177,31 +189,44 @@ def training_specs(seed, n, tables):
     """n chunk specs: every generator kind with parameters spread over the
     ranges the selector must handle."""
     r = _u(seed, 6 * n).reshape(n, 6)
-    sorted_means = (2, 3, 4, 10, 32, 100, 1000)
-    rle_means = (1, 2, 4, 8, 16, 64, 256, 1024)
-    specs = []
-    for i in range(n):
-        k = int(r[i, 0] % np.uint64(5))
-        s = int(r[i, 1])
-        q2, q3, q4 = int(r[i, 2]), int(r[i, 3]), int(r[i, 4])
-        if k == GEN_UNIFORM:
-            specs.append((k, q2 % 65, 0, 0, s, 0))
-        elif k == GEN_DVBP:
-            lo = 1 + q2 % 48
-            hi = min(64, lo + q3 % 33)
-            specs.append((k, lo, hi, 0, s, 0))
-        elif k == GEN_SORTED:
-            a = sorted_means[q2 % len(sorted_means)]
-            base = q3 >> (q4 % 64)
-            specs.append((k, a, 0, tables(a), s, base))
-        elif k == GEN_FOR:
-            bits = 1 + q2 % 40
-            base = q3 >> (q4 % 64)
-            specs.append((k, bits, 0, 0, s, base))
-        else:
-            a = rle_means[q2 % len(rle_means)]
-            specs.append((k, a, 1 + q3 % 64, tables(a), s, 0))
-    return specs
+    sorted_means = np.array((2, 3, 4, 10, 32, 100, 1000), np.uint64)
+    rle_means = np.array((1, 2, 4, 8, 16, 64, 256, 1024), np.uint64)
+    for a in sorted_means.tolist() + rle_means.tolist():
+        tables(a)
+    tab_of = np.zeros(1025, np.uint32)
+    for a, k in tables.index.items():
+        if a <= 1024:
+            tab_of[a] = k
+    kind = (r[:, 0] % np.uint64(5)).astype(np.uint32)
+    q2, q3, q4 = r[:, 2], r[:, 3], r[:, 4]
+    out = np.zeros(n, SPEC_DTYPE)
+    out["kind"] = kind
+    out["seed"] = r[:, 1]
+    # uniform: a ~ U{0..64}
+    a = (q2 % np.uint64(65)).astype(np.uint32)
+    b = np.zeros(n, np.uint32)
+    tab = np.zeros(n, np.uint32)
+    base = np.zeros(n, np.uint64)
+    m = kind == GEN_DVBP
+    lo = (1 + q2 % np.uint64(48)).astype(np.uint32)
+    a[m] = lo[m]
+    b[m] = np.minimum(64, lo + (q3 % np.uint64(33)).astype(np.uint32))[m]
+    shifted = q3 >> (q4 % np.uint64(64))
+    m = kind == GEN_SORTED
+    sm = sorted_means[(q2 % np.uint64(len(sorted_means))).astype(np.int64)]
+    a[m] = sm[m].astype(np.uint32)
+    tab[m] = tab_of[sm[m].astype(np.int64)]
+    base[m] = shifted[m]
+    m = kind == GEN_FOR
+    a[m] = (1 + q2 % np.uint64(40)).astype(np.uint32)[m]
+    base[m] = shifted[m]
+    m = kind == GEN_RLE
+    rm = rle_means[(q2 % np.uint64(len(rle_means))).astype(np.int64)]
+    a[m] = rm[m].astype(np.uint32)
+    b[m] = (1 + q3 % np.uint64(64)).astype(np.uint32)[m]
+    tab[m] = tab_of[rm[m].astype(np.int64)]
+    out["a"], out["b"], out["table"], out["base"] = a, b, tab, base
+    return out
 
 
 def mixture_specs(seed, col, nchunks, tables):
@@ -241,7 +266,7 @@ def _calib_values(cand, bw, nchunks, seed):
     return u
 
 
-def calibrate(nchunks=16, copies=8, reps=5, stream=None, log=None):
+def calibrate(nchunks=16, copies=64, reps=5, stream=None, log=None):
     """ns per 512-value unit of the batched fused select+sum, per candidate
     and bit width (1..64), and rle's fixed and per-run cost: a MS_NCAND x 65
     table for ms_chunk_stats."""
@@ -296,6 +321,7 @@ def label(specs, tables, cost, batch=2048, stream=None):
     """Generates the chunks of `specs` on the device in batches and returns
     device arrays (features n x 24, targets n x 10) and host exact sizes and
     time labels."""
+    specs = spec_array(specs)
     n = len(specs)
     d_feat = hip.DeviceArray(n * NFEAT, np.float32)
     sizes = np.zeros((n, NCAND), np.uint64)
@@ -313,3 +339,170 @@ def label(specs, tables, cost, batch=2048, stream=None):
         times[c0:c0 + m] = d_tl.numpy(m * NCAND).reshape(m, NCAND)
     d_y = hip.DeviceArray.from_numpy(targets(sizes, times).reshape(-1))
     return d_feat, d_y, sizes, times
+
+
+# ------------------------------------------------------------------ workload
+def _masked_sum_col(col, val, res):
+    ea.select_sum(2, val, col, res)
+    return int(res.get()[0])
+
+
+def column_level_choice(esz, max_bw):
+    """Whole-column format for the per-column path: the candidate whose
+    column size (sum of the chunks' exact sizes; static_vbp at the column's
+    maximum bit width) is smallest."""
+    tot = esz.sum(axis=0).astype(np.float64)
+    tot[0] = len(esz) * (CHUNK // 8) * float(max_bw)
+    k = int(np.argmin(tot))
+    return k, int(max_bw)
+
+
+def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epochs=30, seed=0x4D6F7270 + 4,
+                           val=1 << 24, steps=10, warmup=2, log=None, cost=None):
+    """The learned-selection workload: ncols columns of chunks_per_col
+    chunks (a mixture of the format-cascade-sweep distributions switching
+    every 2^16 values).  Trains the selector on ntrain generated chunks,
+    picks a format per chunk, and times SELECT SUM(x) WHERE x < val over
+    every column: per-column calls on whole columns in their column-level
+    best format (plain, eager and as one hipGraph) against one autotuned
+    batch over the learned per-chunk columns and over the exact-size
+    per-chunk columns.  Every path's per-column sums must equal the sums
+    over the uncompressed columns."""
+    import time
+    say = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    tables = GeoTables()
+    if cost is None:
+        cost = calibrate(log=say)
+    t_cal = time.perf_counter() - t0
+    specs = training_specs(seed, ntrain, tables)
+    d_x, d_y, _, _ = label(specs, tables, cost)
+    net = MLP(seed=seed)
+    t1 = time.perf_counter()
+    # Adam with the step size halved after each third of the epochs
+    losses = np.concatenate([net.train(d_x, d_y, ntrain, np.ones(NOUT, np.float32), epochs=e, batch=4096, lr=lr,
+                                       seed=seed + k)
+                             for k, (e, lr) in enumerate(((epochs // 3, 2e-3), (epochs // 3, 1e-3),
+                                                          (epochs - 2 * (epochs // 3), 5e-4)))])
+    hip.synchronize()
+    t_train = time.perf_counter() - t1
+    say(f"selector: {ntrain} labelled chunks, {epochs} epochs in {t_train:.1f}s, loss {losses[0]:.4f} -> "
+        f"{losses[-1]:.4f}")
+    del d_x, d_y
+    n = chunks_per_col
+    d_vals = hip.DeviceArray(n * CHUNK, np.uint64)
+    d_st = hip.DeviceArray(n * NSTAT, np.uint64)
+    d_f = hip.DeviceArray(n * NFEAT, np.float32)
+    d_sz = hip.DeviceArray(n * NCAND, np.uint64)
+    res1 = ea.DeviceU64()
+    learned, exact, plain, want = [], [], [], []
+    foot = {"learned": 0, "exact": 0, "plain": 0}
+    fallbacks, picks = 0, np.zeros(NCAND, np.int64)
+    t2 = time.perf_counter()
+    for c in range(ncols):
+        gen_chunks(mixture_specs(seed, c, n, tables), tables, d_vals)
+        chunk_stats(d_vals, n, d_st)
+        chunk_features(d_st, n, d_f, d_sz)
+        st = d_st.numpy(n * NSTAT).reshape(n, NSTAT)
+        esz = d_sz.numpy(n * NCAND).reshape(n, NCAND)
+        pred = net.predict(d_f, n)
+        choice, fb = select_formats(pred, d_sz, n)
+        ex = esz.argmin(axis=1)
+        fallbacks += int(fb.sum())
+        picks += np.bincount(choice, minlength=NCAND)
+        whole = ea.from_device(d_vals.ptr(), n * CHUNK)
+        want.append(_masked_sum_col(whole, val, res1))
+        k, mbw = column_level_choice(esz, st[:, ST_MAX_BW].max())
+        pc = ea.morph(candidate_format(k, mbw), whole)
+        plain.append(pc)
+        foot["plain"] += pc.meta().size_used
+        del whole
+        for i in range(n):
+            raw = ea.from_device(d_vals.ptr(i * CHUNK), CHUNK)
+            bw = int(st[i, ST_MAX_BW])
+            lc = ea.morph(candidate_format(int(choice[i]), bw), raw)
+            ec = lc if ex[i] == choice[i] else ea.morph(candidate_format(int(ex[i]), bw), raw)
+            learned.append(lc)
+            exact.append(ec)
+            foot["learned"] += int(esz[i, choice[i]])
+            foot["exact"] += int(esz[i, ex[i]])
+    t_prep = time.perf_counter() - t2
+    nchunks = ncols * n
+    say(f"selection workload: {ncols} columns x {n} chunks prepared in {t_prep:.1f}s; footprint learned "
+        f"{foot['learned']} exact {foot['exact']} plain {foot['plain']}; fallbacks {fallbacks}")
+    res = ea.DeviceU64(nchunks)
+    resc = ea.DeviceU64(ncols)
+    slots = [resc.at(k) for k in range(ncols)]
+
+    def check_cols(got, tag):
+        if got != want:
+            bad = [k for k in range(ncols) if got[k] != want[k]]
+            raise RuntimeError(f"selection workload: {tag} sums differ at columns {bad[:5]}")
+
+    def per_col(r):
+        with np.errstate(over="ignore"):
+            return r.reshape(ncols, n).sum(axis=1, dtype=np.uint64).tolist()
+
+    def timed(fn, reps):
+        for _ in range(warmup):
+            fn()
+        hip.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        return e0.elapsed_ms(e1) / reps
+
+    def plain_fn(stream=None):
+        for k, col in enumerate(plain):
+            ea.select_sum(2, val, col, slots[k], stream)
+    plain_fn()
+    check_cols(resc.get().tolist(), "plain")
+    ms_plain = timed(plain_fn, max(2, steps // 2))
+    st_ = hip.Stream()
+    plan = ea.Plan()
+    with plan.record(st_.handle):
+        plain_fn(st_.handle)
+    with plan.record(st_.handle):
+        plain_fn(st_.handle)
+    resc.zero()
+    plan.launch(st_.handle)
+    st_.synchronize()
+    check_cols(resc.get().tolist(), "plain graph")
+    ms_graph = timed(lambda: plan.launch(None), steps)
+    plan.free()
+    out = {}
+    for tag, cols in (("learned", learned), ("exact", exact)):
+        b = ea.Batch()
+        for col in cols:
+            b.enqueue(col, 2, val)
+        b.run(res)
+        check_cols(per_col(res.get()), f"batched {tag}")
+        cfg = b.tune(res, reps=3)
+        b.run(res)
+        check_cols(per_col(res.get()), f"batched {tag} tuned")
+        out[tag] = (timed(lambda: b.run(res), steps), cfg.as_dict())
+    values = nchunks * CHUNK
+    ue = lambda ms: round(8 * values / ms / 1e6, 1)  # noqa: E731
+    rep = {
+        "workload": f"{ncols} uint64 columns of {n} x 2^16 values (a mixture of the format-cascade-sweep "
+                    f"distributions switching every 2^16 values), select(x < {val}) + agg_sum per column",
+        "train_chunks": ntrain, "epochs": epochs, "loss_first_epoch": float(losses[0]),
+        "loss_last_epoch": float(losses[-1]), "train_seconds": round(t_train, 2),
+        "calibration_seconds": round(t_cal, 1),
+        "footprint_bytes": foot, "learned_over_exact": round(foot["learned"] / foot["exact"], 5),
+        "exact_fallbacks": fallbacks, "chunks": nchunks,
+        "learned_picks": {CAND_NAMES[k]: int(picks[k]) for k in range(NCAND)},
+        "plain_ms": round(ms_plain, 3), "plain_graph_ms": round(ms_graph, 3),
+        "batched_learned_ms": round(out["learned"][0], 3), "batched_exact_ms": round(out["exact"][0], 3),
+        "uncompr_equiv_GBps": {"plain": ue(ms_plain), "plain_graph": ue(ms_graph),
+                               "batched_learned": ue(out["learned"][0]), "batched_exact": ue(out["exact"][0])},
+        "compressed_GBps_batched_learned": round(foot["learned"] / out["learned"][0] / 1e6, 1),
+        "speedup_batched_learned_vs_plain": round(ms_plain / out["learned"][0], 2),
+        "speedup_batched_learned_vs_plain_graph": round(ms_graph / out["learned"][0], 2),
+        "batch_config": out["learned"][1],
+    }
+    say(f"selection workload: plain {ms_plain:.2f} ms, plain graph {ms_graph:.2f} ms, batched learned "
+        f"{out['learned'][0]:.3f} ms, batched exact {out['exact'][0]:.3f} ms")
+    return rep

@@ -401,8 +401,8 @@ int ms_mlp_free(ms_mlp *m);
 typedef struct ms_select_params {
     float size_off, size_scale;   /* predicted log2 bytes = y * size_scale + size_off */
     float time_off, time_scale;   /* predicted log2 ns    = y * time_scale + time_off */
-    float size_tol;               /* candidates within (1 + size_tol) x the smallest prediction */
-    float time_weight;            /* pick the least  size + time_weight * time among them */
+    float size_tol;               /* candidates within (1 + size_tol) x the smallest size prediction ... */
+    float time_margin;            /* ... replace it when predicted faster by more than this fraction */
     float exact_tol;              /* fallback to the exact smallest beyond (1 + exact_tol) x it */
 } ms_select_params;
 

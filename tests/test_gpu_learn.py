@@ -120,11 +120,38 @@ def test_training_and_selection():
     assert picked.sum() <= 1.03 * best.sum()
     assert np.all(picked <= 1.03 * best + 1e-9)
     assert fell_back.mean() < 0.25, fell_back.mean()
-    # the network's gain over exact sizing alone: no slower decode in total
+    # with this cost table the exact-smallest pick is (nearly) the fastest:
+    # the time-aware pick must not be slower in total
     exact_choice = esz.argmin(axis=1)
     t_learned = etm[np.arange(n), choice].sum()
     t_exact = etm[np.arange(n), exact_choice].sum()
     assert t_learned <= t_exact * 1.02, (t_learned, t_exact)
+
+
+def test_network_gain_over_exact_sizing():
+    """What the trained network adds to exact sizing: it predicts decode
+    time, so where a candidate within 3% of the smallest size decodes much
+    faster (here: for_f is made 3x cheaper per unit than dynamic_vbp), the
+    learned pick takes it; exact sizing alone cannot see the difference."""
+    tables = learn.GeoTables()
+    cost = synthetic_cost()
+    cost[3] = cost[1] / 3.0
+    ntrain = 16384
+    d_x, d_y, _, _ = learn.label(learn.training_specs(SEED + 7, ntrain, tables), tables, cost)
+    m = learn.MLP(seed=11)
+    for e, lr in ((20, 3e-3), (20, 1e-3)):
+        m.train(d_x, d_y, ntrain, np.ones(10, np.float32), epochs=e, batch=1024, lr=lr)
+    specs = [s for c in range(8) for s in learn.mixture_specs(SEED + 8, c, 64, tables)]
+    n = len(specs)
+    d_f, _, esz, etm = learn.label(specs, tables, cost)
+    choice, _ = learn.select_formats(m.predict(d_f, n), hip.DeviceArray.from_numpy(esz.reshape(-1)), n,
+                                     size_tol=0.03)
+    exact_choice = esz.argmin(axis=1)
+    rows = np.arange(n)
+    t_learned, t_exact = etm[rows, choice].sum(), etm[rows, exact_choice].sum()
+    s_learned, s_exact = esz[rows, choice].sum(), esz[rows, exact_choice].sum()
+    assert s_learned <= 1.03 * s_exact
+    assert t_learned < 0.95 * t_exact, (t_learned, t_exact)
 
 
 def test_selected_chunks_are_bit_exact():
@@ -152,3 +179,14 @@ def test_selected_chunks_are_bit_exact():
     res = ea.DeviceU64(len(want))
     b.run(res)
     assert res.get().tolist() == want
+
+
+def test_selection_workload_small():
+    """The whole pipeline at a small scale: every path's per-column sums
+    equal the uncompressed sums (checked inside), footprint within 3% of the
+    exact-best choice."""
+    rep = learn.run_selection_workload(ncols=6, chunks_per_col=8, ntrain=8192, epochs=20, steps=2, warmup=1,
+                                       cost=synthetic_cost())
+    assert rep["learned_over_exact"] <= 1.03
+    assert rep["footprint_bytes"]["learned"] <= rep["footprint_bytes"]["plain"] * 1.03
+    assert rep["chunks"] == 48
