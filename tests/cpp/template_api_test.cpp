@@ -1,0 +1,91 @@
+// The reference's query shape through the header-only template API
+// (select_sum_benchmark.cpp:78-118 and the operator known-answer tests of
+// test/core/operators/*_test.cpp), checked against brute force.  Prints
+// "template api ok" and exits 0 on success.
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "morphstore_hip.hpp"
+
+using namespace morphstore;
+using ve = vectorlib::hip_gfx950;
+
+#define CHECK(c)                                                         \
+    do {                                                                 \
+        if (!(c)) {                                                      \
+            std::fprintf(stderr, "check failed: %s (line %d)\n", #c, __LINE__); \
+            return 1;                                                    \
+        }                                                                \
+    } while (0)
+
+static uint64_t splitmix(uint64_t i)
+{
+    uint64_t z = 0x4D6F7270ull + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int main()
+{
+    ms_check(ms_init(0));
+    // select(x < 2^11) -> positions -> project -> agg_sum over static_vbp<12, 1>
+    const size_t n = (1u << 20) + 77;
+    std::vector<uint64_t> x(n);
+    for (size_t i = 0; i < n; i++) x[i] = splitmix(i) >> 52;
+    using sf = static_vbp_f<vbp_l<12, 1>>;
+    const column<uncompr_f> *base = make_column(x);
+    const column<sf> *c = morph<ve, sf>(base);
+    CHECK(c->get_count_values() == n);
+    CHECK(c->get_size_compr_byte() == (n / 64) * 64 * 12 / 8);
+    CHECK(values_of(c) == x);
+    const column<uncompr_f> *pos = my_select_wit_t<less, ve, uncompr_f, sf>::apply(c, 1u << 11);
+    const column<uncompr_f> *dat = my_project_wit_t<ve, uncompr_f, sf, uncompr_f>::apply(c, pos);
+    const column<uncompr_f> *sum = agg_sum<ve, uncompr_f>(dat);
+    uint64_t want = 0, cnt = 0;
+    std::vector<uint64_t> want_pos;
+    for (size_t i = 0; i < n; i++)
+        if (x[i] < (1u << 11)) { want += x[i]; cnt++; want_pos.push_back(i); }
+    CHECK(values_of(pos) == want_pos);
+    CHECK(values_of(sum) == std::vector<uint64_t>{want});
+    // a same-format morph is the input itself (morph.h:128-151)
+    CHECK(((const void *)morph<ve, sf>(c) == (const void *)c));
+    // compressed positions through the write-iterator, cascades
+    using df = delta_f<1024, 8, dynamic_vbp_f<512, 64, 8>>;
+    using ff = for_f<1024, 8, dynamic_vbp_f<512, 64, 8>>;
+    const column<df> *pd = my_select_wit_t<greaterequal, ve, df, sf>::apply(c, 100);
+    std::vector<uint64_t> ge;
+    for (size_t i = 0; i < n; i++)
+        if (x[i] >= 100) ge.push_back(i);
+    CHECK(values_of(pd) == ge);
+    const column<ff> *cf = morph<ve, ff>(base);
+    CHECK(values_of(cf) == x);
+    // known answers: intersect / merge / calc (intersect_test.cpp, merge_test.cpp, calc_binary_test.cpp shapes)
+    const column<uncompr_f> *l = make_column(std::vector<uint64_t>{1, 3, 5, 7, 9, 11});
+    const column<uncompr_f> *r = make_column(std::vector<uint64_t>{3, 4, 5, 10, 11});
+    const column<uncompr_f> *is = intersect_sorted<ve, uncompr_f>(l, r);
+    CHECK(values_of(is) == (std::vector<uint64_t>{3, 5, 11}));
+    const column<uncompr_f> *ms = merge_sorted<ve, uncompr_f>(l, r);
+    CHECK(values_of(ms) == (std::vector<uint64_t>{1, 3, 4, 5, 7, 9, 10, 11}));
+    const column<uncompr_f> *a = make_column(std::vector<uint64_t>{10, 20, 30});
+    const column<uncompr_f> *b = make_column(std::vector<uint64_t>{1, 2, 3});
+    const column<uncompr_f> *s = calc_binary<std::minus, ve, uncompr_f>(a, b);
+    CHECK(values_of(s) == (std::vector<uint64_t>{9, 18, 27}));
+    // rle round trip
+    const column<rle_f> *rl = morph<ve, rle_f>(make_column(std::vector<uint64_t>{4, 4, 4, 9, 9, 1}));
+    CHECK(rl->get_size_used_byte() == 48);
+    CHECK(values_of(rl) == (std::vector<uint64_t>{4, 4, 4, 9, 9, 1}));
+    // errors surface as exceptions
+    bool threw = false;
+    try {
+        calc_binary<std::plus, ve, uncompr_f>(a, l);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+    delete pos; delete dat; delete sum; delete pd; delete cf; delete is; delete ms; delete s;
+    delete rl; delete a; delete b; delete l; delete r; delete c; delete base;
+    std::printf("template api ok: %zu values, %llu selected\n", n, (unsigned long long)cnt);
+    return 0;
+}
