@@ -2,14 +2,18 @@
 // and mixed formats in one persistent launch.
 //
 // The host cuts every enqueued column into work items (a run of 512-value
-// units, of 2048-value delta tiles, of rle runs, or of raw values) and
-// splits the item list into one contiguous range per wave so that every
-// wave gets the same modelled cost (compressed bytes plus a fixed cost per
-// unit).  The persistent kernel walks its wave's range of that device work
-// queue; each item's select + sum goes to its own partial slot (no atomics),
-// and a second kernel folds the partials of each entry.  The queue shape
-// (workgroups per CU, item sizes per layout, the unit cost of the model) is
-// a run-time configuration that ms_batch_tune picks by timing candidates.
+// units, of 2048-value delta tiles, of rle runs, or of raw values), in
+// entry order.  An item carries what its wave needs (pointers, predicate,
+// the block-index entry of its first unit), so starting one costs a single
+// scalar load, issued one item ahead.  The persistent kernel's W waves take
+// the items round-robin: wave w does items w, w + W, w + 2W, ...; so every
+// column is spread over all waves and neighbouring waves stream
+// neighbouring bytes.  A wave's items of one entry are consecutive in its
+// sequence: it sums them in registers and writes one partial per entry it
+// touched, into the slot of its first item of that entry -- for entry e
+// with items [lo, hi) exactly the slots [lo, min(hi, lo + W)).  A second
+// kernel folds those.  The item sizes per layout and the workgroups per CU
+// are picked by ms_batch_tune.
 #include <algorithm>
 #include <vector>
 
@@ -23,13 +27,28 @@ struct ms_bentry {
     ms_colv c;
     ms_pred p;
     uint64_t item_lo, item_hi;   // the entry's items in the queue
+    uint32_t le;                 // p is x <= hi (pred_le_form)
+    ms_bwtab tab;                // p specialised per bit width for codes = values
 };
 
+// A work item (96 bytes: one scalar load of 6 dwordx4).
 struct ms_bitem {
+    const uint8_t *data;     // image (BI_RAW: the uncompressed values)
+    const uint64_t *index;   // block index (BI_BLOCKS, BI_FOR, BI_DELTA)
+    const uint64_t *refs;    // for_f references, delta_f tile carries
+    uint64_t first;          // first unit / tile / run / value of the item
+    uint64_t ulim;           // BI_DELTA: units of the column's compressed part
+    uint64_t pad2;
+    uint64_t plo, phi;       // predicate range
     uint32_t entry;
-    uint32_t kind_n;   // kind << 28 | number of units / tiles / runs / values
-    uint64_t first;    // first unit / tile / run / value of the item
+    uint32_t kind_n;         // kind << 28 | number of units / tiles / runs / values
+    uint32_t aux;            // BI_STATIC: bit width
+    uint8_t lg_spb, lg_ipo;  // log2 units per block, blocks per for_f block
+    uint8_t pflags;          // bit 0: negated, bit 1: empty, bit 2: x <= hi form
+    uint8_t pad0;
+    uint64_t pad1[2];
 };
+static_assert(sizeof(ms_bitem) == 96, "work item layout");
 
 // Per-unit predicate on codes for x = ref + code, bw <= 32, no wrap.
 __device__ __forceinline__ uint32_t unit_mode(const ms_pred &p, uint32_t bw, uint64_t ref, uint32_t &lo,
@@ -58,38 +77,72 @@ __device__ __attribute__((noinline, pure)) uint64_t slow_codes(const ms_pred p, 
 }
 
 // One unit of an 8-lane bit-packed layout: `base` is the unit's block,
-// g the thread's code group inside it.
+// g the thread's code group inside it.  Codes that are the values take
+// their per-unit predicate from the entry's table; for_f codes (offsets
+// from `ref`) specialise it per unit.
 template <bool IS_FOR>
-__device__ __forceinline__ void vbp_unit(const ms_pred &p, const uint64_t *base, uint32_t j, uint32_t g,
-                                         uint32_t bw, uint64_t ref, uint64_t &acc)
+__device__ __forceinline__ void vbp_unit(const ms_bentry &en, const ms_pred &p, const uint64_t *base, uint32_t j,
+                                         uint32_t g, uint32_t bw, uint64_t ref, uint64_t &acc)
 {
     if (bw > 32 || (IS_FOR && ref + ((1ull << bw) - 1ull) < ref)) {
         acc += slow_codes(p, base, j, g, bw, ref);
         return;
     }
-    uint32_t lo, span;
-    const uint32_t mode = unit_mode(p, bw, ref, lo, span);
+    uint32_t mode, lo, span;
+    if (!IS_FOR) {
+        mode = en.tab.mode[bw];
+        lo = en.tab.lo[bw];
+        span = en.tab.span[bw];
+    } else if (pred_le_form(p)) {
+        mode = unit_mode_le(p, ref, (1ull << bw) - 1ull, lo, span);
+    } else {
+        mode = unit_mode(p, bw, ref, lo, span);
+    }
     if (mode == SM_NONE) return;
     ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, lo, span, ref, acc);
 }
 
+// The item's n <= 64 units: lane k holds unit u0 + k's index entry (and
+// for_f reference), loaded together.
 template <bool IS_FOR>
-__device__ __forceinline__ void item_blocks(const ms_colv &c, const ms_pred &p, uint64_t u0, uint32_t n, uint32_t j,
-                                            uint32_t gl, uint64_t &acc)
+__device__ __forceinline__ void item_blocks(const ms_bentry &en, const ms_bitem &it, const ms_pred &p, uint32_t n,
+                                            uint32_t t, uint32_t j, uint32_t gl, uint64_t &acc)
 {
-    const uint32_t lg_spb = c.g.lg_spb, lg_ipo = c.g.lg_ipo;
+    const uint32_t lg_spb = it.lg_spb, lg_ipo = it.lg_ipo;
     const uint64_t mask = (1ull << lg_spb) - 1ull;
-    uint64_t e_next = uniform64(c.index[u0 >> lg_spb]);
-    uint64_t r_next = IS_FOR ? uniform64(c.refs[(u0 >> lg_spb) >> lg_ipo]) : 0ull;
-    for (uint64_t u = u0; u < u0 + n; u++) {
-        const uint64_t e = e_next;
-        const uint64_t ref = r_next;
-        if (u + 1 < u0 + n) {
-            e_next = uniform64(c.index[(u + 1) >> lg_spb]);
-            if (IS_FOR) r_next = uniform64(c.refs[((u + 1) >> lg_spb) >> lg_ipo]);
-        }
-        vbp_unit<IS_FOR>(p, (const uint64_t *)(c.data + (e >> 8)), j, (uint32_t)(u & mask) * 8 + gl,
+    const uint64_t u0 = it.first;
+    uint64_t my_e = 0, my_r = 0;
+    if (t < n) {
+        my_e = gload(it.index + ((u0 + t) >> lg_spb));
+        if (IS_FOR) my_r = gload(it.refs + (((u0 + t) >> lg_spb) >> lg_ipo));
+    }
+    for (uint32_t k = 0; k < n; k++) {
+        const uint64_t e = readlane64(my_e, k);
+        const uint64_t ref = IS_FOR ? readlane64(my_r, k) : 0ull;
+        const uint64_t u = u0 + k;
+        vbp_unit<IS_FOR>(en, p, (const uint64_t *)(it.data + (e >> 8)), j, (uint32_t)(u & mask) * 8 + gl,
                          (uint32_t)(e & 0xff), ref, acc);
+    }
+}
+
+// The item's n tiles of 4 units (n <= 16), index entries as above.
+__device__ __forceinline__ void item_delta(const ms_bitem &it, const ms_pred &p, uint32_t n, uint32_t t, uint32_t j,
+                                           uint32_t gl, uint64_t &acc)
+{
+    const uint32_t lg_spb = it.lg_spb;
+    const uint64_t mask = (1ull << lg_spb) - 1ull;
+    const uint64_t u0 = it.first * 4;
+    const uint64_t u_end = it.ulim < u0 + 4ull * n ? it.ulim : u0 + 4ull * n;
+    uint64_t my_e = 0;
+    if (u0 + t < u_end) my_e = gload(it.index + ((u0 + t) >> lg_spb));
+    for (uint32_t q = 0; q < n; q++) {
+        const uint64_t tile = it.first + q;
+        uint64_t carry = gload(it.refs + tile * 8 + j);
+        for (uint64_t u = tile * 4; u < tile * 4 + 4 && u < u_end; u++) {
+            const uint64_t e = readlane64(my_e, (uint32_t)(u - u0));
+            delta_unit_ss<true>(p, (const uint64_t *)(it.data + (e >> 8)), (uint32_t)(e & 0xff), j,
+                                (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
+        }
     }
 }
 
@@ -107,80 +160,113 @@ __device__ __attribute__((noinline, pure)) uint64_t item_generic(const ms_bentry
     return acc;
 }
 
-__global__ __launch_bounds__(WG) void k_batch_select_sum(const ms_bentry *__restrict__ E,
-                                                         const ms_bitem *__restrict__ I,
-                                                         const uint32_t *__restrict__ wave_lo, uint64_t *__restrict__ part)
+// The loop body, instantiated by kernels that differ only in their
+// register budget (waves per SIMD), a choice of the autotuner.
+__device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I,
+                                           uint32_t n_items, uint64_t *__restrict__ part)
 {
     const uint32_t w = blockIdx.x * WAVES + wave_index();
-    const uint32_t i0 = __builtin_amdgcn_readfirstlane(wave_lo[w]);
-    const uint32_t i1 = __builtin_amdgcn_readfirstlane(wave_lo[w + 1]);
+    const uint32_t W = gridDim.x * WAVES;
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & 7, gl = t >> 3;
-    for (uint32_t i = i0; i < i1; i++) {
-        const ms_bitem it = I[i];
+    ms_bitem nxt;
+    if (w < n_items) nxt = I[w];
+    uint32_t run_entry = 0xffffffffu, run_slot = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = w; i < n_items; i += W) {
+        const ms_bitem it = nxt;
+        if (i + W < n_items) nxt = I[i + W];   // the next item's descriptor, one item ahead
+        if (it.entry != run_entry) {
+            if (run_entry != 0xffffffffu) {
+                const uint64_t s = wave_sum_u64(acc);
+                if (t == 0) part[run_slot] = s;
+            }
+            run_entry = it.entry;
+            run_slot = i;
+            acc = 0;
+        }
         const ms_bentry &en = E[it.entry];
-        const ms_colv c = en.c;
-        const ms_pred p = en.p;
+        ms_pred p;
+        p.lo = it.plo;
+        p.hi = it.phi;
+        p.neg = it.pflags & 1;
+        p.empty = (it.pflags >> 1) & 1;
         const uint32_t kind = it.kind_n >> 28, n = it.kind_n & 0x0fffffffu;
-        uint64_t acc = 0;
         switch (kind) {
         case BI_STATIC: {
-            const uint32_t bw = c.g.bw;
+            const uint32_t bw = it.aux;
             for (uint64_t u = it.first; u < it.first + n; u++) {
                 // the code group is loop invariant here; hiding that keeps
                 // the compiler from hoisting per-width offsets of all 32
                 // cases into registers
                 uint32_t g = gl;
                 asm volatile("" : "+v"(g));
-                vbp_unit<false>(p, (const uint64_t *)c.data + u * 8 * bw, j, g, bw, 0, acc);
+                vbp_unit<false>(en, p, (const uint64_t *)it.data + u * 8 * bw, j, g, bw, 0, acc);
             }
             break;
         }
-        case BI_BLOCKS: item_blocks<false>(c, p, it.first, n, j, gl, acc); break;
-        case BI_FOR: item_blocks<true>(c, p, it.first, n, j, gl, acc); break;
-        case BI_DELTA: {
-            const ms_parts pt = parts_of(c);
-            for (uint64_t tile = it.first; tile < it.first + n; tile++)
-                delta_tile_ss(c, p, tile, pt.n_compr >> 9, t, j, gl, acc);
-            break;
-        }
+        case BI_BLOCKS: item_blocks<false>(en, it, p, n, t, j, gl, acc); break;
+        case BI_FOR: item_blocks<true>(en, it, p, n, t, j, gl, acc); break;
+        case BI_DELTA: item_delta(it, p, n, t, j, gl, acc); break;
         case BI_RLE: {
-            const u64x2 *r = (const u64x2 *)c.data;
+            const u64x2 *r = (const u64x2 *)it.data;
             for (uint64_t k = it.first + t; k < it.first + n; k += 64) {
-                const u64x2 vl = r[k];
+                const u64x2 vl = gload(r + k);
                 if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
             }
             break;
         }
         case BI_RAW: {
-            const ms_parts pt = parts_of(c);
-            const uint64_t *x = (const uint64_t *)(c.data + pt.rest_off);
+            const uint64_t *x = (const uint64_t *)it.data;
             for (uint64_t k = it.first + t; k < it.first + n; k += 64) {
-                const uint64_t v = x[k];
+                const uint64_t v = gload(x + k);
                 acc += pred_eval(p, v) ? v : 0ull;
             }
             break;
         }
         default:   // BI_GENERIC
-            acc = item_generic(&E[it.entry], it.first, n, t);
+            acc += item_generic(&E[it.entry], it.first, n, t);
             break;
         }
-        acc = wave_sum_u64(acc);
-        if (t == 0) part[i] = acc;
+    }
+    if (run_entry != 0xffffffffu) {
+        const uint64_t s = wave_sum_u64(acc);
+        if (t == 0) part[run_slot] = s;
     }
 }
 
-// One wave per entry: the sum of its items' partials.
-__global__ __launch_bounds__(WG) void k_batch_finish(const ms_bentry *__restrict__ E, uint64_t n_entries,
-                                                     const uint64_t *__restrict__ part, uint64_t *results)
+__global__ __launch_bounds__(WG) void k_batch_select_sum(const ms_bentry *__restrict__ E,
+                                                         const ms_bitem *__restrict__ I, uint32_t n_items,
+                                                         uint64_t *__restrict__ part)
 {
-    const uint64_t e = (uint64_t)blockIdx.x * WAVES + wave_index();
-    if (e >= n_entries) return;
-    const uint32_t t = threadIdx.x & 63;
+    batch_loop(E, I, n_items, part);
+}
+
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_batch_select_sum_w6(
+    const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I, uint32_t n_items, uint64_t *__restrict__ part)
+{
+    batch_loop(E, I, n_items, part);
+}
+
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_select_sum_w8(
+    const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I, uint32_t n_items, uint64_t *__restrict__ part)
+{
+    batch_loop(E, I, n_items, part);
+}
+
+// One workgroup per entry: the sum of its partial slots [lo, min(hi, lo + W)).
+__global__ __launch_bounds__(WG) void k_batch_finish(const ms_bentry *__restrict__ E, const uint64_t *__restrict__ part,
+                                                     uint32_t W, uint64_t *results)
+{
+    const uint64_t e = blockIdx.x;
+    const uint64_t lo = E[e].item_lo, hi = E[e].item_hi < lo + W ? E[e].item_hi : lo + W;
     uint64_t acc = 0;
-    for (uint64_t k = E[e].item_lo + t; k < E[e].item_hi; k += 64) acc += part[k];
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += WG) acc += part[k];
+    __shared__ uint64_t s_part[WAVES];
     acc = wave_sum_u64(acc);
-    if (t == 0) results[e] = acc;
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) results[e] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
 }
 
 __global__ void k_batch_meta(const ms_bentry *E, uint64_t n, ms_col_meta *out)
@@ -188,9 +274,9 @@ __global__ void k_batch_meta(const ms_bentry *E, uint64_t n, ms_col_meta *out)
     const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
     const ms_dev_meta *m = E[e].c.meta;
-    out[e].count = m->count;
-    out[e].size_used = m->size_used;
-    out[e].size_compr = m->size_compr;
+    out[e].count = gload(&m->count);
+    out[e].size_used = gload(&m->size_used);
+    out[e].size_compr = gload(&m->size_compr);
     out[e].size_alloc = 0;
 }
 
@@ -203,26 +289,25 @@ struct ms_batch {
     std::vector<ms_bentry> E;
     std::vector<ms_col_meta> meta;   // read at the first build
     std::vector<ms_bitem> items;
-    std::vector<uint32_t> wave_lo;
     ms_batch_config cfg;
     bool built = false, have_meta = false;
     uint32_t cus = 0;
     uint64_t bytes = 0;
     ms_bentry *dE = nullptr;
     ms_bitem *dI = nullptr;
-    uint32_t *dW = nullptr;
     uint64_t *dP = nullptr;
-    size_t capE = 0, capI = 0, capW = 0, capP = 0;
+    size_t capE = 0, capI = 0, capP = 0;
     uint32_t grid = 0;
 };
 
-static const ms_batch_config k_default_cfg = {4, 8, 2, 4096, 8192, 256};
+static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0};
 
 static bool cfg_ok(const ms_batch_config &c)
 {
-    return c.wg_per_cu >= 1 && c.wg_per_cu <= 16 && c.units_per_item >= 1 && c.units_per_item <= (1u << 20) &&
-           c.tiles_per_item >= 1 && c.tiles_per_item <= (1u << 20) && c.runs_per_item >= 64 &&
-           c.runs_per_item <= (1u << 27) && c.values_per_item >= 64 && c.values_per_item <= (1u << 27);
+    return c.wg_per_cu >= 1 && c.wg_per_cu <= 16 && c.units_per_item >= 1 && c.units_per_item <= 64 &&
+           c.tiles_per_item >= 1 && c.tiles_per_item <= 16 && c.runs_per_item >= 64 &&
+           c.runs_per_item <= (1u << 27) && c.values_per_item >= 64 && c.values_per_item <= (1u << 27) &&
+           (c.waves_per_simd == 0 || c.waves_per_simd == 6 || c.waves_per_simd == 8);
 }
 
 extern "C" int ms_batch_create(ms_batch **out)
@@ -257,6 +342,8 @@ extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uin
     ms_bentry e = {};
     MS_TRY(colv_of(in, e.c));
     e.p = make_pred(cmp, val);
+    e.le = pred_le_form(e.p);
+    e.tab = make_bwtab(e.p);
     if (entry) *entry = b->E.size();
     b->E.push_back(e);
     b->built = false;
@@ -276,19 +363,21 @@ static int ensure(T **p, size_t &cap, size_t n)
     return MS_OK;
 }
 
-// Cuts the entries into items and the item list into per-wave ranges.
+// Cuts the entries into items, in entry order.
 static void make_queue(ms_batch *b)
 {
     const ms_batch_config &cf = b->cfg;
     b->items.clear();
-    std::vector<double> cost;
     b->bytes = 0;
-    auto add = [&](uint32_t e, uint32_t kind, uint64_t first, uint64_t n, uint64_t per, double unit_bytes,
-                   double fixed) {
+    ms_bitem proto = {};
+    auto add = [&](uint32_t e, uint32_t kind, uint64_t first, uint64_t n, uint64_t per) {
         for (uint64_t k = 0; k < n; k += per) {
             const uint64_t m = std::min<uint64_t>(per, n - k);
-            b->items.push_back(ms_bitem{e, (kind << 28) | (uint32_t)m, first + k});
-            cost.push_back(m * (unit_bytes + fixed) + 256.0);
+            ms_bitem it = proto;
+            it.entry = e;
+            it.kind_n = (kind << 28) | (uint32_t)m;
+            it.first = first + k;
+            b->items.push_back(it);
         }
     };
     for (uint32_t e = 0; e < b->E.size(); e++) {
@@ -298,52 +387,47 @@ static void make_queue(ms_batch *b)
         const ms_parts pt = ms_parts_of(g, m.count, m.size_used, m.size_compr);
         b->bytes += m.size_used;
         en.item_lo = b->items.size();
-        const double uc = cf.unit_cost;
+        proto = ms_bitem{};
+        proto.data = en.c.data;
+        proto.index = en.c.index;
+        proto.refs = en.c.refs;
+        proto.plo = en.p.lo;
+        proto.phi = en.p.hi;
+        proto.pflags = (uint8_t)((en.p.neg ? 1 : 0) | (en.p.empty ? 2 : 0) | (pred_le_form(en.p) ? 4 : 0));
+        proto.lg_spb = g.lg_spb;
+        proto.lg_ipo = g.lg_ipo;
+        proto.aux = g.bw;
+        proto.ulim = pt.n_compr / 512;   // BI_DELTA: units of the compressed part
         switch (g.kind) {
         case MS_RLE:
-            add(e, BI_RLE, 0, m.size_used / 16, cf.runs_per_item, 16.0, 0.0);
+            add(e, BI_RLE, 0, m.size_used / 16, cf.runs_per_item);
             break;
         case MS_UNCOMPR:
             break;
         case MS_STATIC_VBP:
-            if (g.lanes == 8) add(e, BI_STATIC, 0, pt.n_compr / 512, cf.units_per_item, 64.0 * g.bw, uc);
-            else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item, 32.0, 0.0);
+            if (g.lanes == 8) add(e, BI_STATIC, 0, pt.n_compr / 512, cf.units_per_item);
+            else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item);
             break;
         case MS_DYNAMIC_VBP:
         case MS_FOR:
         case MS_DELTA: {
             const uint64_t units = pt.n_compr / 512;
-            const double ub = units ? (double)m.size_compr / units : 0.0;
             const bool blocks = g.leaf == MS_DYNAMIC_VBP && g.lanes == 8 && g.lg_spb != 0xff &&
                                 (g.kind == MS_DYNAMIC_VBP || g.lg_ipo != 0xff);
-            if (g.kind == MS_DELTA) add(e, BI_DELTA, 0, (units + 3) / 4, cf.tiles_per_item, 4 * ub, 6 * uc);
-            else if (blocks) add(e, g.kind == MS_FOR ? BI_FOR : BI_BLOCKS, 0, units, cf.units_per_item, ub, uc);
-            else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item, 32.0, 0.0);
+            if (g.kind == MS_DELTA) add(e, BI_DELTA, 0, (units + 3) / 4, cf.tiles_per_item);
+            else if (blocks) add(e, g.kind == MS_FOR ? BI_FOR : BI_BLOCKS, 0, units, cf.units_per_item);
+            else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item);
             break;
         }
         default:
             break;
         }
         // the uncompressed values: the rest part, or the whole uncompr_f column
-        add(e, BI_RAW, 0, pt.n_rest, cf.values_per_item, 8.0, 0.0);
+        proto.data = en.c.data + pt.rest_off;
+        add(e, BI_RAW, 0, pt.n_rest, cf.values_per_item);
         en.item_hi = b->items.size();
     }
-    // equal-cost contiguous ranges, one per wave
     b->grid = b->cus * cf.wg_per_cu;
-    const uint64_t W = (uint64_t)b->grid * WAVES;
-    double total = 0;
-    for (double c : cost) total += c;
-    b->wave_lo.assign(W + 1, (uint32_t)b->items.size());
-    double run = 0;
-    uint64_t w = 0;
-    b->wave_lo[0] = 0;
-    for (size_t i = 0; i < cost.size() && total > 0; i++) {
-        // item i starts in the range of the wave its cost midpoint falls in
-        const double mid = run + cost[i] / 2;
-        const uint64_t owner = std::min<uint64_t>(W - 1, (uint64_t)(mid / total * W));
-        while (w < owner) b->wave_lo[++w] = (uint32_t)i;
-        run += cost[i];
-    }
 }
 
 static int build(ms_batch *b, hipStream_t s)
@@ -366,10 +450,8 @@ static int build(ms_batch *b, hipStream_t s)
     if (b->items.size() >= (1ull << 32) - 1) return ms_fail(MS_ERR_CAPACITY, "too many work items");
     MS_TRY(ensure(&b->dI, b->capI, b->items.size()));
     MS_TRY(ensure(&b->dP, b->capP, b->items.size()));
-    MS_TRY(ensure(&b->dW, b->capW, b->wave_lo.size()));
     MS_HIP(hipMemcpyAsync(b->dE, b->E.data(), nE * sizeof(ms_bentry), hipMemcpyHostToDevice, s));
     MS_HIP(hipMemcpyAsync(b->dI, b->items.data(), b->items.size() * sizeof(ms_bitem), hipMemcpyHostToDevice, s));
-    MS_HIP(hipMemcpyAsync(b->dW, b->wave_lo.data(), b->wave_lo.size() * 4, hipMemcpyHostToDevice, s));
     // host vectors must outlive the copies
     MS_HIP(hipStreamSynchronize(s));
     b->built = true;
@@ -380,11 +462,13 @@ static int launch(ms_batch *b, uint64_t *d_results, hipStream_t s)
 {
     const uint64_t nE = b->E.size();
     if (nE == 0) return MS_OK;
-    hipLaunchKernelGGL(k_batch_select_sum, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE,
-                       (const ms_bitem *)b->dI, (const uint32_t *)b->dW, b->dP);
+    auto kern = b->cfg.waves_per_simd == 8 ? k_batch_select_sum_w8
+              : b->cfg.waves_per_simd == 6 ? k_batch_select_sum_w6 : k_batch_select_sum;
+    hipLaunchKernelGGL(kern, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE, (const ms_bitem *)b->dI,
+                       (uint32_t)b->items.size(), b->dP);
     MS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_finish, dim3((nE + WAVES - 1) / WAVES), dim3(WG), 0, s, (const ms_bentry *)b->dE, nE,
-                       (const uint64_t *)b->dP, d_results);
+    hipLaunchKernelGGL(k_batch_finish, dim3((uint32_t)nE), dim3(WG), 0, s, (const ms_bentry *)b->dE,
+                       (const uint64_t *)b->dP, b->grid * (uint32_t)WAVES, d_results);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
@@ -424,22 +508,22 @@ extern "C" int ms_batch_tune(ms_batch *b, uint64_t *d_results, uint32_t reps, ms
     ms_batch_config cur = b->cfg;
     float cur_ms;
     MS_TRY(time_cfg(b, cur, d_results, reps, s, &cur_ms));
-    // coordinate descent over the fields, two sweeps
-    static const uint32_t wg[] = {1, 2, 3, 4, 6, 8};
-    static const uint32_t upi[] = {1, 2, 4, 8, 16, 32};
+    // coordinate descent over the fields, at most two sweeps
+    static const uint32_t wg[] = {2, 3, 4, 6, 8};
+    static const uint32_t upi[] = {2, 4, 8, 16, 32, 64};
     static const uint32_t tpi[] = {1, 2, 4, 8};
     static const uint32_t rpi[] = {1024, 4096, 16384};
     static const uint32_t vpi[] = {2048, 8192, 32768};
-    static const uint32_t uco[] = {0, 256, 1024};
+    static const uint32_t wps[] = {0, 6, 8};
     for (int sweep = 0; sweep < 2; sweep++) {
         bool changed = false;
         for (int f = 0; f < 6; f++) {
-            const uint32_t *vals = f == 0 ? wg : f == 1 ? upi : f == 2 ? tpi : f == 3 ? rpi : f == 4 ? vpi : uco;
-            const int nv = f == 0 ? 6 : f == 1 ? 6 : f == 2 ? 4 : 3;
+            const uint32_t *vals = f == 0 ? wps : f == 1 ? wg : f == 2 ? upi : f == 3 ? tpi : f == 4 ? rpi : vpi;
+            const int nv = f == 0 ? 3 : f == 1 ? 5 : f == 2 ? 6 : f == 3 ? 4 : 3;
             for (int k = 0; k < nv; k++) {
                 ms_batch_config c = cur;
-                uint32_t *field = f == 0 ? &c.wg_per_cu : f == 1 ? &c.units_per_item : f == 2 ? &c.tiles_per_item
-                                : f == 3 ? &c.runs_per_item : f == 4 ? &c.values_per_item : &c.unit_cost;
+                uint32_t *field = f == 0 ? &c.waves_per_simd : f == 1 ? &c.wg_per_cu : f == 2 ? &c.units_per_item
+                                : f == 3 ? &c.tiles_per_item : f == 4 ? &c.runs_per_item : &c.values_per_item;
                 if (*field == vals[k]) continue;
                 *field = vals[k];
                 float t;
@@ -490,7 +574,6 @@ extern "C" int ms_batch_free(ms_batch *b)
     (void)hipDeviceSynchronize();
     if (b->dE) (void)hipFree(b->dE);
     if (b->dI) (void)hipFree(b->dI);
-    if (b->dW) (void)hipFree(b->dW);
     if (b->dP) (void)hipFree(b->dP);
     delete b;
     return MS_OK;

@@ -34,7 +34,7 @@ int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s)
 // the ratios are powers of two).  dynamic_vbp takes its per-bit-width
 // predicate from a table built on the host; for_f specialises it per block
 // around the block's reference value.
-template <bool IS_FOR>
+template <bool IS_FOR, bool LEFORM>
 __global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, ms_bwtab tab, uint64_t *res)
 {
     const ms_parts pt = parts_of(c);
@@ -70,11 +70,15 @@ __global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, 
                 }
                 continue;
             }
-            const ms_cpred cp = code_pred(p, ref, maxcode);
-            mode = cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
-                 : cp.clo == 0 ? SM_LE : SM_IN;
-            lo = (uint32_t)cp.clo;
-            span = (uint32_t)cp.cspan;
+            if (LEFORM) {
+                mode = unit_mode_le(p, ref, maxcode, lo, span);
+            } else {
+                const ms_cpred cp = code_pred(p, ref, maxcode);
+                mode = cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
+                     : cp.clo == 0 ? SM_LE : SM_IN;
+                lo = (uint32_t)cp.clo;
+                span = (uint32_t)cp.cspan;
+            }
         } else {
             if (bw > 32) {
 #pragma unroll 1
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(WG) void k_delta_select_sum(ms_colv c, ms_pred p, u
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
     for (uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index(); tile < ntiles; tile += W)
-        delta_tile_ss(c, p, tile, nunits, t, j, gl, acc);
+        delta_tile_ss<false>(c.data, c.index, c.refs, c.g.lg_spb, p, tile, nunits, t, j, gl, acc);
     wg_partial(acc, part);
 }
 
@@ -452,26 +456,6 @@ static bool fast_blocks(const ms_colv &v)
            (v.g.kind == MS_DYNAMIC_VBP || v.g.lg_ipo != 0xff);
 }
 
-// ms_bwtab for codes that are the values (ref 0), bit widths 1..32.
-static ms_bwtab make_bwtab(const ms_pred &p)
-{
-    ms_bwtab t = {};
-    for (uint32_t bw = 1; bw <= 32; bw++) {
-        const uint64_t maxcode = (1ull << bw) - 1ull;
-        uint64_t lo = p.lo, hi = p.hi;
-        const bool in_empty = p.empty || lo > maxcode;
-        if (!in_empty && hi > maxcode) hi = maxcode;
-        const bool in_all = !in_empty && lo == 0 && hi == maxcode;
-        uint32_t mode;
-        if (!p.neg) mode = in_empty ? SM_NONE : in_all ? SM_ALL : lo == 0 ? SM_LE : SM_IN;
-        else mode = in_empty ? SM_ALL : in_all ? SM_NONE : SM_OUT;
-        t.mode[bw] = mode;
-        t.lo[bw] = in_empty ? 0 : (uint32_t)lo;
-        t.span[bw] = in_empty ? 0 : (uint32_t)(hi - lo);
-    }
-    return t;
-}
-
 static bool fast_static(const ms_colv &v)
 {
     return v.g.kind == MS_STATIC_VBP && v.g.lg_lanes != 0xff && v.g.lanes <= 64;
@@ -603,8 +587,12 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         if (fast_blocks(v)) {
             grid = units_grid(nb);
             const ms_bwtab tab = make_bwtab(p);
-            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_select_sum_blocks<true>, dim3(grid), dim3(WG), 0, s, v, p, tab, part);
-            else hipLaunchKernelGGL(k_select_sum_blocks<false>, dim3(grid), dim3(WG), 0, s, v, p, tab, part);
+            if (v.g.kind == MS_FOR && pred_le_form(p))
+                hipLaunchKernelGGL((k_select_sum_blocks<true, true>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
+            else if (v.g.kind == MS_FOR)
+                hipLaunchKernelGGL((k_select_sum_blocks<true, false>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
+            else
+                hipLaunchKernelGGL((k_select_sum_blocks<false, false>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
         } else {
             grid = grid_for(nb, WG * 16ull, 2048);
             hipLaunchKernelGGL(k_select_sum_generic, dim3(grid), dim3(WG), 0, s, v, p, part);

@@ -12,8 +12,19 @@
 #include "ms_internal.h"
 
 #define MS_WAVE 64
+#define MS_GLOBAL __attribute__((address_space(1)))
 
 namespace ms {
+
+// Loads through a global-address-space pointer.  Data pointers read from
+// memory (work-queue items) are flat pointers to the compiler, and flat
+// loads also count on the scalar-load counter, so every scalar-load wait
+// would wait for them as well; all column data lives in global memory.
+template <class T>
+__device__ __forceinline__ T gload(const T *p)
+{
+    return *(const MS_GLOBAL T *)p;
+}
 
 template <int BW>
 struct vbp_traits {
@@ -46,7 +57,7 @@ __device__ __forceinline__ void vbp_load(const uint64_t *__restrict__ base, uint
     const uint32_t need = sh + 8 * BW;
 #pragma unroll
     for (int q = 0; q < NW; q++) {
-        if (BW % 8 == 0 || (uint32_t)(q * 64) < need) w[q] = base[(w0 + q) * S + j];
+        if (BW % 8 == 0 || (uint32_t)(q * 64) < need) w[q] = gload(base + (w0 + q) * S + j);
         else w[q] = 0;
     }
 }
@@ -130,7 +141,7 @@ __device__ __forceinline__ void vbp_load_rt(const uint64_t *__restrict__ base, u
     const uint32_t need = sh + 8 * bw;
 #pragma unroll
     for (int q = 0; q < 5; q++)
-        w[q] = (uint32_t)(q * 64) < need ? base[(w0 + q) * S + j] : 0ull;
+        w[q] = (uint32_t)(q * 64) < need ? gload(base + (w0 + q) * S + j) : 0ull;
 }
 
 __device__ __forceinline__ uint64_t readlane64(uint64_t x, uint32_t lane)
@@ -147,8 +158,8 @@ __device__ __forceinline__ uint64_t vbp_read(const uint64_t *__restrict__ base, 
     const uint64_t b = k * bw;
     const uint64_t w = b >> 6;
     const uint32_t off = (uint32_t)(b & 63);
-    uint64_t v = base[w * S + j] >> off;
-    if (off + bw > 64) v |= base[(w + 1) * S + j] << (64 - off);
+    uint64_t v = gload(base + w * S + j) >> off;
+    if (off + bw > 64) v |= gload(base + (w + 1) * S + j) << (64 - off);
     return bw == 64 ? v : (v & ((1ull << bw) - 1ull));
 }
 

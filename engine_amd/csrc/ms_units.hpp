@@ -65,11 +65,11 @@ __device__ __forceinline__ uint64_t read_compr(const ms_colv &c, uint64_t i)
     }
     const uint64_t b = i / c.g.inner_bs;
     const uint64_t r = i - b * c.g.inner_bs;
-    const uint64_t e = c.index[b];
+    const uint64_t e = gload(c.index + b);
     const uint32_t bw = (uint32_t)(e & 0xff);
     const uint32_t S = c.g.lanes;
     uint64_t v = vbp_read((const uint64_t *)(c.data + (e >> 8)), S, (uint32_t)(r % S), r / S, bw);
-    if (c.g.kind == MS_FOR) v += c.refs[b / (c.g.outer_block / c.g.inner_bs)];
+    if (c.g.kind == MS_FOR) v += gload(c.refs + b / (c.g.outer_block / c.g.inner_bs));
     return v;
 }
 
@@ -213,7 +213,7 @@ __device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint3
     // the last word may lie past this thread's bits (and past the block):
     // column buffers are padded, and those bits are never used
 #pragma unroll
-    for (int q = 0; q < NW; q++) w[q] = p[q * 8];
+    for (int q = 0; q < NW; q++) w[q] = gload(p + q * 8);
     uint32_t c[8];
     vbp_cut32<BW>(w, sh, c);
     using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
@@ -251,6 +251,43 @@ struct ms_bwtab {
     uint32_t span[33];
 };
 
+// ms_bwtab for codes that are the values (ref 0), bit widths 1..32.
+inline ms_bwtab make_bwtab(const ms_pred &p)
+{
+    ms_bwtab t = {};
+    for (uint32_t bw = 1; bw <= 32; bw++) {
+        const uint64_t maxcode = (1ull << bw) - 1ull;
+        uint64_t lo = p.lo, hi = p.hi;
+        const bool in_empty = p.empty || lo > maxcode;
+        if (!in_empty && hi > maxcode) hi = maxcode;
+        const bool in_all = !in_empty && lo == 0 && hi == maxcode;
+        uint32_t mode;
+        if (!p.neg) mode = in_empty ? SM_NONE : in_all ? SM_ALL : lo == 0 ? SM_LE : SM_IN;
+        else mode = in_empty ? SM_ALL : in_all ? SM_NONE : SM_OUT;
+        t.mode[bw] = mode;
+        t.lo[bw] = in_empty ? 0 : (uint32_t)lo;
+        t.span[bw] = in_empty ? 0 : (uint32_t)(hi - lo);
+    }
+    return t;
+}
+
+// Predicates of the form x <= hi (LT, LE: not negated, not empty, lo = 0).
+__host__ __device__ inline bool pred_le_form(const ms_pred &p) { return !p.neg && !p.empty && p.lo == 0; }
+
+// Per-unit mode of such a predicate for x = ref + code, code in [0, maxcode]
+// without wrap-around: a few scalar operations instead of code_pred.
+__device__ __forceinline__ uint32_t unit_mode_le(const ms_pred &p, uint64_t ref, uint64_t maxcode, uint32_t &lo,
+                                                 uint32_t &span)
+{
+    lo = 0;
+    span = 0;
+    if (ref > p.hi) return SM_NONE;
+    const uint64_t s = p.hi - ref;
+    if (s >= maxcode) return SM_ALL;
+    span = (uint32_t)s;
+    return SM_LE;
+}
+
 template <bool CNT>
 __device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
                                                  uint32_t mode, uint32_t lo, uint32_t span, uint64_t ref,
@@ -280,7 +317,7 @@ __device__ __forceinline__ void dec32_unit(const uint64_t *__restrict__ base, ui
     const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
     uint64_t w[NW];
 #pragma unroll
-    for (int q = 0; q < NW; q++) w[q] = p[q * 8];   // may over-read one word; buffers are padded
+    for (int q = 0; q < NW; q++) w[q] = gload(p + q * 8);   // may over-read one word; buffers are padded
     vbp_cut32<BW>(w, (byte0 & 7) * 8, c);
 }
 
@@ -306,7 +343,7 @@ __device__ __forceinline__ void dec32_dispatch(uint32_t bw, const uint64_t *base
 // Deltas of more than 26 bits, in 64-bit arithmetic: rare (unsorted
 // data), kept out of line so it does not set the callers' register budget.
 // Returns {select + sum of the thread's 8 values, the lane's new carry}.
-__device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide(const ms_pred p, const uint64_t *base, uint32_t bw,
+__device__ __forceinline__ u64x2 delta_unit_wide(const ms_pred p, const uint64_t *base, uint32_t bw,
                                                                   uint32_t j, uint32_t g, uint32_t t, uint64_t carry)
 {
     uint64_t d[8];
@@ -333,10 +370,20 @@ __device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide(const ms_pred p
     return r;
 }
 
+// The same, out of line: kernels that decode many layouts keep their
+// register budget (and their scalar loads: it is pure).
+__device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide_call(const ms_pred p, const uint64_t *base,
+                                                                       uint32_t bw, uint32_t j, uint32_t g,
+                                                                       uint32_t t, uint64_t carry)
+{
+    return delta_unit_wide(p, base, bw, j, g, t, carry);
+}
+
 // One 512-value unit of an 8-lane delta_f column: `base` is the unit's
 // block, g = the thread's code group in it, t = lane id in the wave
 // (t = 8 * (g % 8) + j).  carry holds lane j's running sum before the unit
 // and is advanced past it.
+template <bool WIDE_OUT_OF_LINE>
 __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *base, uint32_t bw, uint32_t j,
                                               uint32_t g, uint32_t t, uint64_t &carry, uint64_t &acc)
 {
@@ -352,23 +399,30 @@ __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *
             const uint32_t v = __shfl_up(incl, o, MS_WAVE);
             if (t >= o) incl += v;
         }
-        const uint64_t bt = carry + (incl - tot);   // x_m = bt + d[m]
-        if (bt + tot >= bt) {
-            const ms_cpred cp = code_pred(p, bt, tot);
-            uint32_t lo = 0, span = 0xffffffffu;
-            bool out = false;
-            if (cp.mode == PM_NONE) { lo = 0xffffffffu; span = 0; }
-            else if (cp.mode != PM_ALL) { lo = (uint32_t)cp.clo; span = (uint32_t)cp.cspan; out = cp.mode == PM_OUT; }
-            uint32_t s = 0, cnt = 0;
+        const uint64_t bt = carry + (incl - tot);   // x_m = bt + d[m], d[0] <= .. <= d[7] = tot
+        const uint64_t be = bt + tot;
+        // the thread's values lie in [bt, be]: wholly inside or wholly
+        // outside the predicate's interval is the common case
+        const bool in0 = !p.empty && bt - p.lo <= p.hi - p.lo;
+        const bool in1 = !p.empty && be - p.lo <= p.hi - p.lo;
+        const bool nowrap = be >= bt;
+        if (nowrap && in0 && in1) {
+            if (!p.neg) {
+                uint32_t s = 0;
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const bool take = ((d[m] - lo) <= span) != out;
-                s += take ? d[m] : 0u;
-                cnt += take;
+                for (int m = 0; m < 8; m++) s += d[m];
+                acc += (uint64_t)s + 8ull * bt;
             }
-            acc += (uint64_t)s + (uint64_t)cnt * bt;
-        } else {
+        } else if (nowrap && !in0 && !in1 && !(bt < p.lo && be > p.hi)) {
+            if (p.neg) {
+                uint32_t s = 0;
 #pragma unroll
+                for (int m = 0; m < 8; m++) s += d[m];
+                acc += (uint64_t)s + 8ull * bt;
+            }
+        } else {
+            // a thread whose values straddle the predicate's bounds (rare)
+#pragma unroll 1
             for (int m = 0; m < 8; m++) {
                 const uint64_t x = bt + d[m];
                 acc += pred_eval(p, x) ? x : 0ull;
@@ -377,27 +431,30 @@ __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *
         carry += __shfl(incl, 56 + j, MS_WAVE);
         return;
     }
-    const u64x2 r = delta_unit_wide(p, base, bw, j, g, t, carry);
+    const u64x2 r = WIDE_OUT_OF_LINE ? delta_unit_wide_call(p, base, bw, j, g, t, carry)
+                                     : delta_unit_wide(p, base, bw, j, g, t, carry);
     acc += r.x;
     carry = r.y;
 }
 
 // A delta tile (MS_DELTA_TILE = 4 units) of a column with 512-value-
-// multiple inner blocks; units at or past `nunits` are skipped.
-__device__ __forceinline__ void delta_tile_ss(const ms_colv &c, const ms_pred &p, uint64_t tile, uint64_t nunits,
+// multiple inner blocks (data, block index, tile carries, log2 units per
+// block); units at or past `nunits` are skipped.
+template <bool WIDE_OUT_OF_LINE>
+__device__ __forceinline__ void delta_tile_ss(const uint8_t *data, const uint64_t *index, const uint64_t *refs,
+                                              uint32_t lg_spb, const ms_pred &p, uint64_t tile, uint64_t nunits,
                                               uint32_t t, uint32_t j, uint32_t gl, uint64_t &acc)
 {
-    uint64_t carry = c.refs[tile * 8 + j];
-    const uint32_t lg_spb = c.g.lg_spb;
+    uint64_t carry = gload(refs + tile * 8 + j);
     const uint64_t mask = (1ull << lg_spb) - 1ull;
     const uint64_t u_end = nunits < tile * 4 + 4 ? nunits : tile * 4 + 4;
-    uint64_t e_next = tile * 4 < u_end ? uniform64(c.index[(tile * 4) >> lg_spb]) : 0ull;
+    uint64_t e_next = tile * 4 < u_end ? uniform64(gload(index + ((tile * 4) >> lg_spb))) : 0ull;
 #pragma unroll 1
     for (uint64_t u = tile * 4; u < u_end; u++) {
         const uint64_t e = e_next;
-        if (u + 1 < u_end) e_next = uniform64(c.index[(u + 1) >> lg_spb]);
-        delta_unit_ss(p, (const uint64_t *)(c.data + (e >> 8)), (uint32_t)(e & 0xff), j,
-                      (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
+        if (u + 1 < u_end) e_next = uniform64(gload(index + ((u + 1) >> lg_spb)));
+        delta_unit_ss<WIDE_OUT_OF_LINE>(p, (const uint64_t *)(data + (e >> 8)), (uint32_t)(e & 0xff), j,
+                                        (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
     }
 }
 

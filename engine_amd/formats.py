@@ -62,7 +62,7 @@ class ColMeta(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     """ms_batch_config."""
     _fields_ = [(n, ctypes.c_uint32) for n in ("wg_per_cu", "units_per_item", "tiles_per_item", "runs_per_item",
-                                                "values_per_item", "unit_cost")]
+                                                "values_per_item", "waves_per_simd")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

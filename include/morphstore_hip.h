@@ -280,11 +280,12 @@ typedef struct ms_batch ms_batch;
 /* Launch and work-queue shape of a batch; picked by ms_batch_tune. */
 typedef struct ms_batch_config {
     uint32_t wg_per_cu;        /* persistent workgroups of 256 threads per CU    */
-    uint32_t units_per_item;   /* 512-value units per work item (bit-packed)     */
-    uint32_t tiles_per_item;   /* 2048-value tiles per work item (delta_f)       */
+    uint32_t units_per_item;   /* 512-value units per work item (bit-packed), <= 64 */
+    uint32_t tiles_per_item;   /* 2048-value tiles per work item (delta_f), <= 16   */
     uint32_t runs_per_item;    /* runs per work item (rle_f)                     */
     uint32_t values_per_item;  /* values per work item (uncompressed parts)      */
-    uint32_t unit_cost;        /* modelled fixed cost of a unit, in bytes        */
+    uint32_t waves_per_simd;   /* register budget of the kernel: 0 (the compiler's),
+                                  6 or 8 resident waves per SIMD                 */
 } ms_batch_config;
 
 int ms_batch_create(ms_batch **out);

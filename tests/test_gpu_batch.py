@@ -85,7 +85,8 @@ def test_batch_edge_columns_all_cmps():
         assert int(ea.select_sum(cmp, val, col).get()[0]) == e
 
 
-@pytest.mark.parametrize("cfg", [(1, 1, 1, 64, 64, 0), (8, 32, 8, 16384, 32768, 1024), (3, 3, 3, 100, 1000, 7)])
+@pytest.mark.parametrize("cfg", [(1, 1, 1, 64, 64, 0), (8, 32, 8, 16384, 32768, 6), (3, 3, 3, 100, 1000, 8),
+                                 (16, 64, 16, 64, 64, 0), (2, 64, 16, 64, 64, 8)])
 def test_batch_configs(cfg):
     cv = edge_columns()
     b, expect, _ = make_batch(cv, lambda k, x: MS_CMP_LT, pick_val)
@@ -97,6 +98,12 @@ def test_batch_rejects_bad_config_and_delta_layout():
     b = ea.Batch()
     with pytest.raises(ea.MsError):
         b.config = BatchConfig(0, 1, 1, 64, 64, 0)
+    with pytest.raises(ea.MsError):
+        b.config = BatchConfig(1, 1, 1, 63, 64, 0)
+    with pytest.raises(ea.MsError):
+        b.config = BatchConfig(1, 65, 1, 64, 64, 0)
+    with pytest.raises(ea.MsError):
+        b.config = BatchConfig(1, 1, 1, 64, 64, 7)
     col = ea.morph(default_delta_dynamic_vbp_f(4), ea.upload_values(datagen.sorted_geometric(1, 5000)))
     with pytest.raises(ea.MsError):
         b.enqueue(col, MS_CMP_LT, 100)
