@@ -95,6 +95,106 @@ def timed(fn, steps, warmup, hip):
     return e0.elapsed_ms(e1) / steps
 
 
+def c1_workload(args, ea, hip, datagen):
+    """2^24 values uniform in [0, 2^12) as static_vbp<12,1>: the reference's
+    query shape select(x < 2^11) -> positions -> project -> agg_sum,
+    materialised (three operator calls) and fused (ms_select_sum)."""
+    from engine_amd.formats import static_vbp_f, uncompr_f
+    n = 1 << 24
+    x = datagen.uniform_bits(datagen.BASE_SEED + 0, n, 12)
+    col = ea.morph(static_vbp_f(12, 1), ea.upload_values(x))
+    want = masked_sum(x, 1 << 11)
+    su = col.meta().size_used
+
+    def query():
+        pos = ea.select(CMP_LT, 1 << 11, uncompr_f(), col, 1)
+        dat = ea.project(uncompr_f(), col, pos, 1)
+        return ea.agg_sum(dat)
+    got = int(query().values()[0])
+    res = ea.DeviceU64()
+    fused = int(ea.select_sum(CMP_LT, 1 << 11, col, res).get()[0])
+    if got != want or fused != want:
+        raise RuntimeError(f"c1 sums differ: {got} {fused} {want}")
+    ms_q = timed(query, max(2, args.steps // 2), 1, hip)
+    ms_f = timed(lambda: ea.select_sum(CMP_LT, 1 << 11, col, res), args.steps, args.warmup, hip)
+    log(f"c1: materialised query {ms_q * 1e3:.1f} us, fused {ms_f * 1e3:.1f} us ({su / ms_f / 1e6:.1f} GB/s)")
+    return {"workload": "2^24 uint64 values uniform in [0, 2^12), static_vbp<12,1>, select(x < 2^11) -> "
+                        "project -> agg_sum", "size_used": su, "input_hash": datagen.column_hash(x),
+            "materialised_us": round(ms_q * 1e3, 2), "fused_us": round(ms_f * 1e3, 2),
+            "fused_GBps": round(su / ms_f / 1e6, 1)}
+
+
+def c3_workload(args, ea, hip, datagen):
+    """Two-column plan over 2^k rows (key U[0,2^20), measure U[0,2^16), flag
+    Zipf(1.1) over 8 values, each dynamic_vbp<512,2048,8>):
+    select(flag==3) -> delta+dvbp positions, intersect with select(key<2^18),
+    project(measure), agg_sum -- recorded once, replayed as one hipGraph."""
+    from engine_amd.formats import default_delta_dynamic_vbp_f, dynamic_vbp_f, uncompr_f
+    n = 1 << args.c3_log2n
+    seed = datagen.BASE_SEED + 2
+    f = dynamic_vbp_f(512, 2048, 8)
+    fl = datagen.zipf_flags(seed, n)
+    ky = datagen.uniform_bits(seed + 1, n, 20)
+    ms = datagen.uniform_bits(seed + 2, n, 16)
+    sel = (fl == 3) & (ky < (1 << 18))
+    with np.errstate(over="ignore"):
+        want = int(np.sum(ms[sel], dtype=np.uint64))
+    nsel = int(sel.sum())
+    st = hip.Stream()
+    cols = [ea.morph(f, ea.upload_values(v), st.handle) for v in (fl, ky, ms)]
+    del fl, ky, ms, sel
+    in_bytes = sum(c.meta(st.handle).size_used for c in cols)
+    pos_f = default_delta_dynamic_vbp_f(8)
+
+    def query(stream):
+        p1 = ea.select(0, 3, pos_f, cols[0], 8, stream)
+        p2 = ea.select(CMP_LT, 1 << 18, pos_f, cols[1], 8, stream)
+        i = ea.intersect_sorted(pos_f, p1, p2, stream)
+        m = ea.project(uncompr_f(), cols[2], i, 8, stream)
+        return ea.agg_sum(m, stream), (p1, p2, i, m)
+    r, keep = query(st.handle)
+    got = int(r.values(st.handle)[0])
+    if got != want:
+        raise RuntimeError(f"c3 sum {got} != {want}")
+    inter = [k.meta(st.handle).size_used for k in keep]
+    e0, e1 = hip.Event(), hip.Event()
+    st.synchronize()
+    e0.record(st.handle)
+    for _ in range(3):
+        query(st.handle)
+    e1.record(st.handle)
+    ms_eager = e0.elapsed_ms(e1) / 3
+    plan = ea.Plan()
+    with plan.record(st.handle):
+        query(st.handle)
+    with plan.record(st.handle):
+        res, keep2 = query(st.handle)
+    plan.launch(st.handle)
+    st.synchronize()
+    if int(res.values(st.handle)[0]) != want:
+        raise RuntimeError("c3 plan replay differs")
+    plan.launch(st.handle)
+    st.synchronize()
+    e0.record(st.handle)
+    reps = max(2, args.steps // 2)
+    for _ in range(reps):
+        plan.launch(st.handle)
+    e1.record(st.handle)
+    ms_graph = e0.elapsed_ms(e1) / reps
+    if int(res.values(st.handle)[0]) != want:
+        raise RuntimeError("c3 plan replay differs after timing")
+    # algorithmic bytes: inputs once, compressed position lists written and read
+    algo = in_bytes + 2 * sum(inter[:3]) + inter[3] * 2
+    log(f"c3: {n} rows, {nsel} selected, eager {ms_eager:.3f} ms, plan (hipGraph) {ms_graph:.3f} ms, "
+        f"{algo / ms_graph / 1e6:.1f} GB/s algorithmic")
+    plan.free()
+    return {"workload": f"2^{args.c3_log2n} rows x 3 dynamic_vbp columns, select(flag==3) & select(key<2^18) "
+                        f"-> intersect -> project(measure) -> agg_sum as one hipGraph",
+            "input_bytes": in_bytes, "intermediate_bytes": inter, "selected": nsel,
+            "eager_ms": round(ms_eager, 3), "graph_ms": round(ms_graph, 3),
+            "algorithmic_GBps": round(algo / ms_graph / 1e6, 1)}
+
+
 def c4_workload(args, ea, hip, datagen):
     """Uneven mixed-format columns: per-column calls vs one tuned batch."""
     t0 = time.perf_counter()
@@ -191,9 +291,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=27)
     ap.add_argument("--c4-cols", type=int, default=4096, help="columns of the uneven workload (0: skip it)")
+    ap.add_argument("--c3-log2n", type=int, default=28, help="rows of the two-column plan workload (0: skip it)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2^24-value static_vbp query")
     ap.add_argument("--c5-cols", type=int, default=1024,
                     help="columns of 64 x 2^16 values of the learned-selection workload (0: skip it)")
     ap.add_argument("--c5-train", type=int, default=1_000_000, help="labelled training chunks of the selector")
+    ap.add_argument("--batch-config", default=None,
+                    help="JSON ms_batch_config for the sweep's batch instead of autotuning (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="only the layouts whose name starts with this (profiling)")
     args = ap.parse_args()
@@ -213,7 +317,11 @@ def main():
         if args.only and not name.startswith(args.only):
             continue
         t0 = time.perf_counter()
-        col = ea.morph(fmt, ea.upload_values(x)) if fmt.kind else ea.upload_values(x)
+        raw = ea.upload_values(x)
+        raw.meta()
+        up_s = time.perf_counter() - t0
+        col = ea.morph(fmt, raw) if fmt.kind else raw
+        del raw
         m = col.meta()
         expect = masked_sum(x, val)
         got = int(ea.select_sum(CMP_LT, val, col).get()[0])
@@ -221,7 +329,7 @@ def main():
             log(f"MISMATCH {name}: gpu {got} numpy {expect}")
             sys.exit(1)
         cols.append(dict(name=name, fmt=fmt, col=col, val=val, size_used=m.size_used, count=m.count,
-                         hash=datagen.column_hash(x), sum=expect))
+                         hash=datagen.column_hash(x), sum=expect, upload_GBps=8 * n / up_s / 1e9))
         log(f"{name}: n={n} sizeUsed={m.size_used} ({8 * n / m.size_used:.2f}x) prepared in "
             f"{time.perf_counter() - t0:.1f}s, sum verified")
         del x
@@ -249,7 +357,12 @@ def main():
     if res.get().tolist() != [c["sum"] for c in cols]:
         log("MISMATCH batched step")
         sys.exit(1)
-    cfg = batch.tune(res, reps=5)
+    if args.batch_config:
+        from engine_amd.formats import BatchConfig
+        batch.config = BatchConfig(**json.loads(args.batch_config))
+        cfg = batch.config
+    else:
+        cfg = batch.tune(res, reps=5)
     batch.run(res)
     if res.get().tolist() != [c["sum"] for c in cols]:
         log("MISMATCH batched step (tuned)")
@@ -282,13 +395,21 @@ def main():
                   "pct_hbm_peak": round(100 * value_plain / HBM_PEAK_GBS, 2)},
         "batched_vs_plain": round(ms_plain / ms_step, 3),
         "uncompr_equiv_GBps": round(nc * 8 * n / ms_step / 1e6, 1),
+        "host_upload_GBps": round(statistics.median(c["upload_GBps"] for c in cols), 2) if cols else None,
         "wall_s_timed": round(wall, 3),
         "roofline": {"bound": "hbm", "kernel": "k_batch_select_sum (+ k_batch_finish)",
                      "achieved": round(value, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(value / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
-    if args.c4_cols > 0 and not args.only:
-        out["uneven_columns"] = c4_workload(args, ea, hip, datagen)
+    extra = [("static_vbp_query", c1_workload, not args.no_c1), ("plan_query", c3_workload, args.c3_log2n > 0),
+             ("uneven_columns", c4_workload, args.c4_cols > 0)]
+    for key, fn, on in extra:
+        if on and not args.only:
+            try:
+                out[key] = fn(args, ea, hip, datagen)
+            except Exception as e:     # the sweep's line above stands on its own
+                log(f"{key} workload failed: {e!r}")
+                out[key] = {"error": repr(e)}
     if args.c5_cols > 0 and not args.only:
         from engine_amd import learn
         try:

@@ -254,19 +254,25 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     batch_loop(E, I, n_items, part);
 }
 
-// One workgroup per entry: the sum of its partial slots [lo, min(hi, lo + W)).
-__global__ __launch_bounds__(WG) void k_batch_finish(const ms_bentry *__restrict__ E, const uint64_t *__restrict__ part,
-                                                     uint32_t W, uint64_t *results)
+// One workgroup (256 or 1024 threads) per entry: the sum of its partial
+// slots [lo, min(hi, lo + W)).
+__global__ __launch_bounds__(1024) void k_batch_finish(const ms_bentry *__restrict__ E,
+                                                       const uint64_t *__restrict__ part, uint32_t W,
+                                                       uint64_t *results)
 {
     const uint64_t e = blockIdx.x;
     const uint64_t lo = E[e].item_lo, hi = E[e].item_hi < lo + W ? E[e].item_hi : lo + W;
     uint64_t acc = 0;
-    for (uint64_t k = lo + threadIdx.x; k < hi; k += WG) acc += part[k];
-    __shared__ uint64_t s_part[WAVES];
+    for (uint64_t k = lo + threadIdx.x; k < hi; k += blockDim.x) acc += part[k];
+    __shared__ uint64_t s_part[16];
     acc = wave_sum_u64(acc);
     if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) results[e] = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    if (threadIdx.x == 0) {
+        uint64_t sum = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; w++) sum += s_part[w];
+        results[e] = sum;
+    }
 }
 
 __global__ void k_batch_meta(const ms_bentry *E, uint64_t n, ms_col_meta *out)
@@ -298,6 +304,7 @@ struct ms_batch {
     uint64_t *dP = nullptr;
     size_t capE = 0, capI = 0, capP = 0;
     uint32_t grid = 0;
+    uint32_t finish_threads = WG;   // 1024 when an entry has thousands of partials
 };
 
 static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0};
@@ -428,6 +435,10 @@ static void make_queue(ms_batch *b)
         en.item_hi = b->items.size();
     }
     b->grid = b->cus * cf.wg_per_cu;
+    uint64_t most = 0;
+    for (const ms_bentry &en : b->E) most = std::max<uint64_t>(most, en.item_hi - en.item_lo);
+    most = std::min<uint64_t>(most, (uint64_t)b->grid * WAVES);
+    b->finish_threads = most > 2048 ? 1024 : WG;
 }
 
 static int build(ms_batch *b, hipStream_t s)
@@ -467,7 +478,7 @@ static int launch(ms_batch *b, uint64_t *d_results, hipStream_t s)
     hipLaunchKernelGGL(kern, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE, (const ms_bitem *)b->dI,
                        (uint32_t)b->items.size(), b->dP);
     MS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_batch_finish, dim3((uint32_t)nE), dim3(WG), 0, s, (const ms_bentry *)b->dE,
+    hipLaunchKernelGGL(k_batch_finish, dim3((uint32_t)nE), dim3(b->finish_threads), 0, s, (const ms_bentry *)b->dE,
                        (const uint64_t *)b->dP, b->grid * (uint32_t)WAVES, d_results);
     MS_HIP(hipGetLastError());
     return MS_OK;
