@@ -62,6 +62,8 @@ ABI = {
     "ms_batch_set_config": (ctypes.c_int, [VP, P(BatchConfig)]),
     "ms_batch_stats": (ctypes.c_int, [VP, P(U64), P(U64), P(U64)]),
     "ms_batch_free": (ctypes.c_int, [VP]),
+    "ms_batch_timing": (ctypes.c_int, [VP, U32]),
+    "ms_batch_kernel_times": (ctypes.c_int, [VP, VP, P(U32)]),
     "ms_gen_chunks": (ctypes.c_int, [VP, VP, U64, VP, U32, VP]),
     "ms_chunk_stats": (ctypes.c_int, [VP, U64, VP, VP, VP, VP]),
     "ms_chunk_features": (ctypes.c_int, [VP, U64, VP, VP, VP]),
@@ -328,6 +330,17 @@ class Batch:
     @config.setter
     def config(self, c):
         check(lib().ms_batch_set_config(self.h, ctypes.byref(c)))
+
+    def timing(self, slots):
+        """Record the persistent kernel's duration of the next runs (ring of
+        `slots`; 0: off)."""
+        check(lib().ms_batch_timing(self.h, slots))
+
+    def kernel_times(self, n):
+        ms = np.zeros(max(n, 1), np.float32)
+        cnt = U32(n)
+        check(lib().ms_batch_kernel_times(self.h, ms.ctypes.data, ctypes.byref(cnt)))
+        return ms[:cnt.value].tolist()
 
     def stats(self):
         e, i, b = U64(), U64(), U64()

@@ -305,6 +305,8 @@ struct ms_batch {
     size_t capE = 0, capI = 0, capP = 0;
     uint32_t grid = 0;
     uint32_t finish_threads = WG;   // 1024 when an entry has thousands of partials
+    std::vector<hipEvent_t> ev;     // kernel timing: a ring of (start, end) pairs
+    uint32_t ev_next = 0, ev_used = 0;
 };
 
 static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0};
@@ -475,8 +477,16 @@ static int launch(ms_batch *b, uint64_t *d_results, hipStream_t s)
     if (nE == 0) return MS_OK;
     auto kern = b->cfg.waves_per_simd == 8 ? k_batch_select_sum_w8
               : b->cfg.waves_per_simd == 6 ? k_batch_select_sum_w6 : k_batch_select_sum;
+    const uint32_t slots = (uint32_t)b->ev.size() / 2;
+    const uint32_t k = slots ? b->ev_next % slots : 0;
+    if (slots) MS_HIP(hipEventRecord(b->ev[2 * k], s));
     hipLaunchKernelGGL(kern, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE, (const ms_bitem *)b->dI,
                        (uint32_t)b->items.size(), b->dP);
+    if (slots) {
+        MS_HIP(hipEventRecord(b->ev[2 * k + 1], s));
+        b->ev_next++;
+        b->ev_used = std::min(b->ev_used + 1, slots);
+    }
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_batch_finish, dim3((uint32_t)nE), dim3(b->finish_threads), 0, s, (const ms_bentry *)b->dE,
                        (const uint64_t *)b->dP, b->grid * (uint32_t)WAVES, d_results);
@@ -579,10 +589,49 @@ extern "C" int ms_batch_stats(const ms_batch *b, uint64_t *entries, uint64_t *it
     return MS_OK;
 }
 
+static void drop_events(ms_batch *b)
+{
+    for (hipEvent_t e : b->ev) (void)hipEventDestroy(e);
+    b->ev.clear();
+    b->ev_next = b->ev_used = 0;
+}
+
+extern "C" int ms_batch_timing(ms_batch *b, uint32_t slots)
+{
+    if (!b) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (slots > 4096) return ms_fail(MS_ERR_INVALID, "at most 4096 timing slots");
+    MS_HIP(hipDeviceSynchronize());
+    drop_events(b);
+    for (uint32_t i = 0; i < 2 * slots; i++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            drop_events(b);
+            return ms_fail(MS_ERR_HIP, "hipEventCreate");
+        }
+        b->ev.push_back(e);
+    }
+    return MS_OK;
+}
+
+extern "C" int ms_batch_kernel_times(ms_batch *b, float *ms, uint32_t *n)
+{
+    if (!b || !ms || !n) return ms_fail(MS_ERR_INVALID, "null argument");
+    const uint32_t slots = (uint32_t)b->ev.size() / 2;
+    const uint32_t cnt = std::min(*n, b->ev_used);
+    for (uint32_t i = 0; i < cnt; i++) {
+        const uint32_t k = (b->ev_next - cnt + i) % slots;
+        MS_HIP(hipEventSynchronize(b->ev[2 * k + 1]));
+        MS_HIP(hipEventElapsedTime(&ms[i], b->ev[2 * k], b->ev[2 * k + 1]));
+    }
+    *n = cnt;
+    return MS_OK;
+}
+
 extern "C" int ms_batch_free(ms_batch *b)
 {
     if (!b) return MS_OK;
     (void)hipDeviceSynchronize();
+    drop_events(b);
     if (b->dE) (void)hipFree(b->dE);
     if (b->dI) (void)hipFree(b->dI);
     if (b->dP) (void)hipFree(b->dP);

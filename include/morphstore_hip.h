@@ -313,6 +313,14 @@ int ms_batch_set_config(ms_batch *b, const ms_batch_config *cfg);
  * (the sum of sizeUsed over the entries). */
 int ms_batch_stats(const ms_batch *b, uint64_t *entries, uint64_t *items, uint64_t *bytes);
 
+/* Kernel timing: with slots > 0 every run records HIP events around the
+ * persistent kernel alone (a ring of `slots` launches; 0 turns it off). */
+int ms_batch_timing(ms_batch *b, uint32_t slots);
+
+/* Durations in ms of the latest min(*n, recorded) timed launches, oldest
+ * first; *n receives their number.  Waits for those launches. */
+int ms_batch_kernel_times(ms_batch *b, float *ms, uint32_t *n);
+
 int ms_batch_free(ms_batch *b);
 
 /* ------------------------------------------------------------------ */
