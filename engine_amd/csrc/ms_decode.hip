@@ -271,7 +271,7 @@ __global__ __launch_bounds__(WG) void k_delta_select_sum(ms_colv c, ms_pred p, u
     const uint64_t nunits = pt.n_compr >> 9;
     const uint64_t ntiles = (nunits + 3) >> 2;
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t & 7, gl = t >> 3;
+    const uint32_t j = t >> 3, gl = t & 7;   // delta lane j in 8 consecutive threads
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
     for (uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index(); tile < ntiles; tile += W)

@@ -304,11 +304,52 @@ __device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *ba
 
 // ------------------------------------------------------------------
 // delta_f select + sum, one wave per unit of deltas with the lanes'
-// running sums ("carries") at the unit's start.  Deltas of up to 26 bits
-// are decoded and scanned in 32-bit arithmetic (64 codes of a lane sum to
-// less than 2^32); the predicate is specialised per thread around the
-// thread's first running sum, so the 8 values cost 32-bit compares only.
+// running sums ("carries") at the unit's start.  Thread t handles delta
+// lane j = t / 8 and code group gl = t % 8, so each lane's 64 codes sit in
+// 8 consecutive threads and its running sum is a scan within 8-thread
+// groups done with DPP row shifts (no LDS traffic).  Deltas of up to 26
+// bits are decoded and scanned in 32-bit arithmetic (64 codes of a lane
+// sum to less than 2^32); a thread's 8 values are then wholly inside or
+// outside the predicate's range except at its edges.
 // ------------------------------------------------------------------
+
+// Value of the lane k (1, 2 or 4) places below within its 8-thread group,
+// or 0 (DPP row_shr with bound control, masked at the group's start).
+template <int K>
+__device__ __forceinline__ uint32_t group8_up(uint32_t x, uint32_t t)
+{
+    const uint32_t y = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + K, 0xf, 0xf, true);
+    return (t & 7) >= (uint32_t)K ? y : 0u;
+}
+
+// Inclusive scan within each 8-thread group.
+__device__ __forceinline__ uint32_t group8_scan(uint32_t x, uint32_t t)
+{
+    x += group8_up<1>(x, t);
+    x += group8_up<2>(x, t);
+    x += group8_up<4>(x, t);
+    return x;
+}
+
+// Sum over each 8-thread group, in every thread of the group (quad swaps,
+// then the half-row mirror pairs the two quads).
+__device__ __forceinline__ uint32_t group8_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xb1, 0xf, 0xf, false);    // quad_perm [1,0,3,2]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4e, 0xf, 0xf, false);    // quad_perm [2,3,0,1]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xf, 0xf, false);   // row_half_mirror
+    return x;
+}
+
+__device__ __forceinline__ uint64_t group8_scan64(uint64_t x, uint32_t t)
+{
+#pragma unroll
+    for (uint32_t o = 1; o < 8; o <<= 1) {
+        const uint64_t v = __shfl_up(x, o, MS_WAVE);
+        if ((t & 7) >= o) x += v;
+    }
+    return x;
+}
 template <int BW>
 __device__ __forceinline__ void dec32_unit(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t (&c)[8])
 {
@@ -351,12 +392,7 @@ __device__ __forceinline__ u64x2 delta_unit_wide(const ms_pred p, const uint64_t
 #pragma unroll
     for (int m = 1; m < 8; m++) d[m] += d[m - 1];
     const uint64_t tot = d[7];
-    uint64_t incl = tot;
-#pragma unroll
-    for (uint32_t o = 8; o < 64; o <<= 1) {
-        const uint64_t v = __shfl_up(incl, o, MS_WAVE);
-        if (t >= o) incl += v;
-    }
+    const uint64_t incl = group8_scan64(tot, t);
     const uint64_t bt = carry + incl - tot;
     uint64_t acc = 0;
 #pragma unroll
@@ -366,7 +402,7 @@ __device__ __forceinline__ u64x2 delta_unit_wide(const ms_pred p, const uint64_t
     }
     u64x2 r;
     r.x = acc;
-    r.y = carry + __shfl(incl, 56 + j, MS_WAVE);
+    r.y = carry + __shfl(incl, (t & ~7u) + 7, MS_WAVE);
     return r;
 }
 
@@ -381,7 +417,7 @@ __device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide_call(const ms_p
 
 // One 512-value unit of an 8-lane delta_f column: `base` is the unit's
 // block, g = the thread's code group in it, t = lane id in the wave
-// (t = 8 * (g % 8) + j).  carry holds lane j's running sum before the unit
+// (t = 8 * j + g % 8).  carry holds lane j's running sum before the unit
 // and is advanced past it.
 template <bool WIDE_OUT_OF_LINE>
 __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *base, uint32_t bw, uint32_t j,
@@ -393,12 +429,7 @@ __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *
 #pragma unroll
         for (int m = 1; m < 8; m++) d[m] += d[m - 1];
         const uint32_t tot = d[7];
-        uint32_t incl = tot;
-#pragma unroll
-        for (uint32_t o = 8; o < 64; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o, MS_WAVE);
-            if (t >= o) incl += v;
-        }
+        const uint32_t incl = group8_scan(tot, t);
         const uint64_t bt = carry + (incl - tot);   // x_m = bt + d[m], d[0] <= .. <= d[7] = tot
         const uint64_t be = bt + tot;
         // the thread's values lie in [bt, be]: wholly inside or wholly
@@ -428,7 +459,7 @@ __device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *
                 acc += pred_eval(p, x) ? x : 0ull;
             }
         }
-        carry += __shfl(incl, 56 + j, MS_WAVE);
+        carry += group8_sum(tot);
         return;
     }
     const u64x2 r = WIDE_OUT_OF_LINE ? delta_unit_wide_call(p, base, bw, j, g, t, carry)
