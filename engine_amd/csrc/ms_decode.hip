@@ -675,10 +675,115 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
 
 __global__ void k_count_of(const ms_dev_meta *m, uint64_t *dst) { *dst = m->count; }
 
+__global__ void k_units_of(const ms_dev_meta *m, uint64_t *dst) { *dst = (m->count + 511) / 512; }
+
+// Flags of unit u's 512 values (bit t of the result's word m = value
+// 64 * (t >> 3) + 8 * m + (t & 7) of the unit, lane-major per thread):
+// decoded from the compressed part, or read raw from the rest.
+__device__ __forceinline__ void unit_flags(const ms_colv &c, const ms_parts &pt, const ms_pred &p, uint64_t u,
+                                           uint32_t t, uint64_t (&B)[8])
+{
+    const uint32_t j = t & 7, gl = t >> 3;
+    const uint64_t ncu = pt.n_compr >> 9;
+    uint64_t d[8];
+    bool ok[8];
+    if (u < ncu) {
+        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
+        const uint64_t e = uniform64(gload(c.index + b));
+        const uint64_t ref = c.g.kind == MS_FOR ? uniform64(gload(c.refs + (b >> c.g.lg_ipo))) : 0ull;
+        dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            d[m] += ref;
+            ok[m] = true;
+        }
+    } else {
+        // the raw rest: value i of the unit is rest value (u - ncu) * 512 + i
+        const uint64_t *x = (const uint64_t *)(c.data + pt.rest_off);
+        const uint64_t r0 = (u - ncu) * 512;
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const uint64_t r = r0 + 64 * gl + 8 * m + j;
+            ok[m] = r < pt.n_rest;
+            d[m] = ok[m] ? gload(x + r) : 0ull;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; m++) B[m] = __ballot(ok[m] && pred_eval(p, d[m]));
+}
+
+// Rows of 64 consecutive values of a unit from the per-m ballots: value
+// 64 r + 8 m + j is bit (8 r + j) of B[m]; row r's bit (8 m + j).
+__device__ __forceinline__ uint64_t unit_row(const uint64_t (&B)[8], uint32_t r)
+{
+    uint64_t F = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) F |= ((B[m] >> (8 * r)) & 0xffull) << (8 * m);
+    return F;
+}
+
+__global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, const uint64_t *d_units, uint64_t *cnt)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t nunits = *d_units;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+        uint64_t B[8];
+        unit_flags(c, pt, p, u, t, B);
+        uint32_t n = 0;
+#pragma unroll
+        for (int m = 0; m < 8; m++) n += __popcll(B[m]);
+        if (t == 0) cnt[u] = n;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_sel_compact_blocks(ms_colv c, ms_pred p, const uint64_t *d_units,
+                                                           const uint64_t *off, uint64_t *pos, uint64_t *d_cnt)
+{
+    const ms_parts pt = parts_of(c);
+    const uint64_t nunits = *d_units;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    const uint64_t below = t ? (~0ull >> (64 - t)) : 0ull;
+    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+        uint64_t B[8];
+        unit_flags(c, pt, p, u, t, B);
+        uint64_t o = off[u];
+#pragma unroll
+        for (uint32_t r = 0; r < 8; r++) {
+            const uint64_t F = unit_row(B, r);
+            if ((F >> t) & 1) pos[o + __popcll(F & below)] = u * 512 + 64 * r + t;
+            o += __popcll(F);
+        }
+        if (u == nunits - 1 && t == 0) *d_cnt = o;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && nunits == 0) *d_cnt = 0;
+}
+
 int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc)
 {
     hipStream_t s = sc.s;
     const uint64_t nb = in->count_bound;
+    ms_colv v;
+    MS_TRY(colv_of(in, v));
+    if ((v.g.kind == MS_DYNAMIC_VBP || v.g.kind == MS_FOR) && fast_blocks(v)) {
+        // decode, test and compact straight from the compressed units
+        const uint64_t units = nb / 512 + 1;
+        uint64_t *d_u = (uint64_t *)sc.get(64);
+        uint64_t *cnt = (uint64_t *)sc.get(units * 8 + 64);
+        uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
+        if (!d_u || !cnt || !off) return ms_fail(MS_ERR_NOMEM, "select scratch");
+        hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
+        const dim3 grid(units_grid(nb));
+        hipLaunchKernelGGL(k_sel_count_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt);
+        MS_HIP(hipGetLastError());
+        MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 0, units, sc));
+        hipLaunchKernelGGL(k_sel_compact_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u,
+                           (const uint64_t *)off, d_pos, d_cnt);
+        MS_HIP(hipGetLastError());
+        return MS_OK;
+    }
     const uint64_t *x;
     if (in->g.kind == MS_UNCOMPR) {
         x = (const uint64_t *)in->data;
@@ -693,12 +798,125 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
     uint64_t *cnt = (uint64_t *)sc.get(units * 8 + 64);
     uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
     if (!d_n || !cnt || !off) return ms_fail(MS_ERR_NOMEM, "select scratch");
+    uint64_t *d_u = (uint64_t *)sc.get(64);
+    if (!d_u) return ms_fail(MS_ERR_NOMEM, "select scratch");
     hipLaunchKernelGGL(k_count_of, dim3(1), dim3(1), 0, s, in->meta, d_n);
+    hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
     hipLaunchKernelGGL(k_sel_count, dim3(grid_for(units, WAVES * 2ull)), dim3(WG), 0, s, x, d_n, p, cnt, units);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan(cnt, off, units, sc));
+    MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 0, units, sc));
     hipLaunchKernelGGL(k_sel_compact, dim3(grid_for(units, WAVES * 2ull)), dim3(WG), 0, s, x, d_n, p, off, d_pos,
                        units, d_cnt);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------
+// Exclusive scan of the first min(*d_n + extra, bound) values, sized from
+// the device count: G workgroups each take one contiguous chunk (sum,
+// scan of the G sums, chunk scan with its offset).  Arrays allocated for
+// a host bound are scanned only as far as they are used.
+// ------------------------------------------------------------------
+__device__ __forceinline__ void chunk_of(const uint64_t *d_n, uint64_t extra, uint64_t bound, uint64_t &lo,
+                                         uint64_t &hi)
+{
+    uint64_t n = *d_n + extra;
+    if (n > bound) n = bound;
+    const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    lo = (uint64_t)blockIdx.x * chunk;
+    hi = lo + chunk < n ? lo + chunk : n;
+    if (lo > hi) lo = hi;
+}
+
+__device__ __forceinline__ uint64_t block_sum(uint64_t v)
+{
+    __shared__ uint64_t s_w[WAVES];
+    v = wave_sum_u64(v);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t t = 0;
+#pragma unroll
+    for (int w = 0; w < WAVES; w++) t += s_w[w];
+    __syncthreads();
+    return t;
+}
+
+// Exclusive scan over the workgroup; *total receives the sum.
+__device__ __forceinline__ uint64_t block_exscan(uint64_t v, uint64_t *total)
+{
+    __shared__ uint64_t s_w[WAVES];
+    const uint32_t t = threadIdx.x & 63;
+    uint64_t incl = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(incl, o, MS_WAVE);
+        if (t >= o) incl += y;
+    }
+    if (t == 63) s_w[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint64_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < WAVES; w++) {
+        const uint64_t x = s_w[w];
+        if (w < (threadIdx.x >> 6)) before += x;
+        all += x;
+    }
+    __syncthreads();
+    *total = all;
+    return before + incl - v;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_part(const uint64_t *in, const uint64_t *d_n, uint64_t extra,
+                                                  uint64_t bound, uint64_t *part)
+{
+    uint64_t lo, hi;
+    chunk_of(d_n, extra, bound, lo, hi);
+    uint64_t acc = 0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += WG) acc += in[i];
+    acc = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_top(uint64_t *part, uint32_t G)
+{
+    uint64_t carry = 0;
+    for (uint32_t base = 0; base < G; base += WG) {
+        const uint32_t i = base + threadIdx.x;
+        const uint64_t v = i < G ? part[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_exscan(v, &tot);
+        if (i < G) part[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_scan_apply(const uint64_t *in, uint64_t *out, const uint64_t *d_n,
+                                                   uint64_t extra, uint64_t bound, const uint64_t *part)
+{
+    uint64_t lo, hi;
+    chunk_of(d_n, extra, bound, lo, hi);
+    uint64_t carry = part[blockIdx.x];
+    for (uint64_t base = lo; base < hi; base += WG) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = i < hi ? in[i] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_exscan(v, &tot);
+        if (i < hi) out[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_t *d_n, uint64_t extra,
+                            uint64_t bound, scratch &sc)
+{
+    if (bound == 0) return MS_OK;
+    const uint32_t G = grid_for(bound, 4096, 1024);
+    uint64_t *part = (uint64_t *)sc.get(G * 8 + 64);
+    if (!part) return ms_fail(MS_ERR_NOMEM, "scan scratch");
+    hipLaunchKernelGGL(k_scan_part, dim3(G), dim3(WG), 0, sc.s, d_in, d_n, extra, bound, part);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(WG), 0, sc.s, part, G);
+    hipLaunchKernelGGL(k_scan_apply, dim3(G), dim3(WG), 0, sc.s, d_in, d_out, d_n, extra, bound,
+                       (const uint64_t *)part);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

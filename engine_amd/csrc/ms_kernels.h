@@ -42,6 +42,9 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
 int launch_delta_carries(const ms_col *in, scratch &sc);
 // exclusive prefix sums of n u64 values (in-place allowed)
 int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc);
+// the same over the first min(*d_n + extra, bound) values (device count)
+int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_t *d_n, uint64_t extra,
+                            uint64_t bound, scratch &sc);
 
 // encode side -------------------------------------------------------
 // Emits the m values d_v[0 .. *d_m) into `out` (already allocated, format

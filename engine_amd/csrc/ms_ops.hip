@@ -96,17 +96,14 @@ __global__ __launch_bounds__(WG) void k_isect_flags(const uint64_t *a, const uin
                                                     const uint64_t *d_nb, uint64_t bound, uint64_t *flag)
 {
     const uint64_t na = *d_na, nb = *d_nb;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < bound; i += (uint64_t)gridDim.x * WG) {
-        uint64_t f = 0;
-        if (i < na) {
-            const uint64_t v = a[i];
-            const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
-            const uint64_t lb = lower_bound(b, nb, v);
-            uint64_t cnt = 0;
-            if (lb < nb && b[lb] == v) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
-            f = rank < cnt;
-        }
-        flag[i] = f;
+    (void)bound;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < na; i += (uint64_t)gridDim.x * WG) {
+        const uint64_t v = a[i];
+        const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
+        const uint64_t lb = lower_bound(b, nb, v);
+        uint64_t cnt = 0;
+        if (lb < nb && b[lb] == v) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+        flag[i] = rank < cnt;
     }
 }
 
@@ -130,7 +127,7 @@ int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound,
     hipLaunchKernelGGL(k_isect_flags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, bound,
                        flag);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan(flag, off, bound, sc));
+    MS_TRY(launch_exclusive_scan_n(flag, off, d_na, 0, bound, sc));
     hipLaunchKernelGGL(k_compact_flagged, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, flag, off,
                        d_out, d_cnt);
     MS_HIP(hipGetLastError());
@@ -142,7 +139,8 @@ __global__ __launch_bounds__(WG) void k_merge_rflags(const uint64_t *a, const ui
                                                      const uint64_t *d_nb, uint64_t bound, uint64_t *flag)
 {
     const uint64_t na = *d_na, nb = *d_nb;
-    for (uint64_t j = (uint64_t)blockIdx.x * WG + threadIdx.x; j < bound; j += (uint64_t)gridDim.x * WG) {
+    const uint64_t lim = nb + 1 < bound ? nb + 1 : bound;   // flag[nb] = 0 closes the scan
+    for (uint64_t j = (uint64_t)blockIdx.x * WG + threadIdx.x; j < lim; j += (uint64_t)gridDim.x * WG) {
         uint64_t f = 0;
         if (j < nb) {
             const uint64_t v = b[j];
@@ -179,7 +177,7 @@ int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, con
     hipLaunchKernelGGL(k_merge_rflags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, bound,
                        flag);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan(flag, er, bound, sc));
+    MS_TRY(launch_exclusive_scan_n(flag, er, d_nb, 1, bound, sc));
     hipLaunchKernelGGL(k_merge_place, dim3(grid_for(na_bound + nb_bound + 1, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na,
                        b, d_nb, flag, er, d_out, d_cnt);
     MS_HIP(hipGetLastError());
