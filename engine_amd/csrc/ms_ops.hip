@@ -92,18 +92,81 @@ int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_n
 }
 
 // ------------------------------------------------------------------ intersect
-__global__ __launch_bounds__(WG) void k_isect_flags(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
-                                                    const uint64_t *d_nb, uint64_t bound, uint64_t *flag)
+// Intersection flags (multiset minimum, scalar/intersect_uncompr.h:69-80):
+// a[i] is kept when its rank among equal values of a is below the number
+// of equal values in b.
+// The same flags, a workgroup per chunk of ISECT_CHUNK values of a: the
+// part of b that the chunk's value range can meet is staged in LDS when
+// it fits, so the per-value searches in b run in LDS.
+constexpr uint32_t ISECT_CHUNK = 1024;
+constexpr uint32_t ISECT_LDS = 8192;   // b values staged per workgroup (64 KiB)
+
+template <class Arr>
+__device__ __forceinline__ uint64_t lb_in(const Arr &a, uint64_t n, uint64_t v)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <class Arr>
+__device__ __forceinline__ uint64_t ub_in(const Arr &a, uint64_t n, uint64_t v)
+{
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// b's range [lower_bound(first), upper_bound(last)) for every chunk of a,
+// one thread per chunk (all searches in flight at once).
+__global__ __launch_bounds__(WG) void k_isect_ranges(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                     const uint64_t *d_nb, uint64_t *rng)
 {
     const uint64_t na = *d_na, nb = *d_nb;
-    (void)bound;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < na; i += (uint64_t)gridDim.x * WG) {
-        const uint64_t v = a[i];
-        const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
-        const uint64_t lb = lower_bound(b, nb, v);
-        uint64_t cnt = 0;
-        if (lb < nb && b[lb] == v) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
-        flag[i] = rank < cnt;
+    const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
+    for (uint64_t ck = (uint64_t)blockIdx.x * WG + threadIdx.x; ck < nchunks; ck += (uint64_t)gridDim.x * WG) {
+        const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
+        rng[2 * ck] = lower_bound(b, nb, a[i0]);
+        rng[2 * ck + 1] = upper_bound(b, nb, a[i1 - 1]);
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_isect_flags_lds(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                        const uint64_t *d_nb, const uint64_t *rng, uint64_t *flag)
+{
+    __shared__ uint64_t sb[ISECT_LDS];
+    const uint64_t na = *d_na, nb = *d_nb;
+    const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
+    for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
+        const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
+        const uint64_t b0 = rng[2 * ck], len = rng[2 * ck + 1] - b0;
+        const bool staged = len <= ISECT_LDS;
+        if (staged)
+            for (uint64_t k = threadIdx.x; k < len; k += WG) sb[k] = b[b0 + k];
+        __syncthreads();
+        for (uint64_t i = i0 + threadIdx.x; i < i1; i += WG) {
+            const uint64_t v = a[i];
+            const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
+            uint64_t lb, cnt = 0;
+            bool hit;
+            if (staged) {
+                lb = lb_in(sb, len, v);
+                hit = lb < len && sb[lb] == v;
+                if (hit) cnt = rank == 0 ? 1 : ub_in(sb, len, v) - lb;
+            } else {
+                lb = lower_bound(b, nb, v);
+                hit = lb < nb && b[lb] == v;
+                if (hit) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+            }
+            flag[i] = rank < cnt;
+        }
+        __syncthreads();
     }
 }
 
@@ -124,8 +187,12 @@ int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound,
     uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
     uint64_t *off = (uint64_t *)sc.get(bound * 8 + 64);
     if (!flag || !off) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
-    hipLaunchKernelGGL(k_isect_flags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, bound,
-                       flag);
+    const uint64_t chunks = bound / ISECT_CHUNK + 1;
+    uint64_t *rng = (uint64_t *)sc.get(chunks * 16 + 64);
+    if (!rng) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
+    hipLaunchKernelGGL(k_isect_ranges, dim3(grid_for(chunks, WG, 2048)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, rng);
+    hipLaunchKernelGGL(k_isect_flags_lds, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na, b, d_nb,
+                       (const uint64_t *)rng, flag);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(flag, off, d_na, 0, bound, sc));
     hipLaunchKernelGGL(k_compact_flagged, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, flag, off,
