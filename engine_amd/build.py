@@ -19,9 +19,16 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-u
          "-munsafe-fp-atomics", "-I" + os.path.join(HERE, "..", "include")]
 
 
+# The decode kernels branch on per-block bit widths held in scalar
+# registers (wave-uniform): left unstructured, each such switch is a plain
+# binary tree of scalar branches instead of a chain of exec-mask flow blocks.
+UNIFORM_SWITCH = ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+EXTRA = {"ms_decode.hip": UNIFORM_SWITCH, "ms_batch.hip": UNIFORM_SWITCH}
+
+
 def _compile(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + EXTRA.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")

@@ -50,58 +50,6 @@ struct ms_bitem {
 };
 static_assert(sizeof(ms_bitem) == 96, "work item layout");
 
-// Per-unit predicate on codes for x = ref + code, bw <= 32, no wrap.
-__device__ __forceinline__ uint32_t unit_mode(const ms_pred &p, uint32_t bw, uint64_t ref, uint32_t &lo,
-                                              uint32_t &span)
-{
-    const ms_cpred cp = code_pred(p, ref, (1ull << bw) - 1ull);
-    lo = (uint32_t)cp.clo;
-    span = (uint32_t)cp.cspan;
-    return cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
-         : cp.clo == 0 ? SM_LE : SM_IN;
-}
-
-// Value-by-value select + sum of a thread's 8 codes (wide or wrapping):
-// rare, kept out of line so it does not set the kernel's register budget
-// (and declared pure, so the kernel's uniform loads stay scalar).
-__device__ __attribute__((noinline, pure)) uint64_t slow_codes(const ms_pred p, const uint64_t *base, uint32_t j,
-                                                               uint64_t g, uint32_t bw, uint64_t ref)
-{
-    uint64_t acc = 0;
-#pragma unroll 1
-    for (int m = 0; m < 8; m++) {
-        const uint64_t x = ref + vbp_read(base, 8, j, 8 * g + m, bw);
-        if (pred_eval(p, x)) acc += x;
-    }
-    return acc;
-}
-
-// One unit of an 8-lane bit-packed layout: `base` is the unit's block,
-// g the thread's code group inside it.  Codes that are the values take
-// their per-unit predicate from the entry's table; for_f codes (offsets
-// from `ref`) specialise it per unit.
-template <bool IS_FOR>
-__device__ __forceinline__ void vbp_unit(const ms_bentry &en, const ms_pred &p, const uint64_t *base, uint32_t j,
-                                         uint32_t g, uint32_t bw, uint64_t ref, uint64_t &acc)
-{
-    if (bw > 32 || (IS_FOR && ref + ((1ull << bw) - 1ull) < ref)) {
-        acc += slow_codes(p, base, j, g, bw, ref);
-        return;
-    }
-    uint32_t mode, lo, span;
-    if (!IS_FOR) {
-        mode = en.tab.mode[bw];
-        lo = en.tab.lo[bw];
-        span = en.tab.span[bw];
-    } else if (pred_le_form(p)) {
-        mode = unit_mode_le(p, ref, (1ull << bw) - 1ull, lo, span);
-    } else {
-        mode = unit_mode(p, bw, ref, lo, span);
-    }
-    if (mode == SM_NONE) return;
-    ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, lo, span, ref, acc);
-}
-
 // The item's n <= 64 units: lane k holds unit u0 + k's index entry (and
 // for_f reference), loaded together.
 template <bool IS_FOR>
@@ -116,12 +64,17 @@ __device__ __forceinline__ void item_blocks(const ms_bentry &en, const ms_bitem 
         my_e = gload(it.index + ((u0 + t) >> lg_spb));
         if (IS_FOR) my_r = gload(it.refs + (((u0 + t) >> lg_spb) >> lg_ipo));
     }
-    for (uint32_t k = 0; k < n; k++) {
-        const uint64_t e = readlane64(my_e, k);
-        const uint64_t ref = IS_FOR ? readlane64(my_r, k) : 0ull;
-        const uint64_t u = u0 + k;
-        vbp_unit<IS_FOR>(en, p, (const uint64_t *)(it.data + (e >> 8)), j, (uint32_t)(u & mask) * 8 + gl,
-                         (uint32_t)(e & 0xff), ref, acc);
+    const bool le = IS_FOR && pred_le_form(p);
+    for (uint32_t k = 0; k < n; k += 4) {
+        const uint32_t cnt = n - k < 4 ? n - k : 4u;
+        uint64_t e[4], r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e[u] = readlane64(my_e, k + u);
+            r[u] = IS_FOR ? readlane64(my_r, k + u) : 0ull;
+        }
+        if (le) vbp_units_ss<IS_FOR, true>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
+        else vbp_units_ss<IS_FOR, false>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
     }
 }
 
@@ -137,12 +90,14 @@ __device__ __forceinline__ void item_delta(const ms_bitem &it, const ms_pred &p,
     if (u0 + t < u_end) my_e = gload(it.index + ((u0 + t) >> lg_spb));
     for (uint32_t q = 0; q < n; q++) {
         const uint64_t tile = it.first + q;
+        const uint64_t ut = tile * 4;
+        if (ut >= u_end) break;
         uint64_t carry = gload(it.refs + tile * 8 + j);
-        for (uint64_t u = tile * 4; u < tile * 4 + 4 && u < u_end; u++) {
-            const uint64_t e = readlane64(my_e, (uint32_t)(u - u0));
-            delta_unit_ss<true>(p, (const uint64_t *)(it.data + (e >> 8)), (uint32_t)(e & 0xff), j,
-                                (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
-        }
+        const uint32_t cnt = u_end - ut < 4 ? (uint32_t)(u_end - ut) : 4u;
+        uint64_t e[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) e[u] = readlane64(my_e, 4 * q + u);
+        delta_units_ss<true>(it.data, e, cnt, ut, mask, p, t, j, gl, carry, acc);
     }
 }
 
@@ -194,14 +149,18 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
         const uint32_t kind = it.kind_n >> 28, n = it.kind_n & 0x0fffffffu;
         switch (kind) {
         case BI_STATIC: {
+            // unit u of a static_vbp column: 64 bw bytes at u * 64 bw,
+            // given to the indexed-layout routine as index entries
             const uint32_t bw = it.aux;
-            for (uint64_t u = it.first; u < it.first + n; u++) {
-                // the code group is loop invariant here; hiding that keeps
-                // the compiler from hoisting per-width offsets of all 32
-                // cases into registers
-                uint32_t g = gl;
-                asm volatile("" : "+v"(g));
-                vbp_unit<false>(en, p, (const uint64_t *)it.data + u * 8 * bw, j, g, bw, 0, acc);
+            for (uint32_t k = 0; k < n; k += 4) {
+                const uint32_t cnt = n - k < 4 ? n - k : 4u;
+                uint64_t e[4], r[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    e[u] = (((it.first + k + u) * 64ull * bw) << 8) | bw;
+                    r[u] = 0;
+                }
+                vbp_units_ss<false, false>(it.data, e, r, cnt, 0, 0, p, en.tab, j, gl, acc);
             }
             break;
         }

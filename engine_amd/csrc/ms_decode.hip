@@ -8,6 +8,9 @@
 // Mapping: a wave decodes a 512-value unit.  For a vertical-bit-packed
 // layout with S lanes, thread t handles lane j = t % S and the 8 codes
 // k = 8g .. 8g+7 with g = unit*(64/S) + t/S (values S*k + j).
+#include <map>
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 #include "ms_units.hpp"
@@ -41,59 +44,36 @@ __global__ __launch_bounds__(WG) void k_select_sum_blocks(ms_colv c, ms_pred p, 
     const uint32_t lg_spb = c.g.lg_spb;
     const uint32_t lg_ipo = c.g.lg_ipo;
     const uint64_t n = pt.n_compr >> 9;
+    const uint64_t nq = (n + 3) >> 2;   // groups of 4 units
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & 7, gl = t >> 3;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    const uint64_t mask = (1ull << lg_spb) - 1ull;
     uint64_t acc = 0;
-    uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
-    uint64_t e_next = u < n ? uniform64(c.index[u >> lg_spb]) : 0ull;
-    uint64_t r_next = IS_FOR && u < n ? uniform64(c.refs[(u >> lg_spb) >> lg_ipo]) : 0ull;
-    for (; u < n; u += W) {
-        const uint64_t e = e_next;
-        uint64_t ref = r_next;
-        if (u + W < n) {
-            e_next = uniform64(c.index[(u + W) >> lg_spb]);
-            if (IS_FOR) r_next = uniform64(c.refs[((u + W) >> lg_spb) >> lg_ipo]);
+    uint64_t q = (uint64_t)blockIdx.x * WAVES + wave_index();
+    // lanes 0..3 hold the group's index entries (and for_f references),
+    // requested one group ahead
+    uint64_t my_e = 0, my_r = 0;
+    if (q < nq && t < 4 && q * 4 + t < n) {
+        my_e = gload(c.index + ((q * 4 + t) >> lg_spb));
+        if (IS_FOR) my_r = gload(c.refs + (((q * 4 + t) >> lg_spb) >> lg_ipo));
+    }
+    for (; q < nq; q += W) {
+        const uint64_t u0 = q * 4;
+        const uint32_t cnt = n - u0 < 4 ? (uint32_t)(n - u0) : 4u;
+        uint64_t e[4], r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            e[u] = readlane64(my_e, u);
+            r[u] = IS_FOR ? readlane64(my_r, u) : 0ull;
         }
-        const uint32_t bw = (uint32_t)(e & 0xff);
-        const uint64_t *base = (const uint64_t *)(c.data + (e >> 8));
-        const uint32_t g = (uint32_t)(u & ((1ull << lg_spb) - 1)) * 8 + gl;
-        uint32_t mode, lo, span;
-        if (IS_FOR) {
-            const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
-            if (bw > 32 || ref + maxcode < ref) {
-                // wide or wrapping codes (rare): value by value on x itself
-#pragma unroll 1
-                for (int m = 0; m < 8; m++) {
-                    const uint64_t x = ref + vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
-                    if (pred_eval(p, x)) acc += x;
-                }
-                continue;
-            }
-            if (LEFORM) {
-                mode = unit_mode_le(p, ref, maxcode, lo, span);
-            } else {
-                const ms_cpred cp = code_pred(p, ref, maxcode);
-                mode = cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
-                     : cp.clo == 0 ? SM_LE : SM_IN;
-                lo = (uint32_t)cp.clo;
-                span = (uint32_t)cp.cspan;
-            }
-        } else {
-            if (bw > 32) {
-#pragma unroll 1
-                for (int m = 0; m < 8; m++) {
-                    const uint64_t x = vbp_read(base, 8, j, 8 * (uint64_t)g + m, bw);
-                    if (pred_eval(p, x)) acc += x;
-                }
-                continue;
-            }
-            mode = tab.mode[bw];
-            lo = tab.lo[bw];
-            span = tab.span[bw];
+        const uint64_t un = u0 + 4 * W + t;
+        my_e = my_r = 0;
+        if (q + W < nq && t < 4 && un < n) {
+            my_e = gload(c.index + (un >> lg_spb));
+            if (IS_FOR) my_r = gload(c.refs + ((un >> lg_spb) >> lg_ipo));
         }
-        if (mode == SM_NONE) continue;
-        ss_fast_dispatch<IS_FOR>(bw, base, j, g, mode, lo, span, ref, acc);
+        vbp_units_ss<IS_FOR, LEFORM>(c.data, e, r, cnt, u0, mask, p, tab, j, gl, acc);
     }
     wg_partial(acc, res);
 }
@@ -274,8 +254,15 @@ __global__ __launch_bounds__(WG) void k_delta_select_sum(ms_colv c, ms_pred p, u
     const uint32_t j = t >> 3, gl = t & 7;   // delta lane j in 8 consecutive threads
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
-    for (uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index(); tile < ntiles; tile += W)
-        delta_tile_ss<false>(c.data, c.index, c.refs, c.g.lg_spb, p, tile, nunits, t, j, gl, acc);
+    uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index();
+    // each tile's index entries are requested one tile ahead
+    uint64_t my_e = tile < ntiles ? delta_tile_entries(c.index, c.g.lg_spb, tile, nunits, t) : 0ull;
+    for (; tile < ntiles; tile += W) {
+        const uint64_t e_next =
+            tile + W < ntiles ? delta_tile_entries(c.index, c.g.lg_spb, tile + W, nunits, t) : 0ull;
+        delta_tile_ss<false>(c.data, my_e, c.refs, c.g.lg_spb, p, tile, nunits, t, j, gl, acc);
+        my_e = e_next;
+    }
     wg_partial(acc, part);
 }
 
@@ -551,6 +538,35 @@ int launch_delta_carries(const ms_col *in, scratch &sc)
 
 static uint32_t units_grid(uint64_t count_bound) { return grid_for(count_bound / 512 + 1, WAVES * 8ull, 2048); }
 
+// Workgroups of `kernel` that the device holds at once (occupancy per CU
+// times CUs), queried once per kernel: grid-stride kernels that prefetch
+// their next piece of work run best as one resident wave of workgroups.
+static uint32_t resident_grid(const void *kernel)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, uint32_t> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 2048;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(dev, kernel);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int cus = 0, blocks = 0;
+    uint32_t g = 2048;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, WG, 0) == hipSuccess && cus > 0 && blocks > 0)
+        g = (uint32_t)(cus * blocks);
+    cache[key] = g;
+    return g;
+}
+
+// Grid of a grid-stride kernel over `items` pieces of one wave each.
+static uint32_t stride_grid(const void *kernel, uint64_t items)
+{
+    const uint32_t r = resident_grid(kernel);
+    return grid_for(items, WAVES, r < 8192 ? r : 8192);
+}
+
 int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s)
 {
     (void)accumulate;
@@ -585,14 +601,13 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     case MS_DYNAMIC_VBP:
     case MS_FOR:
         if (fast_blocks(v)) {
-            grid = units_grid(nb);
             const ms_bwtab tab = make_bwtab(p);
-            if (v.g.kind == MS_FOR && pred_le_form(p))
-                hipLaunchKernelGGL((k_select_sum_blocks<true, true>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
-            else if (v.g.kind == MS_FOR)
-                hipLaunchKernelGGL((k_select_sum_blocks<true, false>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
-            else
-                hipLaunchKernelGGL((k_select_sum_blocks<false, false>), dim3(grid), dim3(WG), 0, s, v, p, tab, part);
+            const uint64_t quads = nb / 2048 + 1;   // groups of 4 units
+            auto k = v.g.kind == MS_FOR && pred_le_form(p) ? k_select_sum_blocks<true, true>
+                   : v.g.kind == MS_FOR                    ? k_select_sum_blocks<true, false>
+                                                           : k_select_sum_blocks<false, false>;
+            grid = stride_grid((const void *)k, quads);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(WG), 0, s, v, p, tab, part);
         } else {
             grid = grid_for(nb, WG * 16ull, 2048);
             hipLaunchKernelGGL(k_select_sum_generic, dim3(grid), dim3(WG), 0, s, v, p, part);
@@ -602,7 +617,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         const uint64_t *arr;
         MS_TRY(delta_src(in, v, sc, &arr));
         if (!arr && v.g.lg_spb != 0xff) {
-            grid = grid_for(nb / DELTA_TILE + 1, WAVES * 2ull, 2048);
+            grid = stride_grid((const void *)k_delta_select_sum, nb / DELTA_TILE + 1);
             hipLaunchKernelGGL(k_delta_select_sum, dim3(grid), dim3(WG), 0, s, v, p, part);
             break;
         }

@@ -201,21 +201,41 @@ __device__ __forceinline__ void dec_dispatch(uint32_t bw, const uint64_t *base, 
 // x in [lo, hi] / x outside [lo, hi].
 enum { SM_NONE = 0, SM_ALL = 1, SM_LE = 2, SM_IN = 3, SM_OUT = 4 };
 
-template <int BW, bool CNT>
-__device__ __forceinline__ void ss_fast(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t mode,
-                                        uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
+// Words of lane j holding its codes 8g .. 8g+7 (BW <= 32: at most 5
+// words).  The last word may lie past the thread's bits (and past the
+// block): column buffers are padded, and those bits are never used.
+template <int BW>
+__device__ __forceinline__ void unit_words(const uint64_t *__restrict__ base, uint32_t j, uint32_t g,
+                                           uint64_t (&w)[5])
 {
     constexpr int NW = vbp_traits<BW>::NW;
+    static_assert(BW <= 32 && NW <= 5, "at most 5 words for 32-bit codes");
     const uint32_t byte0 = g * BW;
     const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
-    const uint32_t sh = (byte0 & 7) * 8;
-    uint64_t w[NW];
-    // the last word may lie past this thread's bits (and past the block):
-    // column buffers are padded, and those bits are never used
 #pragma unroll
     for (int q = 0; q < NW; q++) w[q] = gload(p + q * 8);
+}
+
+__device__ __forceinline__ void unit_words_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
+                                                    uint64_t (&w)[5])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: unit_words<B>(base, j, g, w); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default: break;
+    }
+}
+
+// Select + sum of the thread's 8 codes cut from its words, under the
+// unit's predicate mode; CNT adds ref once per selected code (for_f).
+template <int BW, bool CNT>
+__device__ __forceinline__ void ss_cut_sum(const uint64_t (&w)[5], uint32_t g, uint32_t mode, uint32_t lo,
+                                           uint32_t span, uint64_t ref, uint64_t &acc)
+{
     uint32_t c[8];
-    vbp_cut32<BW>(w, sh, c);
+    vbp_cut32<BW>(w, ((g * BW) & 7) * 8, c);
     using A = typename std::conditional<(BW <= 29), uint32_t, uint64_t>::type;
     A s = 0;
     uint32_t cnt = 0;
@@ -289,17 +309,113 @@ __device__ __forceinline__ uint32_t unit_mode_le(const ms_pred &p, uint64_t ref,
 }
 
 template <bool CNT>
-__device__ __forceinline__ void ss_fast_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
-                                                 uint32_t mode, uint32_t lo, uint32_t span, uint64_t ref,
-                                                 uint64_t &acc)
+__device__ __forceinline__ void ss_cut_sum_dispatch(uint32_t bw, const uint64_t (&w)[5], uint32_t g, uint32_t mode,
+                                                    uint32_t lo, uint32_t span, uint64_t ref, uint64_t &acc)
 {
     switch (bw) {
 #define MS_CASE(B) \
-    case B: ss_fast<B, CNT>(base, j, g, mode, lo, span, ref, acc); break;
+    case B: ss_cut_sum<B, CNT>(w, g, mode, lo, span, ref, acc); break;
         MS_BW32_CASES(MS_CASE)
 #undef MS_CASE
     default: break;
     }
+}
+
+// Value-by-value select + sum of a thread's 8 codes (wide or wrapping):
+// rare, kept out of line so it does not set the kernels' register budget
+// (and declared pure, so the callers' uniform loads stay scalar).
+__device__ __attribute__((noinline, pure)) uint64_t slow_codes(const ms_pred p, const uint64_t *base, uint32_t j,
+                                                               uint64_t g, uint32_t bw, uint64_t ref)
+{
+    uint64_t acc = 0;
+#pragma unroll 1
+    for (int m = 0; m < 8; m++) {
+        const uint64_t x = ref + vbp_read(base, 8, j, 8 * g + m, bw);
+        if (pred_eval(p, x)) acc += x;
+    }
+    return acc;
+}
+
+enum { SM_SLOW = 5 };   // wide (> 32-bit) or wrapping codes: value by value
+
+// Up to four consecutive 512-value units u0 .. u0+cnt-1 of an 8-lane
+// indexed layout (dynamic_vbp_f blocks, or for_f's inner blocks): e[u] =
+// their block index entries, r[u] = their for_f reference values, mask =
+// units per block - 1; thread (gl, j) takes codes 8 gl' .. of lane j with
+// gl' = its unit's offset in the block * 8 + gl.  Every unit's predicate
+// mode is decided and its words requested before the first unit is
+// decoded, so a wave keeps up to four units' loads in flight; a unit no
+// value of which can pass is not read.
+//
+// vbp_unit_prep decides unit u's predicate mode and requests its words;
+// vbp_unit_finish decodes them.
+template <bool IS_FOR, bool LEFORM>
+__device__ __forceinline__ void vbp_unit_prep(const uint8_t *data, uint64_t e, uint64_t r, bool valid, uint64_t u,
+                                              uint64_t mask, const ms_pred &p, const ms_bwtab &tab, uint32_t j,
+                                              uint32_t gl, uint32_t &mode, uint32_t &lo, uint32_t &span,
+                                              uint64_t (&w)[5])
+{
+    mode = SM_NONE;
+    lo = span = 0;
+    if (!valid) return;
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    if (IS_FOR) {
+        const uint64_t maxcode = bw >= 64 ? ~0ull : (1ull << bw) - 1ull;
+        if (bw > 32 || r + maxcode < r) {
+            mode = SM_SLOW;
+        } else if (LEFORM) {
+            mode = unit_mode_le(p, r, maxcode, lo, span);
+        } else {
+            const ms_cpred cp = code_pred(p, r, maxcode);
+            mode = cp.mode == PM_NONE ? SM_NONE : cp.mode == PM_ALL ? SM_ALL : cp.mode == PM_OUT ? SM_OUT
+                 : cp.clo == 0 ? SM_LE : SM_IN;
+            lo = (uint32_t)cp.clo;
+            span = (uint32_t)cp.cspan;
+        }
+    } else if (bw > 32) {
+        mode = SM_SLOW;
+    } else {
+        mode = tab.mode[bw];
+        lo = tab.lo[bw];
+        span = tab.span[bw];
+    }
+    if (mode != SM_NONE && mode != SM_SLOW) {
+        uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+        asm volatile("" : "+v"(g));   // keeps per-width offsets of g from being hoisted
+        unit_words_dispatch(bw, (const uint64_t *)(data + (e >> 8)), j, g, w);
+    }
+}
+
+template <bool IS_FOR>
+__device__ __forceinline__ void vbp_unit_finish(const uint8_t *data, uint64_t e, uint64_t r, uint64_t u,
+                                                uint64_t mask, const ms_pred &p, uint32_t j, uint32_t gl,
+                                                uint32_t mode, uint32_t lo, uint32_t span, const uint64_t (&w)[5],
+                                                uint64_t &acc)
+{
+    if (mode == SM_NONE) return;
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));
+    if (mode == SM_SLOW) acc += slow_codes(p, (const uint64_t *)(data + (e >> 8)), j, g, bw, IS_FOR ? r : 0ull);
+    else ss_cut_sum_dispatch<IS_FOR>(bw, w, g, mode, lo, span, r, acc);
+}
+
+template <bool IS_FOR, bool LEFORM>
+__device__ __forceinline__ void vbp_units_ss(const uint8_t *data, const uint64_t (&e)[4], const uint64_t (&r)[4],
+                                             uint32_t cnt, uint64_t u0, uint64_t mask, const ms_pred &p,
+                                             const ms_bwtab &tab, uint32_t j, uint32_t gl, uint64_t &acc)
+{
+    // four named word arrays (not one indexed array): they stay in registers
+    uint32_t m0, m1, m2, m3, l0, l1, l2, l3, s0, s1, s2, s3;
+    uint64_t w0[5], w1[5], w2[5], w3[5];
+    vbp_unit_prep<IS_FOR, LEFORM>(data, e[0], r[0], cnt > 0, u0, mask, p, tab, j, gl, m0, l0, s0, w0);
+    vbp_unit_prep<IS_FOR, LEFORM>(data, e[1], r[1], cnt > 1, u0 + 1, mask, p, tab, j, gl, m1, l1, s1, w1);
+    vbp_unit_prep<IS_FOR, LEFORM>(data, e[2], r[2], cnt > 2, u0 + 2, mask, p, tab, j, gl, m2, l2, s2, w2);
+    vbp_unit_prep<IS_FOR, LEFORM>(data, e[3], r[3], cnt > 3, u0 + 3, mask, p, tab, j, gl, m3, l3, s3, w3);
+    vbp_unit_finish<IS_FOR>(data, e[0], r[0], u0, mask, p, j, gl, m0, l0, s0, w0, acc);
+    vbp_unit_finish<IS_FOR>(data, e[1], r[1], u0 + 1, mask, p, j, gl, m1, l1, s1, w1, acc);
+    vbp_unit_finish<IS_FOR>(data, e[2], r[2], u0 + 2, mask, p, j, gl, m2, l2, s2, w2, acc);
+    vbp_unit_finish<IS_FOR>(data, e[3], r[3], u0 + 3, mask, p, j, gl, m3, l3, s3, w3, acc);
 }
 
 // ------------------------------------------------------------------
@@ -350,28 +466,16 @@ __device__ __forceinline__ uint64_t group8_scan64(uint64_t x, uint32_t t)
     }
     return x;
 }
-template <int BW>
-__device__ __forceinline__ void dec32_unit(const uint64_t *__restrict__ base, uint32_t j, uint32_t g, uint32_t (&c)[8])
-{
-    constexpr int NW = vbp_traits<BW>::NW;
-    const uint32_t byte0 = g * BW;
-    const uint64_t *p = base + (uint64_t)(byte0 >> 3) * 8 + j;
-    uint64_t w[NW];
-#pragma unroll
-    for (int q = 0; q < NW; q++) w[q] = gload(p + q * 8);   // may over-read one word; buffers are padded
-    vbp_cut32<BW>(w, (byte0 & 7) * 8, c);
-}
-
 #define MS_BW26_CASES(X)                                                                                            \
     X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)     \
     X(21) X(22) X(23) X(24) X(25) X(26)
 
-__device__ __forceinline__ void dec32_dispatch(uint32_t bw, const uint64_t *base, uint32_t j, uint32_t g,
-                                               uint32_t (&c)[8])
+// Cuts the 8 codes (<= 26 bits) out of words loaded by unit_words_dispatch.
+__device__ __forceinline__ void dec32_cut_dispatch(uint32_t bw, const uint64_t (&w)[5], uint32_t g, uint32_t (&c)[8])
 {
     switch (bw) {
 #define MS_CASE(B) \
-    case B: dec32_unit<B>(base, j, g, c); break;
+    case B: vbp_cut32<B>(w, ((g * B) & 7) * 8, c); break;
         MS_BW26_CASES(MS_CASE)
 #undef MS_CASE
     default:
@@ -415,78 +519,137 @@ __device__ __attribute__((noinline, pure)) u64x2 delta_unit_wide_call(const ms_p
     return delta_unit_wide(p, base, bw, j, g, t, carry);
 }
 
-// One 512-value unit of an 8-lane delta_f column: `base` is the unit's
-// block, g = the thread's code group in it, t = lane id in the wave
-// (t = 8 * j + g % 8).  carry holds lane j's running sum before the unit
-// and is advanced past it.
-template <bool WIDE_OUT_OF_LINE>
-__device__ __forceinline__ void delta_unit_ss(const ms_pred &p, const uint64_t *base, uint32_t bw, uint32_t j,
-                                              uint32_t g, uint32_t t, uint64_t &carry, uint64_t &acc)
+// Select + sum of one unit from the thread's 8 deltas (<= 26 bits each);
+// carry = lane j's running sum before the unit, advanced past it.  Lane
+// j's 64 values in the unit are carry plus running sums of its deltas:
+// non-decreasing, inside [carry, carry + lane total] when that does not
+// wrap.  A lane whose whole range is inside or outside the predicate's
+// interval needs no scan: its 8 threads' values add up to
+//   sum over g of (8 carry + 8 (7 - g) tot_g + psum_g)
+// (tot_g = thread g's delta total, psum_g = the sum of its 8 running
+// sums), each thread adding its own term.  Only a wave holding a lane that
+// straddles a bound (rare) scans and tests per thread, then per value.
+__device__ __forceinline__ void delta_codes_ss(const ms_pred &p, uint32_t (&d)[8], uint32_t gl, uint32_t t,
+                                               uint64_t &carry, uint64_t &acc)
 {
-    if (bw <= 26) {
-        uint32_t d[8];
-        dec32_dispatch(bw, base, j, g, d);
+    uint32_t psum = d[0];
 #pragma unroll
-        for (int m = 1; m < 8; m++) d[m] += d[m - 1];
-        const uint32_t tot = d[7];
+    for (int m = 1; m < 8; m++) {
+        d[m] += d[m - 1];
+        psum += d[m];
+    }
+    const uint32_t tot = d[7];
+    const uint64_t c0 = carry, c1 = carry + group8_sum(tot);
+    const uint64_t span = p.hi - p.lo;
+    const bool in0 = !p.empty && c0 - p.lo <= span;
+    const bool in1 = !p.empty && c1 - p.lo <= span;
+    const bool whole = c1 >= c0 && in0 == in1 && (p.empty || !(c0 < p.lo && c1 > p.hi));
+    const bool sel = in0 != (bool)p.neg;
+    if (__builtin_amdgcn_ballot_w64(!whole) == 0) {
+        if (sel) acc += 8ull * c0 + (uint64_t)(8u * (7u - gl)) * tot + psum;
+    } else {
         const uint32_t incl = group8_scan(tot, t);
-        const uint64_t bt = carry + (incl - tot);   // x_m = bt + d[m], d[0] <= .. <= d[7] = tot
-        const uint64_t be = bt + tot;
-        // the thread's values lie in [bt, be]: wholly inside or wholly
-        // outside the predicate's interval is the common case
-        const bool in0 = !p.empty && bt - p.lo <= p.hi - p.lo;
-        const bool in1 = !p.empty && be - p.lo <= p.hi - p.lo;
-        const bool nowrap = be >= bt;
-        if (nowrap && in0 && in1) {
-            if (!p.neg) {
-                uint32_t s = 0;
-#pragma unroll
-                for (int m = 0; m < 8; m++) s += d[m];
-                acc += (uint64_t)s + 8ull * bt;
-            }
-        } else if (nowrap && !in0 && !in1 && !(bt < p.lo && be > p.hi)) {
-            if (p.neg) {
-                uint32_t s = 0;
-#pragma unroll
-                for (int m = 0; m < 8; m++) s += d[m];
-                acc += (uint64_t)s + 8ull * bt;
-            }
+        if (whole) {
+            if (sel) acc += 8ull * c0 + (uint64_t)(8u * (7u - gl)) * tot + psum;
         } else {
-            // a thread whose values straddle the predicate's bounds (rare)
+            // the thread's values lie in [bt, be]
+            const uint64_t bt = c0 + (incl - tot), be = bt + tot;
+            const bool i0 = !p.empty && bt - p.lo <= span;
+            const bool i1 = !p.empty && be - p.lo <= span;
+            if (be >= bt && i0 == i1 && (p.empty || !(bt < p.lo && be > p.hi))) {
+                if (i0 != (bool)p.neg) acc += 8ull * bt + psum;
+            } else {
 #pragma unroll 1
-            for (int m = 0; m < 8; m++) {
-                const uint64_t x = bt + d[m];
-                acc += pred_eval(p, x) ? x : 0ull;
+                for (int m = 0; m < 8; m++) {
+                    const uint64_t x = bt + d[m];
+                    acc += pred_eval(p, x) ? x : 0ull;
+                }
             }
         }
-        carry += group8_sum(tot);
-        return;
     }
-    const u64x2 r = WIDE_OUT_OF_LINE ? delta_unit_wide_call(p, base, bw, j, g, t, carry)
-                                     : delta_unit_wide(p, base, bw, j, g, t, carry);
-    acc += r.x;
-    carry = r.y;
+    carry = c1;
+}
+
+// Requests the words of delta unit u (entry e) and decodes them (below).
+__device__ __forceinline__ void delta_unit_prep(const uint8_t *data, uint64_t e, bool valid, uint64_t u,
+                                                uint64_t mask, uint32_t j, uint32_t gl, uint64_t (&w)[5])
+{
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    if (!valid || bw > 26) return;
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));   // keeps per-width offsets of g from being hoisted
+    unit_words_dispatch(bw, (const uint64_t *)(data + (e >> 8)), j, g, w);
+}
+
+template <bool WIDE_OUT_OF_LINE>
+__device__ __forceinline__ void delta_unit_finish(const uint8_t *data, uint64_t e, bool valid, uint64_t u,
+                                                  uint64_t mask, const ms_pred &p, uint32_t t, uint32_t j,
+                                                  uint32_t gl, const uint64_t (&w)[5], uint64_t &carry,
+                                                  uint64_t &acc)
+{
+    if (!valid) return;
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));
+    if (bw <= 26) {
+        uint32_t d[8];
+        dec32_cut_dispatch(bw, w, g, d);
+        delta_codes_ss(p, d, gl, t, carry, acc);
+    } else {
+        const uint64_t *base = (const uint64_t *)(data + (e >> 8));
+        const u64x2 r = WIDE_OUT_OF_LINE ? delta_unit_wide_call(p, base, bw, j, g, t, carry)
+                                         : delta_unit_wide(p, base, bw, j, g, t, carry);
+        acc += r.x;
+        carry = r.y;
+    }
+}
+
+// Units u0 .. u0+cnt-1 (cnt <= 4: one delta tile) of an 8-lane delta_f
+// column with 512-value-multiple inner blocks; e[u] = their block index
+// entries, mask = units per inner block - 1, carry = lane j's running sum
+// at u0.  The words of all units are requested before the first unit is
+// decoded, so a wave keeps four units' loads in flight.
+template <bool WIDE_OUT_OF_LINE>
+__device__ __forceinline__ void delta_units_ss(const uint8_t *data, const uint64_t (&e)[4], uint32_t cnt,
+                                               uint64_t u0, uint64_t mask, const ms_pred &p, uint32_t t,
+                                               uint32_t j, uint32_t gl, uint64_t &carry, uint64_t &acc)
+{
+    uint64_t w0[5], w1[5], w2[5], w3[5];   // named arrays: they stay in registers
+    delta_unit_prep(data, e[0], cnt > 0, u0, mask, j, gl, w0);
+    delta_unit_prep(data, e[1], cnt > 1, u0 + 1, mask, j, gl, w1);
+    delta_unit_prep(data, e[2], cnt > 2, u0 + 2, mask, j, gl, w2);
+    delta_unit_prep(data, e[3], cnt > 3, u0 + 3, mask, j, gl, w3);
+    delta_unit_finish<WIDE_OUT_OF_LINE>(data, e[0], cnt > 0, u0, mask, p, t, j, gl, w0, carry, acc);
+    delta_unit_finish<WIDE_OUT_OF_LINE>(data, e[1], cnt > 1, u0 + 1, mask, p, t, j, gl, w1, carry, acc);
+    delta_unit_finish<WIDE_OUT_OF_LINE>(data, e[2], cnt > 2, u0 + 2, mask, p, t, j, gl, w2, carry, acc);
+    delta_unit_finish<WIDE_OUT_OF_LINE>(data, e[3], cnt > 3, u0 + 3, mask, p, t, j, gl, w3, carry, acc);
+}
+
+// Block index entries of a delta tile's units, in lanes 0..3 (0 past
+// `nunits`).
+__device__ __forceinline__ uint64_t delta_tile_entries(const uint64_t *index, uint32_t lg_spb, uint64_t tile,
+                                                       uint64_t nunits, uint32_t t)
+{
+    const uint64_t u = tile * 4 + t;
+    return t < 4 && u < nunits ? gload(index + (u >> lg_spb)) : 0ull;
 }
 
 // A delta tile (MS_DELTA_TILE = 4 units) of a column with 512-value-
-// multiple inner blocks (data, block index, tile carries, log2 units per
-// block); units at or past `nunits` are skipped.
+// multiple inner blocks (data, tile carries, log2 units per block), its
+// units' index entries in lanes 0..3 of my_e; units at or past `nunits`
+// are skipped.
 template <bool WIDE_OUT_OF_LINE>
-__device__ __forceinline__ void delta_tile_ss(const uint8_t *data, const uint64_t *index, const uint64_t *refs,
+__device__ __forceinline__ void delta_tile_ss(const uint8_t *data, uint64_t my_e, const uint64_t *refs,
                                               uint32_t lg_spb, const ms_pred &p, uint64_t tile, uint64_t nunits,
                                               uint32_t t, uint32_t j, uint32_t gl, uint64_t &acc)
 {
     uint64_t carry = gload(refs + tile * 8 + j);
-    const uint64_t mask = (1ull << lg_spb) - 1ull;
-    const uint64_t u_end = nunits < tile * 4 + 4 ? nunits : tile * 4 + 4;
-    uint64_t e_next = tile * 4 < u_end ? uniform64(gload(index + ((tile * 4) >> lg_spb))) : 0ull;
-#pragma unroll 1
-    for (uint64_t u = tile * 4; u < u_end; u++) {
-        const uint64_t e = e_next;
-        if (u + 1 < u_end) e_next = uniform64(gload(index + ((u + 1) >> lg_spb)));
-        delta_unit_ss<WIDE_OUT_OF_LINE>(p, (const uint64_t *)(data + (e >> 8)), (uint32_t)(e & 0xff), j,
-                                        (uint32_t)(u & mask) * 8 + gl, t, carry, acc);
-    }
+    const uint64_t u0 = tile * 4;
+    const uint32_t cnt = nunits - u0 < 4 ? (uint32_t)(nunits - u0) : 4u;
+    uint64_t e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) e[u] = readlane64(my_e, u);
+    delta_units_ss<WIDE_OUT_OF_LINE>(data, e, cnt, u0, (1ull << lg_spb) - 1ull, p, t, j, gl, carry, acc);
 }
 
 }  // namespace ms
