@@ -167,14 +167,7 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
         case BI_BLOCKS: item_blocks<false>(en, it, p, n, t, j, gl, acc); break;
         case BI_FOR: item_blocks<true>(en, it, p, n, t, j, gl, acc); break;
         case BI_DELTA: item_delta(it, p, n, t, t >> 3, t & 7, acc); break;   // lane-major threads
-        case BI_RLE: {
-            const u64x2 *r = (const u64x2 *)it.data;
-            for (uint64_t k = it.first + t; k < it.first + n; k += 64) {
-                const u64x2 vl = gload(r + k);
-                if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
-            }
-            break;
-        }
+        case BI_RLE: acc += rle_runs_ss((const u64x2 *)it.data, it.first, it.first + n, p, t); break;
         case BI_RAW: {
             const uint64_t *x = (const uint64_t *)it.data;
             for (uint64_t k = it.first + t; k < it.first + n; k += 64) {

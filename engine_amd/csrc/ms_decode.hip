@@ -136,14 +136,19 @@ __global__ __launch_bounds__(WG) void k_select_sum_generic(ms_colv c, ms_pred p,
 }
 
 // rle_f: a run of `len` copies of v contributes len * v when v passes.
+static constexpr uint64_t RLE_CHUNK = 512;   // runs per wave and step
+
 __global__ __launch_bounds__(WG) void k_select_sum_rle(ms_colv c, ms_pred p, uint64_t *res)
 {
     const uint64_t runs = c.meta->size_used / 16;
-    const uint64_t *w = (const uint64_t *)c.data;
+    const u64x2 *w = (const u64x2 *)c.data;
+    const uint64_t nch = (runs + RLE_CHUNK - 1) / RLE_CHUNK;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    const uint32_t t = threadIdx.x & 63;
     uint64_t acc = 0;
-    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * WG) {
-        const u64x2 vl = *((const u64x2 *)w + r);
-        if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
+    for (uint64_t ch = (uint64_t)blockIdx.x * WAVES + wave_index(); ch < nch; ch += W) {
+        const uint64_t r0 = ch * RLE_CHUNK, r1 = r0 + RLE_CHUNK < runs ? r0 + RLE_CHUNK : runs;
+        acc += rle_runs_ss(w, r0, r1, p, t);
     }
     wg_partial(acc, res);
 }
@@ -585,7 +590,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         with_rest = 0;
         break;
     case MS_RLE:
-        grid = grid_for(nb / 64 + 1, WG * 4ull, 2048);
+        grid = stride_grid((const void *)k_select_sum_rle, in->size_alloc / 16 / RLE_CHUNK + 1);
         hipLaunchKernelGGL(k_select_sum_rle, dim3(grid), dim3(WG), 0, s, v, p, part);
         with_rest = 0;
         break;

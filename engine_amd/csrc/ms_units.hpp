@@ -419,6 +419,30 @@ __device__ __forceinline__ void vbp_units_ss(const uint8_t *data, const uint64_t
 }
 
 // ------------------------------------------------------------------
+// rle_f select + sum over runs [r0, r1) of the (value, length) pairs, by
+// one wave: lane t takes runs r0 + t + 64 i, four loads in flight.
+// ------------------------------------------------------------------
+__device__ __forceinline__ uint64_t rle_runs_ss(const u64x2 *runs, uint64_t r0, uint64_t r1, const ms_pred &p,
+                                                uint32_t t)
+{
+    uint64_t acc = 0;
+    uint64_t k = r0;
+    for (; k + 256 <= r1; k += 256) {
+        u64x2 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = gload(runs + k + 64 * q + t);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (pred_eval(p, v[q].x)) acc += v[q].x * v[q].y;
+    }
+    for (k += t; k < r1; k += 64) {
+        const u64x2 vl = gload(runs + k);
+        if (pred_eval(p, vl.x)) acc += vl.x * vl.y;
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------------
 // delta_f select + sum, one wave per unit of deltas with the lanes'
 // running sums ("carries") at the unit's start.  Thread t handles delta
 // lane j = t / 8 and code group gl = t % 8, so each lane's 64 codes sit in
@@ -541,10 +565,17 @@ __device__ __forceinline__ void delta_codes_ss(const ms_pred &p, uint32_t (&d)[8
     const uint32_t tot = d[7];
     const uint64_t c0 = carry, c1 = carry + group8_sum(tot);
     const uint64_t span = p.hi - p.lo;
-    const bool in0 = !p.empty && c0 - p.lo <= span;
-    const bool in1 = !p.empty && c1 - p.lo <= span;
-    const bool whole = c1 >= c0 && in0 == in1 && (p.empty || !(c0 < p.lo && c1 > p.hi));
-    const bool sel = in0 != (bool)p.neg;
+    bool whole, sel;
+    if (pred_le_form(p)) {   // x <= hi (uniform): two compares
+        const bool in0 = c0 <= p.hi, in1 = c1 <= p.hi;
+        whole = c1 >= c0 && in0 == in1;
+        sel = in0;
+    } else {
+        const bool in0 = !p.empty && c0 - p.lo <= span;
+        const bool in1 = !p.empty && c1 - p.lo <= span;
+        whole = c1 >= c0 && in0 == in1 && (p.empty || !(c0 < p.lo && c1 > p.hi));
+        sel = in0 != (bool)p.neg;
+    }
     if (__builtin_amdgcn_ballot_w64(!whole) == 0) {
         if (sel) acc += 8ull * c0 + (uint64_t)(8u * (7u - gl)) * tot + psum;
     } else {
