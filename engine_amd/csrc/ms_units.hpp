@@ -448,9 +448,9 @@ __device__ __forceinline__ uint64_t rle_runs_ss(const u64x2 *runs, uint64_t r0, 
 // lane j = t / 8 and code group gl = t % 8, so each lane's 64 codes sit in
 // 8 consecutive threads and its running sum is a scan within 8-thread
 // groups done with DPP row shifts (no LDS traffic).  Deltas of up to 26
-// bits are decoded and scanned in 32-bit arithmetic (64 codes of a lane
-// sum to less than 2^32); a thread's 8 values are then wholly inside or
-// outside the predicate's range except at its edges.
+// bits are decoded and summed in 32-bit arithmetic (64 codes of a lane
+// sum to less than 2^32); most lanes' values lie wholly inside or outside
+// the predicate's range, and those need no scan (delta_codes_ss).
 // ------------------------------------------------------------------
 
 // Value of the lane k (1, 2 or 4) places below within its 8-thread group,
