@@ -124,12 +124,13 @@ __device__ __forceinline__ uint64_t ub_in(const Arr &a, uint64_t n, uint64_t v)
 }
 
 // b's range [lower_bound(first), upper_bound(last)) for every chunk of a,
-// one thread per chunk (all searches in flight at once).
+// one thread per chunk (all searches in flight at once); d_nch = chunks.
 __global__ __launch_bounds__(WG) void k_isect_ranges(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
-                                                     const uint64_t *d_nb, uint64_t *rng)
+                                                     const uint64_t *d_nb, uint64_t *rng, uint64_t *d_nch)
 {
     const uint64_t na = *d_na, nb = *d_nb;
     const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_nch = nchunks;
     for (uint64_t ck = (uint64_t)blockIdx.x * WG + threadIdx.x; ck < nchunks; ck += (uint64_t)gridDim.x * WG) {
         const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
         rng[2 * ck] = lower_bound(b, nb, a[i0]);
@@ -137,10 +138,16 @@ __global__ __launch_bounds__(WG) void k_isect_ranges(const uint64_t *a, const ui
     }
 }
 
-__global__ __launch_bounds__(WG) void k_isect_flags_lds(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
-                                                        const uint64_t *d_nb, const uint64_t *rng, uint64_t *flag)
+// The flags of chunk ck as a bitmask: bit k of word w (w < ISECT_WORDS)
+// is a[i0 + 64 w + k]'s flag; cnt[ck] = the chunk's kept values.
+constexpr uint32_t ISECT_WORDS = ISECT_CHUNK / 64;
+
+__global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
+                                                   const uint64_t *d_nb, const uint64_t *rng, uint64_t *mask,
+                                                   uint64_t *cnt)
 {
     __shared__ uint64_t sb[ISECT_LDS];
+    __shared__ uint32_t s_cnt[ISECT_WORDS];
     const uint64_t na = *d_na, nb = *d_nb;
     const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
     for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
@@ -150,53 +157,95 @@ __global__ __launch_bounds__(WG) void k_isect_flags_lds(const uint64_t *a, const
         if (staged)
             for (uint64_t k = threadIdx.x; k < len; k += WG) sb[k] = b[b0 + k];
         __syncthreads();
-        for (uint64_t i = i0 + threadIdx.x; i < i1; i += WG) {
-            const uint64_t v = a[i];
-            const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
-            uint64_t lb, cnt = 0;
-            bool hit;
-            if (staged) {
-                lb = lb_in(sb, len, v);
-                hit = lb < len && sb[lb] == v;
-                if (hit) cnt = rank == 0 ? 1 : ub_in(sb, len, v) - lb;
-            } else {
-                lb = lower_bound(b, nb, v);
-                hit = lb < nb && b[lb] == v;
-                if (hit) cnt = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+#pragma unroll 1
+        for (uint32_t q = 0; q < ISECT_CHUNK / WG; q++) {
+            const uint32_t e = q * WG + threadIdx.x;   // index in the chunk
+            const uint64_t i = i0 + e;
+            bool keep = false;
+            if (i < i1) {
+                const uint64_t v = a[i];
+                const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
+                uint64_t lb, c = 0;
+                if (staged) {
+                    lb = lb_in(sb, len, v);
+                    if (lb < len && sb[lb] == v) c = rank == 0 ? 1 : ub_in(sb, len, v) - lb;
+                } else {
+                    lb = lower_bound(b, nb, v);
+                    if (lb < nb && b[lb] == v) c = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+                }
+                keep = rank < c;
             }
-            flag[i] = rank < cnt;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+            if ((threadIdx.x & 63) == 0) {
+                mask[ck * ISECT_WORDS + (e >> 6)] = m;
+                s_cnt[e >> 6] = (uint32_t)__builtin_popcountll(m);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t c = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < ISECT_WORDS; w++) c += s_cnt[w];
+            cnt[ck] = c;
         }
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(WG) void k_compact_flagged(const uint64_t *a, const uint64_t *d_na, const uint64_t *flag,
-                                                        const uint64_t *off, uint64_t *out, uint64_t *d_cnt)
+// Writes chunk ck's kept values from off[ck] on, in order; *d_cnt = all
+// kept values.
+__global__ __launch_bounds__(WG) void k_isect_emit(const uint64_t *a, const uint64_t *d_na, const uint64_t *mask,
+                                                   const uint64_t *cnt, const uint64_t *off, uint64_t *out,
+                                                   uint64_t *d_cnt)
 {
+    __shared__ uint32_t s_pre[ISECT_WORDS];
     const uint64_t na = *d_na;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < na; i += (uint64_t)gridDim.x * WG)
-        if (flag[i]) out[off[i]] = a[i];
-    if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = na ? off[na - 1] + flag[na - 1] : 0;
+    const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
+    const uint32_t lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = nchunks ? off[nchunks - 1] + cnt[nchunks - 1] : 0;
+    for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
+        const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
+        if (threadIdx.x == 0) {
+            uint32_t p = 0;
+#pragma unroll
+            for (uint32_t w = 0; w < ISECT_WORDS; w++) {
+                s_pre[w] = p;
+                p += (uint32_t)__builtin_popcountll(mask[ck * ISECT_WORDS + w]);
+            }
+        }
+        __syncthreads();
+        const uint64_t base = off[ck];
+#pragma unroll 1
+        for (uint32_t q = 0; q < ISECT_CHUNK / WG; q++) {
+            const uint32_t e = q * WG + threadIdx.x;
+            const uint64_t i = i0 + e;
+            const uint64_t m = mask[ck * ISECT_WORDS + (e >> 6)];
+            if (i < i1 && ((m >> lane) & 1))
+                out[base + s_pre[e >> 6] + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = a[i];
+        }
+        __syncthreads();
+    }
 }
 
 int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
                      const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc)
 {
     (void)nb_bound;
-    const uint64_t bound = na_bound + 1;
-    uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
-    uint64_t *off = (uint64_t *)sc.get(bound * 8 + 64);
-    if (!flag || !off) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
-    const uint64_t chunks = bound / ISECT_CHUNK + 1;
+    const uint64_t chunks = na_bound / ISECT_CHUNK + 1;
     uint64_t *rng = (uint64_t *)sc.get(chunks * 16 + 64);
-    if (!rng) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
-    hipLaunchKernelGGL(k_isect_ranges, dim3(grid_for(chunks, WG, 2048)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, rng);
-    hipLaunchKernelGGL(k_isect_flags_lds, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na, b, d_nb,
-                       (const uint64_t *)rng, flag);
+    uint64_t *mask = (uint64_t *)sc.get(chunks * ISECT_WORDS * 8 + 64);
+    uint64_t *cnt = (uint64_t *)sc.get(chunks * 8 + 64);
+    uint64_t *off = (uint64_t *)sc.get(chunks * 8 + 64);
+    uint64_t *d_nch = (uint64_t *)sc.get(64);
+    if (!rng || !mask || !cnt || !off || !d_nch) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
+    hipLaunchKernelGGL(k_isect_ranges, dim3(grid_for(chunks, WG, 2048)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, rng,
+                       d_nch);
+    hipLaunchKernelGGL(k_isect_mark, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na, b, d_nb,
+                       (const uint64_t *)rng, mask, cnt);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan_n(flag, off, d_na, 0, bound, sc));
-    hipLaunchKernelGGL(k_compact_flagged, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, sc.s, a, d_na, flag, off,
-                       d_out, d_cnt);
+    MS_TRY(launch_exclusive_scan_n(cnt, off, d_nch, 0, chunks, sc));
+    hipLaunchKernelGGL(k_isect_emit, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na,
+                       (const uint64_t *)mask, (const uint64_t *)cnt, (const uint64_t *)off, d_out, d_cnt);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
