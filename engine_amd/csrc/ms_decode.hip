@@ -742,7 +742,10 @@ __device__ __forceinline__ uint64_t unit_row(const uint64_t (&B)[8], uint32_t r)
     return F;
 }
 
-__global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, const uint64_t *d_units, uint64_t *cnt)
+// Decodes and tests every unit once: its 8 ballots go to flags[8 u ..]
+// (64 bytes per 512 values) and its selected count to cnt[u].
+__global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, const uint64_t *d_units, uint64_t *cnt,
+                                                         uint64_t *flags)
 {
     const ms_parts pt = parts_of(c);
     const uint64_t nunits = *d_units;
@@ -752,23 +755,31 @@ __global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, c
         uint64_t B[8];
         unit_flags(c, pt, p, u, t, B);
         uint32_t n = 0;
+        uint64_t mine = 0;
 #pragma unroll
-        for (int m = 0; m < 8; m++) n += __popcll(B[m]);
+        for (int m = 0; m < 8; m++) {
+            n += __popcll(B[m]);
+            if (t == (uint32_t)m) mine = B[m];
+        }
+        if (t < 8) flags[8 * u + t] = mine;
         if (t == 0) cnt[u] = n;
     }
 }
 
-__global__ __launch_bounds__(WG) void k_sel_compact_blocks(ms_colv c, ms_pred p, const uint64_t *d_units,
+// Writes the selected positions of every unit from its stored ballots
+// (no second decode).
+__global__ __launch_bounds__(WG) void k_sel_compact_blocks(const uint64_t *flags, const uint64_t *d_units,
                                                            const uint64_t *off, uint64_t *pos, uint64_t *d_cnt)
 {
-    const ms_parts pt = parts_of(c);
     const uint64_t nunits = *d_units;
     const uint32_t t = threadIdx.x & 63;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     const uint64_t below = t ? (~0ull >> (64 - t)) : 0ull;
     for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+        const uint64_t my = t < 8 ? gload(flags + 8 * u + t) : 0ull;
         uint64_t B[8];
-        unit_flags(c, pt, p, u, t, B);
+#pragma unroll
+        for (int m = 0; m < 8; m++) B[m] = readlane64(my, m);
         uint64_t o = off[u];
 #pragma unroll
         for (uint32_t r = 0; r < 8; r++) {
@@ -793,14 +804,15 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
         uint64_t *d_u = (uint64_t *)sc.get(64);
         uint64_t *cnt = (uint64_t *)sc.get(units * 8 + 64);
         uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
-        if (!d_u || !cnt || !off) return ms_fail(MS_ERR_NOMEM, "select scratch");
+        uint64_t *flags = (uint64_t *)sc.get(units * 64 + 64);
+        if (!d_u || !cnt || !off || !flags) return ms_fail(MS_ERR_NOMEM, "select scratch");
         hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
         const dim3 grid(units_grid(nb));
-        hipLaunchKernelGGL(k_sel_count_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt);
+        hipLaunchKernelGGL(k_sel_count_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt, flags);
         MS_HIP(hipGetLastError());
         MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 0, units, sc));
-        hipLaunchKernelGGL(k_sel_compact_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u,
-                           (const uint64_t *)off, d_pos, d_cnt);
+        hipLaunchKernelGGL(k_sel_compact_blocks, grid, dim3(WG), 0, s, (const uint64_t *)flags,
+                           (const uint64_t *)d_u, (const uint64_t *)off, d_pos, d_cnt);
         MS_HIP(hipGetLastError());
         return MS_OK;
     }
