@@ -99,18 +99,7 @@ int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_n
 // part of b that the chunk's value range can meet is staged in LDS when
 // it fits, so the per-value searches in b run in LDS.
 constexpr uint32_t ISECT_CHUNK = 1024;
-constexpr uint32_t ISECT_LDS = 8192;   // b values staged per workgroup (64 KiB)
-
-template <class Arr>
-__device__ __forceinline__ uint64_t lb_in(const Arr &a, uint64_t n, uint64_t v)
-{
-    uint64_t lo = 0, hi = n;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
+constexpr uint32_t ISECT_LDS = 4096;   // b values staged per workgroup (32 KiB: 5 workgroups per CU)
 
 template <class Arr>
 __device__ __forceinline__ uint64_t ub_in(const Arr &a, uint64_t n, uint64_t v)
@@ -156,22 +145,41 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
         const bool staged = len <= ISECT_LDS;
         if (staged)
             for (uint64_t k = threadIdx.x; k < len; k += WG) sb[k] = b[b0 + k];
+        constexpr uint32_t Q = ISECT_CHUNK / WG;   // values per thread
+        uint64_t v[Q], lb[Q];
+        bool in[Q];
+#pragma unroll
+        for (uint32_t q = 0; q < Q; q++) {
+            const uint64_t i = i0 + q * WG + threadIdx.x;
+            in[q] = i < i1;
+            v[q] = in[q] ? a[i] : 0ull;
+            lb[q] = 0;
+        }
         __syncthreads();
+        if (staged && len) {
+            // Q branch-free lower bounds in LDS side by side (same trip count)
+            for (uint64_t step = 1ull << (63 - __builtin_clzll(len)); step; step >>= 1) {
+#pragma unroll
+                for (uint32_t q = 0; q < Q; q++) {
+                    const uint64_t x = lb[q] + step;
+                    if (x <= len && sb[x - 1] < v[q]) lb[q] = x;
+                }
+            }
+        }
 #pragma unroll 1
-        for (uint32_t q = 0; q < ISECT_CHUNK / WG; q++) {
+        for (uint32_t q = 0; q < Q; q++) {
             const uint32_t e = q * WG + threadIdx.x;   // index in the chunk
             const uint64_t i = i0 + e;
             bool keep = false;
-            if (i < i1) {
-                const uint64_t v = a[i];
-                const uint64_t rank = (i == 0 || a[i - 1] != v) ? 0 : i - lower_bound(a, i, v);
-                uint64_t lb, c = 0;
+            if (in[q]) {
+                const uint64_t x = v[q];
+                const uint64_t rank = (i == 0 || a[i - 1] != x) ? 0 : i - lower_bound(a, i, x);
+                uint64_t c = 0;
                 if (staged) {
-                    lb = lb_in(sb, len, v);
-                    if (lb < len && sb[lb] == v) c = rank == 0 ? 1 : ub_in(sb, len, v) - lb;
+                    if (lb[q] < len && sb[lb[q]] == x) c = rank == 0 ? 1 : ub_in(sb, len, x) - lb[q];
                 } else {
-                    lb = lower_bound(b, nb, v);
-                    if (lb < nb && b[lb] == v) c = rank == 0 ? 1 : upper_bound(b, nb, v) - lb;
+                    const uint64_t l = lower_bound(b, nb, x);
+                    if (l < nb && b[l] == x) c = rank == 0 ? 1 : upper_bound(b, nb, x) - l;
                 }
                 keep = rank < c;
             }

@@ -185,6 +185,19 @@ def test_intersect_merge_calc(oracle, out_f):
     assert e.value.code == 4
 
 
+@pytest.mark.parametrize("na,nb,bits", [(3000, 200000, 20), (5000, 5000, 6), (70000, 3, 17), (1, 1024, 4)])
+def test_intersect_shapes(na, nb, bits):
+    """Chunks of the left list whose range of the right list is larger than
+    one workgroup stages (dense right side), runs of equal values across
+    chunk boundaries, a tiny right side, a single left value."""
+    a = np.sort(datagen.uniform_bits(10 + na, na, bits))
+    b = np.sort(datagen.uniform_bits(20 + nb, nb, bits))
+    got = ea.intersect_sorted(uncompr_f(), ea.upload_values(a), ea.upload_values(b)).values()
+    assert np.array_equal(got, rm.multiset_intersect(a, b))
+    got = ea.intersect_sorted(uncompr_f(), ea.upload_values(b), ea.upload_values(a)).values()
+    assert np.array_equal(got, rm.multiset_intersect(b, a))
+
+
 # ------------------------------------------------------------------ KATs
 def test_reference_kats_on_gpu():
     k = KAT["select_less"]
