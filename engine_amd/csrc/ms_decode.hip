@@ -732,15 +732,6 @@ __device__ __forceinline__ void unit_flags(const ms_colv &c, const ms_parts &pt,
     for (int m = 0; m < 8; m++) B[m] = __ballot(ok[m] && pred_eval(p, d[m]));
 }
 
-// Rows of 64 consecutive values of a unit from the per-m ballots: value
-// 64 r + 8 m + j is bit (8 r + j) of B[m]; row r's bit (8 m + j).
-__device__ __forceinline__ uint64_t unit_row(const uint64_t (&B)[8], uint32_t r)
-{
-    uint64_t F = 0;
-#pragma unroll
-    for (int m = 0; m < 8; m++) F |= ((B[m] >> (8 * r)) & 0xffull) << (8 * m);
-    return F;
-}
 
 // Decodes and tests every unit once: its 8 ballots go to flags[8 u ..]
 // (64 bytes per 512 values) and its selected count to cnt[u].
@@ -776,15 +767,17 @@ __global__ __launch_bounds__(WG) void k_sel_compact_blocks(const uint64_t *flags
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     const uint64_t below = t ? (~0ull >> (64 - t)) : 0ull;
     for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+        // value 64 r + t of the unit is bit 8 r + (t & 7) of ballot t / 8:
+        // lane t takes that ballot, and row r's flags are one ballot of its
+        // bit r (a transpose of the 8 x 8 byte matrix, one step per row)
         const uint64_t my = t < 8 ? gload(flags + 8 * u + t) : 0ull;
-        uint64_t B[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) B[m] = readlane64(my, m);
+        const uint64_t mine = __shfl(my, (int)(t >> 3), MS_WAVE);
         uint64_t o = off[u];
 #pragma unroll
         for (uint32_t r = 0; r < 8; r++) {
-            const uint64_t F = unit_row(B, r);
-            if ((F >> t) & 1) pos[o + __popcll(F & below)] = u * 512 + 64 * r + t;
+            const bool take = (mine >> (8 * r + (t & 7))) & 1;
+            const uint64_t F = __ballot(take);
+            if (take) pos[o + __popcll(F & below)] = u * 512 + 64 * r + t;
             o += __popcll(F);
         }
         if (u == nunits - 1 && t == 0) *d_cnt = o;
