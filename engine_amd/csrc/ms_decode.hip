@@ -700,17 +700,29 @@ __global__ void k_units_of(const ms_dev_meta *m, uint64_t *dst) { *dst = (m->cou
 // Flags of unit u's 512 values (bit t of the result's word m = value
 // 64 * (t >> 3) + 8 * m + (t & 7) of the unit, lane-major per thread):
 // decoded from the compressed part, or read raw from the rest.
+// Index entry and for_f reference of unit u (left in vector registers, so
+// that a caller can request them one unit ahead and wait only at use).
+__device__ __forceinline__ void unit_entry(const ms_colv &c, uint64_t ncu, uint64_t u, uint64_t &e, uint64_t &r)
+{
+    e = r = 0;
+    if (u < ncu) {
+        const uint64_t b = u >> c.g.lg_spb;
+        e = gload(c.index + b);
+        if (c.g.kind == MS_FOR) r = gload(c.refs + (b >> c.g.lg_ipo));
+    }
+}
+
 __device__ __forceinline__ void unit_flags(const ms_colv &c, const ms_parts &pt, const ms_pred &p, uint64_t u,
-                                           uint32_t t, uint64_t (&B)[8])
+                                           uint64_t e_v, uint64_t r_v, uint32_t t, uint64_t (&B)[8])
 {
     const uint32_t j = t & 7, gl = t >> 3;
     const uint64_t ncu = pt.n_compr >> 9;
     uint64_t d[8];
     bool ok[8];
     if (u < ncu) {
-        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
-        const uint64_t e = uniform64(gload(c.index + b));
-        const uint64_t ref = c.g.kind == MS_FOR ? uniform64(gload(c.refs + (b >> c.g.lg_ipo))) : 0ull;
+        const uint64_t sub = u & ((1ull << c.g.lg_spb) - 1);
+        const uint64_t e = uniform64(e_v);
+        const uint64_t ref = c.g.kind == MS_FOR ? uniform64(r_v) : 0ull;
         dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
 #pragma unroll
         for (int m = 0; m < 8; m++) {
@@ -740,11 +752,17 @@ __global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, c
 {
     const ms_parts pt = parts_of(c);
     const uint64_t nunits = *d_units;
+    const uint64_t ncu = pt.n_compr >> 9;
     const uint32_t t = threadIdx.x & 63;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
-    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+    uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
+    uint64_t e_n, r_n;   // the next unit's entry, requested one unit ahead
+    unit_entry(c, ncu, u, e_n, r_n);
+    for (; u < nunits; u += W) {
+        const uint64_t e_v = e_n, r_v = r_n;
+        unit_entry(c, ncu, u + W, e_n, r_n);
         uint64_t B[8];
-        unit_flags(c, pt, p, u, t, B);
+        unit_flags(c, pt, p, u, e_v, r_v, t, B);
         uint32_t n = 0;
         uint64_t mine = 0;
 #pragma unroll
