@@ -143,16 +143,30 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
         const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
         const uint64_t b0 = rng[2 * ck], len = rng[2 * ck + 1] - b0;
         const bool staged = len <= ISECT_LDS;
-        if (staged)
-            for (uint64_t k = threadIdx.x; k < len; k += WG) sb[k] = b[b0 + k];
+        if (staged) {
+            // all of a thread's loads in flight at once, then the LDS stores
+            constexpr uint32_t SQ = ISECT_LDS / WG;
+            uint64_t tmp[SQ];
+#pragma unroll
+            for (uint32_t q = 0; q < SQ; q++) {
+                const uint64_t k = q * WG + threadIdx.x;
+                tmp[q] = k < len ? b[b0 + k] : 0ull;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < SQ; q++) {
+                const uint64_t k = q * WG + threadIdx.x;
+                if (k < len) sb[k] = tmp[q];
+            }
+        }
         constexpr uint32_t Q = ISECT_CHUNK / WG;   // values per thread
-        uint64_t v[Q], lb[Q];
+        uint64_t v[Q], prv[Q], lb[Q];
         bool in[Q];
 #pragma unroll
         for (uint32_t q = 0; q < Q; q++) {
             const uint64_t i = i0 + q * WG + threadIdx.x;
             in[q] = i < i1;
             v[q] = in[q] ? a[i] : 0ull;
+            prv[q] = in[q] && i > 0 ? a[i - 1] : 0ull;
             lb[q] = 0;
         }
         __syncthreads();
@@ -173,7 +187,7 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
             bool keep = false;
             if (in[q]) {
                 const uint64_t x = v[q];
-                const uint64_t rank = (i == 0 || a[i - 1] != x) ? 0 : i - lower_bound(a, i, x);
+                const uint64_t rank = (i == 0 || prv[q] != x) ? 0 : i - lower_bound(a, i, x);
                 uint64_t c = 0;
                 if (staged) {
                     if (lb[q] < len && sb[lb[q]] == x) c = rank == 0 ? 1 : ub_in(sb, len, x) - lb[q];
