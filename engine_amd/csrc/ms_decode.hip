@@ -165,8 +165,9 @@ static constexpr uint64_t DELTA_TILE = MS_DELTA_TILE;
 static_assert(MS_DELTA_TILE == WAVES * 512, "a delta tile is one workgroup's 4 units");
 
 template <int SRC>
+// e_v: unit u's index entry (SRC 0), loaded by the caller.
 __device__ __forceinline__ void delta_load(const ms_colv &c, const uint64_t *arr, uint64_t u, uint64_t nunits,
-                                           uint32_t S, uint32_t j, uint32_t gl, uint64_t (&d)[8])
+                                           uint32_t S, uint32_t j, uint32_t gl, uint64_t e_v, uint64_t (&d)[8])
 {
     if (u >= nunits) {
 #pragma unroll
@@ -174,8 +175,8 @@ __device__ __forceinline__ void delta_load(const ms_colv &c, const uint64_t *arr
         return;
     }
     if (SRC == 0) {
-        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
-        const uint64_t e = uniform64(c.index[b]);
+        const uint64_t sub = u & ((1ull << c.g.lg_spb) - 1);
+        const uint64_t e = uniform64(e_v);
         dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
     } else {
         const uint64_t *a = arr + u * 512;
@@ -225,12 +226,19 @@ __global__ __launch_bounds__(WG) void k_delta_tiles(ms_colv c, const uint64_t *a
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & (S - 1), gl = t >> c.g.lg_dstep;
     uint64_t acc = 0;
+    // SRC 0: each unit's index entry is requested one tile ahead
+    uint64_t e_n = 0;
+    if (SRC == 0 && blockIdx.x * WAVES + wv < nunits) e_n = gload(c.index + ((blockIdx.x * WAVES + wv) >> c.g.lg_spb));
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint64_t u = tile * WAVES + wv;
+        const uint64_t carry = gload(c.refs + tile * 8 + j);   // needed after the scan: requested first
+        const uint64_t e_v = e_n;
+        const uint64_t un = (tile + gridDim.x) * WAVES + wv;
+        if (SRC == 0) e_n = un < nunits ? gload(c.index + (un >> c.g.lg_spb)) : 0ull;
         uint64_t d[8];
-        delta_load<SRC>(c, arr, u, nunits, S, j, gl, d);
+        delta_load<SRC>(c, arr, u, nunits, S, j, gl, e_v, d);
         uint64_t tile_tot;
-        const uint64_t base = c.refs[tile * 8 + j] + delta_tile_scan(d, S, j, t, wv, s_wtot, tile_tot);
+        const uint64_t base = carry + delta_tile_scan(d, S, j, t, wv, s_wtot, tile_tot);
         if (u < nunits) {
             if (MODE == 0) {
                 uint64_t *o = out + u * 512;
@@ -283,8 +291,10 @@ __global__ __launch_bounds__(WG) void k_delta_tile_sums(ms_colv c, const uint64_
     const uint32_t t = threadIdx.x & 63;
     const uint32_t j = t & (S - 1), gl = t >> c.g.lg_dstep;
     for (uint64_t tile = blockIdx.x; tile < ntb; tile += gridDim.x) {
+        const uint64_t u = tile * WAVES + wv;
+        const uint64_t e_v = SRC == 0 && u < nunits ? gload(c.index + (u >> c.g.lg_spb)) : 0ull;
         uint64_t d[8];
-        delta_load<SRC>(c, arr, tile * WAVES + wv, nunits, S, j, gl, d);
+        delta_load<SRC>(c, arr, u, nunits, S, j, gl, e_v, d);
         uint64_t tile_tot;
         (void)delta_tile_scan(d, S, j, t, wv, s_wtot, tile_tot);
         if (threadIdx.x < S) sums[(uint64_t)threadIdx.x * ntb + tile] = tile_tot;
