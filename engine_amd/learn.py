@@ -16,7 +16,7 @@ engine
      time of the 5 candidates);
   4. selects per chunk: the smallest predicted candidate, or one predicted
      within `size_tol` of it and faster by more than `time_margin`; exact
-     sizes judge the pick and replace it by the exact smallest when it is
+     sizes check the pick and replace it by the exact smallest when it is
      more than `exact_tol` larger.
 Whatever is picked, the data stays bit-exact: only the format changes.
 """
