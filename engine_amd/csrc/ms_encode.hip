@@ -159,6 +159,11 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
     extern __shared__ unsigned long long lds[];
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
+    // Lane counts that divide the wave and blocks of whole waves of codes
+    // let each thread own a run of consecutive codes of one lane.
+    const uint32_t S = (uint32_t)l.S;
+    const bool runs = S != 0 && S <= 64 && 64 % S == 0 && l.B % 64 == 0;
+    const uint32_t j = runs ? t % S : 0, g = runs ? t / S : 0, C = (uint32_t)(l.B >> 6);
     for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
         const uint32_t bw = bws[b];
         const uint64_t h = hdr_of(l, b);
@@ -169,14 +174,66 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
         for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
         __syncthreads();
         // OR every code in, unmasked (vbp_routine_gen.py:101, 111)
-        for (uint64_t i = t; i < l.B; i += 64) {
-            const uint64_t c = xform(l, v, b * l.B + i, ref);
-            const uint64_t j = i % l.S, k = i / l.S;
-            const uint64_t bit = k * bw;
-            const uint64_t w = bit >> 6;
-            const uint32_t sh = (uint32_t)(bit & 63);
-            atomicOr(&lds[w * l.S + j], (unsigned long long)(c << sh));
-            if (sh + bw > 64) atomicOr(&lds[(w + 1) * l.S + j], (unsigned long long)(c >> (64 - sh)));
+        if (runs) {
+            // Thread (j, g) owns codes g*C .. g*C+C-1 of lane j: consecutive
+            // codes share words, so it ORs them in registers and issues one
+            // LDS atomic per word it touches instead of one per code.
+            const uint64_t base = b * l.B + j;
+            uint32_t k = g * C, bit = k * bw, cw = bit >> 6;
+            uint64_t acc = 0;
+            // A delta step of one lane width makes each code's predecessor
+            // the thread's previous code: one extra load per run.
+            const bool own_pred = l.kind == MS_DELTA && l.dstep == S;
+            uint64_t prev = 0;
+            if (own_pred) {
+                const uint64_t g0 = base + (uint64_t)k * S;
+                prev = g0 >= S ? v[g0 - S] : 0ull;
+            }
+            for (uint32_t c0 = 0; c0 < C; c0 += 8) {
+                uint64_t x[8];
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    const uint64_t gi = base + (uint64_t)(k + q) * S;
+                    x[q] = c0 + q < C ? (own_pred ? v[gi] : xform(l, v, gi, ref)) : 0ull;
+                }
+                if (own_pred) {
+#pragma unroll
+                    for (uint32_t q = 0; q < 8; ++q) {
+                        const uint64_t r = x[q];
+                        x[q] = r - prev;
+                        prev = r;
+                    }
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    if (c0 + q >= C) break;
+                    const uint32_t w = bit >> 6, sh = bit & 63;
+                    if (w != cw) {
+                        atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+                        acc = 0;
+                        cw = w;
+                    }
+                    acc |= x[q] << sh;
+                    if (sh + bw > 64) {
+                        atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+                        cw = w + 1;
+                        acc = x[q] >> (64 - sh);
+                    }
+                    bit += bw;
+                }
+                k += 8;
+            }
+            atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+        } else {
+            for (uint64_t i = t; i < l.B; i += 64) {
+                const uint64_t c = xform(l, v, b * l.B + i, ref);
+                const uint64_t j = i % l.S, k = i / l.S;
+                const uint64_t bit = k * bw;
+                const uint64_t w = bit >> 6;
+                const uint32_t sh = (uint32_t)(bit & 63);
+                atomicOr(&lds[w * l.S + j], (unsigned long long)(c << sh));
+                if (sh + bw > 64) atomicOr(&lds[(w + 1) * l.S + j], (unsigned long long)(c >> (64 - sh)));
+            }
         }
         __syncthreads();
         uint64_t *dst = (uint64_t *)(out + off);
