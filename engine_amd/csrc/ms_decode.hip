@@ -794,13 +794,23 @@ __global__ __launch_bounds__(WG) void k_sel_compact_blocks(const uint64_t *flags
     const uint32_t t = threadIdx.x & 63;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     const uint64_t below = t ? (~0ull >> (64 - t)) : 0ull;
-    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += W) {
+    uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
+    // One load per lane requests a unit: lanes 0..7 its ballots, lane 8 its
+    // output offset (the other lanes re-read ballot 0).  The next unit is
+    // requested one unit ahead; past the end the current unit again.
+    const auto request = [&](uint64_t uu) {
+        return gload(t < 8 ? flags + 8 * uu + t : (t == 8 ? off + uu : flags + 8 * uu));
+    };
+    uint64_t my_n = u < nunits ? request(u) : 0ull;
+    for (; u < nunits; u += W) {
+        const uint64_t my = my_n;
+        my_n = request(u + W < nunits ? u + W : u);
         // value 64 r + t of the unit is bit 8 r + (t & 7) of ballot t / 8:
         // lane t takes that ballot, and row r's flags are one ballot of its
         // bit r (a transpose of the 8 x 8 byte matrix, one step per row)
-        const uint64_t my = t < 8 ? gload(flags + 8 * u + t) : 0ull;
         const uint64_t mine = __shfl(my, (int)(t >> 3), MS_WAVE);
-        uint64_t o = off[u];
+        uint64_t o = ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(my >> 32), 8) << 32) |
+                     __builtin_amdgcn_readlane((uint32_t)my, 8);
 #pragma unroll
         for (uint32_t r = 0; r < 8; r++) {
             const bool take = (mine >> (8 * r + (t & 7))) & 1;
