@@ -274,8 +274,10 @@ class Plan:
             def __enter__(self):
                 check(lib().ms_plan_begin(plan.h, stream))
 
-            def __exit__(self, *exc):
-                check(lib().ms_plan_end(plan.h, stream))
+            def __exit__(self, exc_type, *exc):
+                rc = lib().ms_plan_end(plan.h, stream)
+                if exc_type is None:        # an operator's own error takes precedence
+                    check(rc)
                 return False
         return _Rec()
 
@@ -351,8 +353,9 @@ class Batch:
         return len(self.cols)
 
     def free(self):
+        """Releases the batch (refused while a live plan has recorded it)."""
         if self.h:
-            lib().ms_batch_free(self.h)
+            check(lib().ms_batch_free(self.h))
             self.h = None
 
     def __del__(self):

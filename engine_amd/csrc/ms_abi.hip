@@ -42,6 +42,7 @@ struct ms_plan {
     hipGraphExec_t exec;
     bool recording;
     std::vector<void *> deferred;   // frees of eager memory requested while capturing
+    std::vector<std::pair<void (*)(void *), void *>> holds;   // objects the graph reads
 };
 
 static thread_local ms_plan *g_rec = nullptr;
@@ -49,6 +50,15 @@ static thread_local ms_plan *g_rec = nullptr;
 static uint64_t round256(uint64_t x) { return (x + 255) / 256 * 256; }
 
 namespace ms {
+
+bool plan_capturing() { return g_rec && g_rec->recording && g_rec->mode == 1; }
+
+int plan_hold(void (*release)(void *), void *obj)
+{
+    if (!plan_capturing()) return ms_fail(MS_ERR_INVALID, "no plan is capturing");
+    g_rec->holds.emplace_back(release, obj);
+    return MS_OK;
+}
 
 void *dev_alloc(size_t bytes, hipStream_t s)
 {
@@ -264,12 +274,26 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     return MS_OK;
 }
 
-static uint64_t out_capacity(const ms_format_desc &f, uint64_t count_bound)
+// Bytes a write-iterator with m_CountBuffer = K (0: a column-level morph,
+// one morph_batch call) writes for at most m values.  Every K-value call
+// starts fresh dynamic_vbp meta pages and for_f reference pages
+// (write_iterator.h:78-185), so the calls are bounded one by one: at most
+// m / K full calls and one done() call of fewer than K values, each at most
+// F::get_size_max_byte(K) (monotone in its count); then the rest gap, its
+// 64-byte alignment and a raw rest of fewer than m_BlockSize values.
+static uint64_t wit_size_max(const ms_format_desc &f, uint64_t m, uint64_t K)
 {
-    // pessimistic size, plus the rest gap (for short outputs the rest sits
-    // behind a gap larger than the pessimistic size), plus room for raw
-    // appends behind the rest
-    return size_max_any_len(f, count_bound) + ms_make_geom(f).gap + 8 * 64 + 256;
+    if (f.kind == MS_UNCOMPR) return 8 * m;
+    if (f.kind == MS_RLE) return 16 * m;
+    const ms_geom g = ms_make_geom(f);
+    if (K == 0) return size_max_any_len(f, m) + g.gap;
+    return (m / K + 1) * size_max(f, K) + g.gap + 64 + 8 * g.bs;
+}
+
+static uint64_t out_capacity(const ms_format_desc &f, uint64_t count_bound, uint64_t K)
+{
+    // plus room for the raw appends of select/project behind the rest
+    return wit_size_max(f, count_bound, K) + 8 * 64 + 256;
 }
 
 static hipStream_t S(void *stream) { return (hipStream_t)stream; }
@@ -572,6 +596,7 @@ extern "C" int ms_plan_free(ms_plan *p)
     if (g_rec == p) g_rec = nullptr;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (auto &h : p->holds) h.first(h.second);
     if (p->arena) {
         (void)hipDeviceSynchronize();
         (void)hipFree(p->arena);
@@ -640,7 +665,7 @@ static int emit_new(const ms_format_desc &f, const uint64_t *vals, const uint64_
                     uint64_t chunk, uint64_t extra_values, scratch &sc, ms_col **out, uint64_t **d_append)
 {
     ms_col *c;
-    MS_TRY(new_col(f, m_bound + extra_values, out_capacity(f, m_bound + extra_values), sc.s, &c));
+    MS_TRY(new_col(f, m_bound + extra_values, out_capacity(f, m_bound + extra_values, chunk), sc.s, &c));
     *d_append = &c->meta->aux[0];
     int rc;
     if (f.kind == MS_UNCOMPR) {
@@ -688,11 +713,15 @@ __global__ void k_split_project(const ms_dev_meta *pos, ms_geom g, uint32_t lane
     *d_flag = r != 0;
 }
 
-__global__ void k_append_raw(ms_dev_meta *m, uint8_t *data, const uint64_t *v, const uint64_t *d_na,
+__global__ void k_append_raw(ms_dev_meta *m, uint8_t *data, uint64_t cap, const uint64_t *v, const uint64_t *d_na,
                              const uint64_t *d_total, const uint64_t *d_flag)
 {
     const uint64_t na = *d_na, total = *d_total;
     const uint64_t app = m->aux[0];
+    if (*d_flag && app + 8 * (total - na) > cap) {
+        if (threadIdx.x == 0) m->status = MS_ERR_CAPACITY;
+        return;
+    }
     uint64_t *dst = (uint64_t *)(data + app);
     for (uint64_t i = na + threadIdx.x; i < total; i += blockDim.x) dst[i - na] = v[i];
     if (threadIdx.x == 0 && *d_flag) {
@@ -704,7 +733,8 @@ __global__ void k_append_raw(ms_dev_meta *m, uint8_t *data, const uint64_t *v, c
 static int finish_split(ms_col *c, const uint64_t *v, const uint64_t *d_na, const uint64_t *d_total,
                         const uint64_t *d_flag, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_append_raw, dim3(1), dim3(64), 0, s, c->meta, c->data, v, d_na, d_total, d_flag);
+    hipLaunchKernelGGL(k_append_raw, dim3(1), dim3(64), 0, s, c->meta, c->data, c->size_alloc, v, d_na, d_total,
+                       d_flag);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
@@ -747,7 +777,7 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
         *out = c;
         return MS_OK;
     }
-    MS_TRY(new_col(*dst, nb, out_capacity(*dst, nb), S(stream), &c));
+    MS_TRY(new_col(*dst, nb, out_capacity(*dst, nb, 0), S(stream), &c));
     int rc = launch_emit(c, vals, d_n, nb, 0, nullptr, sc);
     if (rc) { ms_col_free(c); return rc; }
     *out = c;

@@ -259,7 +259,19 @@ struct ms_batch {
     uint32_t finish_threads = WG;   // 1024 when an entry has thousands of partials
     std::vector<hipEvent_t> ev;     // kernel timing: a ring of (start, end) pairs
     uint32_t ev_next = 0, ev_used = 0;
+    uint32_t holds = 0;             // live plans whose graph reads dE, dI and dP
 };
+
+static void batch_release(void *p) { ((ms_batch *)p)->holds--; }
+
+// A batch recorded into a plan keeps its queue until every such plan is
+// freed: rebuilding or freeing it would leave the graph reading stale or
+// released memory.
+static int check_unheld(const ms_batch *b)
+{
+    if (b->holds) return ms_fail(MS_ERR_INVALID, "the batch is recorded in %u live plan(s); free them first", b->holds);
+    return MS_OK;
+}
 
 static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0};
 
@@ -297,6 +309,7 @@ static bool delta_batchable(const ms_geom &g)
 extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uint64_t val, uint64_t *entry)
 {
     if (!b || !in) return ms_fail(MS_ERR_INVALID, "null argument");
+    MS_TRY(check_unheld(b));
     if (cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "comparison %u", cmp);
     if (in->g.kind == MS_DELTA && !delta_batchable(in->g))
         return ms_fail(MS_ERR_INVALID, "batched delta_f needs step 8 over an 8-lane dynamic_vbp inner format");
@@ -429,7 +442,7 @@ static int launch(ms_batch *b, uint64_t *d_results, hipStream_t s)
     if (nE == 0) return MS_OK;
     auto kern = b->cfg.waves_per_simd == 8 ? k_batch_select_sum_w8
               : b->cfg.waves_per_simd == 6 ? k_batch_select_sum_w6 : k_batch_select_sum;
-    const uint32_t slots = (uint32_t)b->ev.size() / 2;
+    const uint32_t slots = plan_capturing() ? 0 : (uint32_t)b->ev.size() / 2;
     const uint32_t k = slots ? b->ev_next % slots : 0;
     if (slots) MS_HIP(hipEventRecord(b->ev[2 * k], s));
     hipLaunchKernelGGL(kern, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE, (const ms_bitem *)b->dI,
@@ -450,6 +463,14 @@ extern "C" int ms_batch_run(ms_batch *b, uint64_t *d_results, void *stream)
 {
     if (!b || (!d_results && !b->E.empty())) return ms_fail(MS_ERR_INVALID, "null argument");
     hipStream_t s = (hipStream_t)stream;
+    if (plan_capturing()) {
+        // the graph keeps reading this queue: it must exist before the
+        // capture (no allocation or synchronisation inside it)
+        if (!b->built) return ms_fail(MS_ERR_INVALID, "a batch must run once before it is recorded into a plan");
+        MS_TRY(plan_hold(batch_release, b));
+        b->holds++;
+        return launch(b, d_results, s);
+    }
     if (!b->built) MS_TRY(build(b, s));
     return launch(b, d_results, s);
 }
@@ -476,6 +497,8 @@ static int time_cfg(ms_batch *b, const ms_batch_config &c, uint64_t *d_res, uint
 extern "C" int ms_batch_tune(ms_batch *b, uint64_t *d_results, uint32_t reps, ms_batch_config *best, void *stream)
 {
     if (!b || !d_results) return ms_fail(MS_ERR_INVALID, "null argument");
+    MS_TRY(check_unheld(b));
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "ms_batch_tune cannot be recorded into a plan");
     hipStream_t s = (hipStream_t)stream;
     if (reps == 0) reps = 5;
     ms_batch_config cur = b->cfg;
@@ -526,6 +549,7 @@ extern "C" int ms_batch_get_config(const ms_batch *b, ms_batch_config *cfg)
 extern "C" int ms_batch_set_config(ms_batch *b, const ms_batch_config *cfg)
 {
     if (!b || !cfg) return ms_fail(MS_ERR_INVALID, "null argument");
+    MS_TRY(check_unheld(b));
     if (!cfg_ok(*cfg)) return ms_fail(MS_ERR_INVALID, "batch configuration out of range");
     b->cfg = *cfg;
     b->built = false;
@@ -551,6 +575,7 @@ static void drop_events(ms_batch *b)
 extern "C" int ms_batch_timing(ms_batch *b, uint32_t slots)
 {
     if (!b) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "ms_batch_timing cannot be recorded into a plan");
     if (slots > 4096) return ms_fail(MS_ERR_INVALID, "at most 4096 timing slots");
     MS_HIP(hipDeviceSynchronize());
     drop_events(b);
@@ -582,6 +607,7 @@ extern "C" int ms_batch_kernel_times(ms_batch *b, float *ms, uint32_t *n)
 extern "C" int ms_batch_free(ms_batch *b)
 {
     if (!b) return MS_OK;
+    MS_TRY(check_unheld(b));
     (void)hipDeviceSynchronize();
     drop_events(b);
     if (b->dE) (void)hipFree(b->dE);

@@ -11,7 +11,6 @@
 #include <map>
 #include <mutex>
 
-#include <hipcub/hipcub.hpp>
 
 #include "ms_units.hpp"
 
@@ -878,12 +877,14 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
 // Exclusive scan of the first min(*d_n + extra, bound) values, sized from
 // the device count: G workgroups each take one contiguous chunk (sum,
 // scan of the G sums, chunk scan with its offset).  Arrays allocated for
-// a host bound are scanned only as far as they are used.
+// a host bound are scanned only as far as they are used; without a device
+// count (d_n null) all `bound` values are scanned.  Counts are 64-bit
+// throughout.
 // ------------------------------------------------------------------
 __device__ __forceinline__ void chunk_of(const uint64_t *d_n, uint64_t extra, uint64_t bound, uint64_t &lo,
                                          uint64_t &hi)
 {
-    uint64_t n = *d_n + extra;
+    uint64_t n = d_n ? *d_n + extra : bound;
     if (n > bound) n = bound;
     const uint64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     lo = (uint64_t)blockIdx.x * chunk;
@@ -986,13 +987,7 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
 
 int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc)
 {
-    if (n == 0) return MS_OK;
-    size_t bytes = 0;
-    MS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d_in, d_out, (int)n, sc.s));
-    void *tmp = sc.get(bytes + 64);
-    if (!tmp) return ms_fail(MS_ERR_NOMEM, "scan scratch");
-    MS_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, d_in, d_out, (int)n, sc.s));
-    return MS_OK;
+    return launch_exclusive_scan_n(d_in, d_out, nullptr, 0, n, sc);
 }
 
 }  // namespace ms

@@ -40,6 +40,8 @@ struct emit_layout {
     uint64_t K;          // values per morph_batch call (0: one call)
     uint64_t bs;         // outer m_BlockSize
     uint64_t gap;        // rest gap
+    uint64_t cap;        // bytes of the output image
+    uint64_t *status;    // the output's status word (MS_ERR_CAPACITY past cap)
 };
 
 struct emit_plan {
@@ -63,6 +65,13 @@ __device__ __forceinline__ emit_plan plan_of(const emit_layout &l, uint64_t m)
     p.nb = mc / l.B;
     p.nfb = mc / l.L;
     return p;
+}
+
+// Records that a write would pass the end of the output image; the write
+// is dropped (the host sizes outputs so that this does not happen).
+__device__ __forceinline__ void emit_overflow(const emit_layout &l)
+{
+    atomicCAS((unsigned long long *)l.status, 0ull, (unsigned long long)MS_ERR_CAPACITY);
 }
 
 // Bytes of headers in front of innermost block b.
@@ -171,6 +180,10 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
         const uint64_t fb = b / l.bpo;
         const uint64_t ref = l.kind == MS_FOR ? refs[fb] : 0;
         const uint64_t words = l.B * bw / 64;
+        if (off + 8 * words > l.cap) {            // uniform per wave
+            if (t == 0) emit_overflow(l);
+            continue;
+        }
         for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
         __syncthreads();
         // OR every code in, unmasked (vbp_routine_gen.py:101, 111)
@@ -289,6 +302,10 @@ __global__ __launch_bounds__(WG) void k_pack_static(emit_layout l, const uint64_
     const emit_plan pl = plan_of(l, *d_m);
     const uint64_t bw = l.bw, S = l.S;
     const uint64_t nwords = pl.m_c * bw / 64;
+    if (nwords * 8 > l.cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) emit_overflow(l);
+        return;
+    }
     uint64_t *o = (uint64_t *)out;
     for (uint64_t W = (uint64_t)blockIdx.x * WG + threadIdx.x; W < nwords; W += (uint64_t)gridDim.x * WG) {
         const uint64_t j = W % S, w = W / S;
@@ -326,6 +343,7 @@ __global__ void k_emit_finish(emit_layout l, const uint64_t *d_m, const uint64_t
     else C = start[pl.nb - 1] + hdr_of(l, pl.nb - 1) + l.B * bws[pl.nb - 1] / 8;
     const uint64_t R = ms_align64(C + l.gap);
     const uint64_t rest = pl.m - pl.m_c;
+    if (R + 8 * rest > l.cap) emit_overflow(l);
     meta->count = pl.m;
     meta->size_compr = C;
     meta->size_used = rest ? R + 8 * rest : C;
@@ -339,6 +357,7 @@ __global__ __launch_bounds__(WG) void k_emit_rest(emit_layout l, const uint64_t 
 {
     const emit_plan pl = plan_of(l, *d_m);
     const uint64_t rest = pl.m - pl.m_c;
+    if (meta->aux[1] + 8 * rest > l.cap) return;   // reported by k_emit_finish
     uint64_t *dst = (uint64_t *)(out + meta->aux[1]);
     for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < rest; i += (uint64_t)gridDim.x * WG)
         dst[i] = v[pl.m_c + i];
@@ -361,6 +380,8 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
     l.bs = g.bs;
     l.gap = g.gap;
     l.K = chunk;
+    l.cap = out->size_alloc;
+    l.status = &out->meta->status;
     if (f.kind == MS_DELTA || f.kind == MS_FOR) {
         l.L = g.outer_block;
         l.FP = g.outer_page;

@@ -24,6 +24,13 @@ struct scratch {
 
 int colv_of(const ms_col *c, ms_colv &v);
 
+// Plans: true while the calling thread captures operators into a hipGraph.
+bool plan_capturing();
+// While capturing: the plan calls release(obj) when it is freed, so an
+// object whose device memory the captured graph reads can refuse to change
+// or free that memory until then.
+int plan_hold(void (*release)(void *), void *obj);
+
 // device memory of the operators (from a recording plan's arena if any)
 void *dev_alloc(size_t bytes, hipStream_t s);
 void dev_free(void *p, hipStream_t s);

@@ -299,7 +299,11 @@ int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uint64_t val, 
  * walks the device work queue, a second one folds the per-item partial
  * sums.  The first run after an enqueue builds the queue (it reads the
  * columns' meta data and synchronises `stream` once); later runs only
- * launch the two kernels and may be recorded into a plan. */
+ * launch the two kernels and may be recorded into a plan.  A batch must
+ * have run once before a plan records it, and while a plan that recorded
+ * it is alive its queue is pinned: ms_batch_enqueue, ms_batch_set_config,
+ * ms_batch_tune and ms_batch_free return MS_ERR_INVALID until every such
+ * plan is freed. */
 int ms_batch_run(ms_batch *b, uint64_t *d_results, void *stream);
 
 /* Autotuner: times `reps` runs per candidate shape (coordinate descent over

@@ -592,11 +592,15 @@ int oc_wit_emit(oc_col *out, const ms_format_desc *f, const uint64_t *v, uint64_
         *append_off = 8 * m;
         return MS_OK;
     }
-    oc_size_max_byte_any_len(f, m, &alloc);
     oc_block_size(f, &bs);
-    oc_rest_offset(f, 0, &r);                  /* the rest may sit behind a gap larger than alloc */
-    if (col_alloc(out, f, alloc + r + extra_capacity + 2 * 64)) return MS_ERR_NOMEM;
     K = round_up(bs, 2048);                                      /* m_CountBuffer */
+    /* every K-value morph_batch call starts fresh pages, so the calls are
+     * bounded one by one (at most m / K full calls and one shorter one,
+     * get_size_max_byte being monotone in its count), then the raw rest */
+    oc_size_max_byte(f, K, &alloc);
+    alloc = (m / K + 1) * alloc + 8 * bs;
+    oc_rest_offset(f, 0, &r);                  /* the rest sits behind the gap */
+    if (col_alloc(out, f, alloc + r + extra_capacity + 2 * 64)) return MS_ERR_NOMEM;
     full = m / K;
     for (uint64_t c = 0; c < full; c++) {
         if (f->kind == MS_DELTA) pos += delta_compress(f, v + c * K, K, out->data + pos, last);
@@ -625,6 +629,7 @@ int oc_wit_emit(oc_col *out, const ms_format_desc *f, const uint64_t *v, uint64_
         out->size_used = pos;
         oc_rest_offset(f, pos, append_off);
     }
+    if (out->size_used > out->size_alloc) return MS_ERR_CAPACITY;
     return MS_OK;
 }
 

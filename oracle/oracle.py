@@ -96,6 +96,10 @@ class Column:
         return self.c.size_compr
 
     @property
+    def size_alloc(self):
+        return self.c.size_alloc
+
+    @property
     def fmt(self):
         return self.c.f
 

@@ -141,3 +141,45 @@ def test_batch_in_a_plan_replays():
         assert res.get().tolist() == expect, rep
         assert other.get().tolist() == expect, rep
     plan.free()
+
+
+def test_batch_recorded_in_a_plan_is_pinned():
+    """A captured plan reads the batch's device queue: while the plan is
+    alive, rebuilding (enqueue, set_config, tune) or freeing the batch is
+    refused; after the plan is freed it works again.  A batch that never
+    ran cannot be recorded (its queue would be built inside the capture)."""
+    from engine_amd import hip
+    x = datagen.uniform_bits(77, 5000, 20)
+    col = ea.morph(dynamic_vbp_f(512, 2048, 8), ea.upload_values(x))
+    st = hip.Stream()
+    res = ea.DeviceU64(1)
+    fresh = ea.Batch()
+    fresh.enqueue(col, MS_CMP_LT, 1 << 19)
+    p0 = ea.Plan(1 << 20)
+    with pytest.raises(ea.MsError) as e:
+        with p0.record(st.handle):
+            fresh.run(res, st.handle)
+    assert e.value.code == 1
+    p0.free()
+    b = ea.Batch()
+    b.enqueue(col, MS_CMP_LT, 1 << 19)
+    b.run(res, st.handle)
+    st.synchronize()
+    plan = ea.Plan(1 << 20)
+    with plan.record(st.handle):
+        b.run(res, st.handle)
+    for call in (lambda: b.enqueue(col, MS_CMP_GE, 0), lambda: b.tune(res, 1, st.handle),
+                 lambda: setattr(b, "config", BatchConfig(4, 16, 2, 4096, 8192, 0)), b.free):
+        with pytest.raises(ea.MsError) as e:
+            call()
+        assert e.value.code == 1
+    res.zero()
+    plan.launch(st.handle)
+    st.synchronize()
+    assert res.get().tolist() == [rm.masked_sum(x, MS_CMP_LT, 1 << 19)]
+    plan.free()
+    b.enqueue(col, MS_CMP_GE, 0)
+    res2 = ea.DeviceU64(2)
+    b.run(res2, st.handle)
+    st.synchronize()
+    assert res2.get().tolist() == [rm.masked_sum(x, MS_CMP_LT, 1 << 19), rm.masked_sum(x, MS_CMP_GE, 0)]

@@ -153,6 +153,25 @@ def test_project_byte_identical(oracle, data_f, pos_f, out_f):
     g.check()
 
 
+@pytest.mark.parametrize("out_f", [for_f(512, 1024, dynamic_vbp_f(512, 64, 8)), for_f(512, 8, dynamic_vbp_f(512, 64, 8))],
+                         ids=repr)
+def test_project_output_fits_per_flush_pages(oracle, out_f):
+    """A for_f output whose reference page is larger than a 2048-value
+    flush: every flush starts a fresh page, so the output needs far more
+    than a column-level size bound.  It must fit its allocation, report no
+    capacity error and equal the oracle byte for byte."""
+    n = 1 << 19
+    v = datagen.uniform_bits(19, n, 64)
+    v[::512] |= np.uint64(1 << 63)
+    p = np.arange(n, dtype=np.uint64)
+    g = ea.project(out_f, ea.upload_values(v), ea.upload_values(p), 8)
+    g.check()
+    m = g.meta()
+    assert m.size_used <= m.size_alloc
+    assert_same_column(g, oracle.project(out_f, oracle.from_values(v), oracle.from_values(p), 8))
+    assert np.array_equal(g.values(), v)
+
+
 def test_project_out_of_range_reports():
     d = ea.upload_values(np.arange(10, dtype=np.uint64))
     p = ea.upload_values(np.array([1, 2, 10], np.uint64))
@@ -183,6 +202,34 @@ def test_intersect_merge_calc(oracle, out_f):
     with pytest.raises(ea.MsError) as e:
         ea.calc_binary(MS_OP_ADD, out_f, ga, gb)
     assert e.value.code == 4
+
+
+@pytest.mark.parametrize("fb", [default_delta_dynamic_vbp_f(8), dynamic_vbp_f(512, 2048, 8)], ids=repr)
+def test_intersect_merge_both_compressed(oracle, fb):
+    """Both operands compressed position lists (the shape of a two-column
+    plan): select outputs in delta+dynamic_vbp intersected with a second
+    compressed list, written compressed; byte images equal the oracle's."""
+    n = 300000
+    x = datagen.uniform_bits(31, n, 20)
+    y = datagen.uniform_bits(32, n, 3)
+    fa = default_delta_dynamic_vbp_f(8)
+    gx, gy = ea.upload_values(x), ea.upload_values(y)
+    ox, oy = oracle.from_values(x), oracle.from_values(y)
+    pa = ea.select(MS_CMP_LT, 1 << 18, fa, gx, 8)
+    pb = ea.select(MS_CMP_EQ, 3, fb, gy, 8)
+    oa = oracle.select(MS_CMP_LT, 1 << 18, fa, ox, 8)
+    ob = oracle.select(MS_CMP_EQ, 3, fb, oy, 8)
+    assert_same_column(pa, oa)
+    assert_same_column(pb, ob)
+    a = np.flatnonzero(x < (1 << 18)).astype(np.uint64)
+    b = np.flatnonzero(y == 3).astype(np.uint64)
+    for out_f in (fa, uncompr_f()):
+        gi = ea.intersect_sorted(out_f, pa, pb)
+        assert_same_column(gi, oracle.intersect_sorted(out_f, oa, ob))
+        assert np.array_equal(gi.values(), rm.multiset_intersect(a, b))
+        gm = ea.merge_sorted(out_f, pb, pa)
+        assert_same_column(gm, oracle.merge_sorted(out_f, ob, oa))
+        assert np.array_equal(gm.values(), rm.multiset_union(b, a))
 
 
 @pytest.mark.parametrize("na,nb,bits", [(3000, 200000, 20), (5000, 5000, 6), (70000, 3, 17), (1, 1024, 4)])

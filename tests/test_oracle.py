@@ -226,6 +226,24 @@ def test_write_iterator_chunking(oracle, f, m):
     assert np.array_equal(col.values(), v)
 
 
+@pytest.mark.parametrize("f,page_bytes", [(for_f(512, 1024, dynamic_vbp_f(512, 64, 8)), 8 * 1024),
+                                          (for_f(512, 8, dynamic_vbp_f(512, 64, 8)), 8 * 8)], ids=repr)
+def test_write_iterator_capacity_per_flush(oracle, f, page_bytes):
+    """Every 2048-value flush starts a fresh for_f reference page, so the
+    output of m full-width values takes m / 2048 pages: far more than one
+    page per 1024 * 512 values that a column-level size bound counts.  The
+    image must fit its allocation and have exactly that closed-form size."""
+    m = 1 << 19
+    v = rng_vals(19, m, 64)
+    v[::512] |= np.uint64(1 << 63)                  # every FOR block spans the full 64 bits
+    col, app = oracle.wit_emit(f, v)
+    flushes = m // 2048
+    per_flush = page_bytes + 4 * (64 + 512 * 64 // 8)   # 4 FOR blocks: inner page meta + bw-64 payload
+    assert col.size_compr == col.size_used == flushes * per_flush
+    assert col.size_used <= col.size_alloc
+    assert np.array_equal(col.values(), v)
+
+
 # ------------------------------------------------------------ operators
 CMPS = [MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE]
 
