@@ -11,6 +11,7 @@ import os
 import numpy as np
 
 from .formats import BatchConfig, ColMeta, FormatDesc  # noqa: F401
+from .hip import Stream
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # MS_LIB_PATH selects another build of the same library (e.g. a tuning variant)
@@ -112,10 +113,15 @@ def init(device=0):
 
 
 class Column:
-    """A device column (ms_col*).  Freed when garbage collected."""
+    """A device column (ms_col*).  Freed when garbage collected.
 
-    def __init__(self, handle):
+    `stream` is the stream the column was created on: ms_col_free releases
+    the column in order on it, so a Stream object passed here is kept alive
+    as long as the column."""
+
+    def __init__(self, handle, stream=None):
         self.h = handle
+        self._stream = stream if isinstance(stream, Stream) else None
 
     def __del__(self):
         if self.h and _lib is not None:
@@ -156,7 +162,7 @@ class Column:
 def _new(fn, *args):
     h = VP()
     check(fn(ctypes.byref(h), *args))
-    return Column(h)
+    return Column(h, next((a for a in args if isinstance(a, Stream)), None))
 
 
 def upload_values(values, stream=None):

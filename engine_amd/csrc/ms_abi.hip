@@ -87,7 +87,7 @@ void dev_free(void *p, hipStream_t s)
     (void)hipFreeAsync(p, s);
 }
 
-void *scratch::get(size_t bytes)
+void *temps::get(size_t bytes)
 {
     if (n >= 64) return nullptr;
     void *p = dev_alloc(bytes, s);
@@ -96,7 +96,7 @@ void *scratch::get(size_t bytes)
     return p;
 }
 
-scratch::~scratch()
+temps::~temps()
 {
     for (int i = 0; i < n; i++) dev_free(ptrs[i], s);
 }
@@ -412,7 +412,7 @@ extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *
     if (!refs.empty()) MS_HIP(hipMemcpyAsync(c->refs, refs.data(), refs.size() * 8, hipMemcpyHostToDevice, S(stream)));
     MS_HIP(hipMemcpyAsync(c->meta, &dm, sizeof dm, hipMemcpyHostToDevice, S(stream)));
     if (f->kind == MS_DELTA) {
-        scratch sc(S(stream));
+        temps sc(S(stream));
         int rc = launch_delta_carries(c, sc);
         if (rc) { ms_col_free(c); return rc; }
     }
@@ -608,7 +608,7 @@ extern "C" int ms_plan_free(ms_plan *p)
 // ------------------------------------------------------------------ helpers
 // Uncompressed values of `in` (its own image for uncompr_f, else a
 // decompressed temporary) and a device pointer to their count.
-static int values_of(const ms_col *in, scratch &sc, const uint64_t **vals, const uint64_t **d_n)
+static int values_of(const ms_col *in, temps &sc, const uint64_t **vals, const uint64_t **d_n)
 {
     *d_n = &in->meta->count;
     if (in->f.kind == MS_UNCOMPR) {
@@ -662,7 +662,7 @@ static uint32_t grid_of(uint64_t n)
 // Emits values[0 .. *d_m) of a write-iterator into a new column of format f
 // (not rle_f); uncompr_f is a plain copy.
 static int emit_new(const ms_format_desc &f, const uint64_t *vals, const uint64_t *d_m, uint64_t m_bound,
-                    uint64_t chunk, uint64_t extra_values, scratch &sc, ms_col **out, uint64_t **d_append)
+                    uint64_t chunk, uint64_t extra_values, temps &sc, ms_col **out, uint64_t **d_append)
 {
     ms_col *c;
     MS_TRY(new_col(f, m_bound + extra_values, out_capacity(f, m_bound + extra_values, chunk), sc.s, &c));
@@ -752,7 +752,7 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
     MS_TRY(ms_need_init());
     if (!out || !in) return ms_fail(MS_ERR_INVALID, "null argument");
     if (!valid_format(dst)) return ms_fail(MS_ERR_INVALID, "invalid destination format");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t nb = in->count_bound;
     if (memcmp(dst, &in->f, sizeof *dst) == 0 || (dst->kind == MS_UNCOMPR && in->f.kind == MS_UNCOMPR)) {
         // same format: a copy of the image, meta and index
@@ -791,7 +791,7 @@ extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_form
     MS_TRY(ms_need_init());
     if (!out || !in || cmp > MS_CMP_GE || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t nb = in->count_bound;
     const ms_pred p = make_pred(cmp, val);
     uint64_t *pos = (uint64_t *)sc.get(nb * 8 + 64);
@@ -825,7 +825,7 @@ extern "C" int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_co
     MS_TRY(ms_need_init());
     if (!out || !data || !pos || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t np = pos->count_bound;
     const uint64_t *pv, *d_np;
     MS_TRY(values_of(pos, sc, &pv, &d_np));
@@ -897,7 +897,7 @@ extern "C" int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f, co
     MS_TRY(ms_need_init());
     if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t *a, *b, *d_na, *d_nb;
     MS_TRY(values_of(l, sc, &a, &d_na));
     MS_TRY(values_of(r, sc, &b, &d_nb));
@@ -916,7 +916,7 @@ extern "C" int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f, const 
     MS_TRY(ms_need_init());
     if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t *a, *b, *d_na, *d_nb;
     MS_TRY(values_of(l, sc, &a, &d_na));
     MS_TRY(values_of(r, sc, &b, &d_nb));
@@ -940,7 +940,7 @@ extern "C" int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *o
     MS_TRY(ms_col_get_meta(r, &mr, stream));
     if (ml.count != mr.count)
         return ms_fail(MS_ERR_SIZE, "calc: inDataLCol and inDataRCol must contain the same number of data elements");
-    scratch sc(S(stream));
+    temps sc(S(stream));
     const uint64_t *a, *b, *d_na, *d_nb;
     MS_TRY(values_of(l, sc, &a, &d_na));
     MS_TRY(values_of(r, sc, &b, &d_nb));

@@ -516,7 +516,7 @@ static bool delta_shiftable(const ms_colv &v) { return v.g.lg_dstep != 0xff && v
 
 // Deltas of a delta_f column whose inner layout has no fast decoder,
 // decoded per value into a temporary.
-static int delta_src(const ms_col *in, const ms_colv &v, scratch &sc, const uint64_t **arr)
+static int delta_src(const ms_col *in, const ms_colv &v, temps &sc, const uint64_t **arr)
 {
     *arr = nullptr;
     if (delta_fast(v)) return MS_OK;
@@ -528,7 +528,7 @@ static int delta_src(const ms_col *in, const ms_colv &v, scratch &sc, const uint
     return MS_OK;
 }
 
-int launch_delta_carries(const ms_col *in, scratch &sc)
+int launch_delta_carries(const ms_col *in, temps &sc)
 {
     ms_colv v;
     MS_TRY(colv_of(in, v));
@@ -586,7 +586,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     (void)accumulate;
     ms_colv v;
     MS_TRY(colv_of(in, v));
-    scratch sc(s);
+    temps sc(s);
     const uint64_t nb = in->count_bound;
     uint32_t grid = 1;
     int with_rest = 1;
@@ -660,7 +660,7 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
     case MS_UNCOMPR:
         break;
     case MS_RLE: {
-        scratch sc(s);
+        temps sc(s);
         const uint64_t runs_bound = in->size_alloc / 16 + 1;
         uint64_t *len = (uint64_t *)sc.get(runs_bound * 8);
         uint64_t *st = (uint64_t *)sc.get(runs_bound * 8);
@@ -685,7 +685,7 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         }
         break;
     case MS_DELTA: {
-        scratch sc(s);
+        temps sc(s);
         const uint64_t *arr;
         MS_TRY(delta_src(in, v, sc, &arr));
         const dim3 grid(grid_for(nb / DELTA_TILE + 1, 1, 65536));
@@ -822,7 +822,7 @@ __global__ __launch_bounds__(WG) void k_sel_compact_blocks(const uint64_t *flags
     if (blockIdx.x == 0 && threadIdx.x == 0 && nunits == 0) *d_cnt = 0;
 }
 
-int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc)
+int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, temps &sc)
 {
     hipStream_t s = sc.s;
     const uint64_t nb = in->count_bound;
@@ -971,7 +971,7 @@ __global__ __launch_bounds__(WG) void k_scan_apply(const uint64_t *in, uint64_t 
 }
 
 int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_t *d_n, uint64_t extra,
-                            uint64_t bound, scratch &sc)
+                            uint64_t bound, temps &sc)
 {
     if (bound == 0) return MS_OK;
     const uint32_t G = grid_for(bound, 4096, 1024);
@@ -985,7 +985,7 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
     return MS_OK;
 }
 
-int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc)
+int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, temps &sc)
 {
     return launch_exclusive_scan_n(d_in, d_out, nullptr, 0, n, sc);
 }

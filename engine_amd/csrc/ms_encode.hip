@@ -364,7 +364,7 @@ __global__ __launch_bounds__(WG) void k_emit_rest(emit_layout l, const uint64_t 
 }
 
 int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
-                uint64_t *d_append, scratch &sc)
+                uint64_t *d_append, temps &sc)
 {
     hipStream_t s = sc.s;
     const ms_format_desc &f = out->f;
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(WG) void k_rle_lens(const uint64_t *d_m, const uint
     }
 }
 
-int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, scratch &sc)
+int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc)
 {
     hipStream_t s = sc.s;
     const uint64_t bound = m_bound + 1;

@@ -9,17 +9,17 @@
 
 namespace ms {
 
-struct scratch;   // stream-ordered temporaries of one operator call
+struct temps;   // stream-ordered temporaries of one operator call
 
 // Thin RAII-free arena: every temporary of a call is freed at the end of
 // the call on the call's stream (hipFreeAsync keeps the ordering).
-struct scratch {
+struct temps {
     hipStream_t s;
     void *ptrs[64];
     int n = 0;
-    explicit scratch(hipStream_t st) : s(st) {}
+    explicit temps(hipStream_t st) : s(st) {}
     void *get(size_t bytes);
-    ~scratch();
+    ~temps();
 };
 
 int colv_of(const ms_col *c, ms_colv &v);
@@ -44,14 +44,14 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
 int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
 // ascending positions with pred into d_pos (capacity >= count bound);
 // *d_cnt receives their number
-int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, scratch &sc);
+int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, temps &sc);
 // rebuilds a delta_f column's tile carries from its image and index
-int launch_delta_carries(const ms_col *in, scratch &sc);
+int launch_delta_carries(const ms_col *in, temps &sc);
 // exclusive prefix sums of n u64 values (in-place allowed)
-int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, scratch &sc);
+int launch_exclusive_scan(const uint64_t *d_in, uint64_t *d_out, uint64_t n, temps &sc);
 // the same over the first min(*d_n + extra, bound) values (device count)
 int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_t *d_n, uint64_t extra,
-                            uint64_t bound, scratch &sc);
+                            uint64_t bound, temps &sc);
 
 // encode side -------------------------------------------------------
 // Emits the m values d_v[0 .. *d_m) into `out` (already allocated, format
@@ -60,17 +60,17 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
 // rest, and sets out's meta.  *d_append receives the write-iterator's
 // append offset (done()'s second element).
 int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
-                uint64_t *d_append, scratch &sc);
+                uint64_t *d_append, temps &sc);
 // rle_f compression of d_v[0 .. *d_m)
-int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, scratch &sc);
+int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc);
 
 // operators ---------------------------------------------------------
 int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_np, uint64_t np_bound,
-                  uint64_t *d_out, uint64_t *d_status, scratch &sc);
+                  uint64_t *d_out, uint64_t *d_status, temps &sc);
 int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
-                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc);
+                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, temps &sc);
 int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
-                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc);
+                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, temps &sc);
 int launch_calc(uint32_t op, const uint64_t *a, const uint64_t *b, const uint64_t *d_n, uint64_t n_bound,
                 uint64_t *d_out, hipStream_t s);
 // small device-side bookkeeping kernels

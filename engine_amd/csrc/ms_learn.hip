@@ -539,7 +539,7 @@ extern "C" int ms_gen_chunks(uint64_t *d_out, const ms_chunk_spec *specs, uint64
                          ((p.kind == MS_GEN_SORTED || p.kind == MS_GEN_RLE) && p.table >= ntables);
         if (bad) return ms_fail(MS_ERR_INVALID, "chunk spec %llu out of range", (unsigned long long)c);
     }
-    scratch sc(s);
+    temps sc(s);
     gen_spec *d_specs = (gen_spec *)sc.get(nchunks * sizeof(gen_spec));
     uint64_t *d_tab = (uint64_t *)sc.get((uint64_t)(ntables ? ntables : 1) * MS_GEO_LEN * 8);
     if (!d_specs || !d_tab) return ms_fail(MS_ERR_NOMEM, "generator scratch");
@@ -562,7 +562,7 @@ extern "C" int ms_chunk_stats(const uint64_t *d_values, uint64_t nchunks, uint64
     if (!d_values || !d_stats || (d_time_labels && !cost_table)) return ms_fail(MS_ERR_INVALID, "null argument");
     if (nchunks == 0) return MS_OK;
     hipStream_t s = (hipStream_t)stream;
-    scratch sc(s);
+    temps sc(s);
     float *d_cost = nullptr;
     if (cost_table) {
         d_cost = (float *)sc.get(MS_NCAND * 65 * sizeof(float));
@@ -707,7 +707,7 @@ extern "C" int ms_mlp_loss_grad(ms_mlp *m, const float *d_x, const float *d_y, c
         return ms_fail(MS_ERR_INVALID, "null argument");
     if (n > (1ull << 31)) return ms_fail(MS_ERR_CAPACITY, "too many samples");
     hipStream_t s = (hipStream_t)stream;
-    scratch sc(s);
+    temps sc(s);
     const uint64_t nwg = (n + SPW - 1) / SPW;
     float *part = (float *)sc.get(nwg * NPARAM * 4);
     float *lpart = (float *)sc.get(nwg * 4);
@@ -729,7 +729,7 @@ extern "C" int ms_mlp_train(ms_mlp *m, const float *d_x, const float *d_y, const
         return ms_fail(MS_ERR_INVALID, "bad training arguments (batch must be a multiple of %d)", SPW);
     if (n > (1ull << 31)) return ms_fail(MS_ERR_CAPACITY, "too many samples");
     hipStream_t s = (hipStream_t)stream;
-    scratch sc(s);
+    temps sc(s);
     const uint64_t nwg = batch / SPW;
     float *part = (float *)sc.get(nwg * NPARAM * 4);
     float *lpart = (float *)sc.get(nwg * 4);

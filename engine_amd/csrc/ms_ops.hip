@@ -79,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_gather(ms_colv c, const uint64_t *pos, c
 }
 
 int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_np, uint64_t np_bound,
-                  uint64_t *d_out, uint64_t *d_status, scratch &sc)
+                  uint64_t *d_out, uint64_t *d_status, temps &sc)
 {
     ms_colv v;
     MS_TRY(colv_of(data, v));
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(WG) void k_isect_emit(const uint64_t *a, const uint
 }
 
 int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
-                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc)
+                     const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, temps &sc)
 {
     (void)nb_bound;
     const uint64_t chunks = na_bound / ISECT_CHUNK + 1;
@@ -306,7 +306,7 @@ __global__ __launch_bounds__(WG) void k_merge_place(const uint64_t *a, const uin
 }
 
 int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, const uint64_t *b,
-                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, scratch &sc)
+                 const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, temps &sc)
 {
     const uint64_t bound = nb_bound + 1;
     uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);

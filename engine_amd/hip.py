@@ -41,7 +41,12 @@ def synchronize():
 
 
 class Stream:
-    """A created HIP stream (default priority)."""
+    """A created HIP stream (default priority).
+
+    The object itself is what `handle` returns and what ctypes passes as the
+    hipStream_t (`_as_parameter_`): the library's column wrappers keep the
+    Stream they were created on alive, because a column is released in
+    order on that stream, which therefore must outlive it."""
 
     def __init__(self):
         self.s = ctypes.c_void_p()
@@ -52,6 +57,10 @@ class Stream:
 
     @property
     def handle(self):
+        return self
+
+    @property
+    def _as_parameter_(self):
         return self.s
 
     def __del__(self):

@@ -175,7 +175,9 @@ const void *ms_col_device_data(const ms_col *c);
  * round trip (exact for uploaded columns). */
 uint64_t ms_col_count_bound(const ms_col *c);
 
-/* Releases the column (stream ordered on the stream it was created on). */
+/* Releases the column, stream ordered on the stream it was created on:
+ * that stream must still exist (destroy a stream only after freeing the
+ * columns created on it; the null stream always exists). */
 int ms_col_free(ms_col *c);
 
 /* ------------------------------------------------------------------ */
