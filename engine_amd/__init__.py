@@ -46,6 +46,7 @@ ABI = {
     "ms_project": (ctypes.c_int, [_COLP, _FD, VP, VP, U32, VP]),
     "ms_agg_sum": (ctypes.c_int, [_COLP, VP, VP]),
     "ms_select_sum": (ctypes.c_int, [VP, U32, U64, VP, VP]),
+    "ms_select_sum_noalloc": (ctypes.c_int, [VP, U32, U64, VP, VP]),
     "ms_intersect_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_merge_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_calc_binary": (ctypes.c_int, [_COLP, U32, _FD, VP, VP, VP]),
@@ -257,6 +258,14 @@ def select_sum(cmp, val, col, result=None, stream=None):
     """Fused select + agg_sum; returns the DeviceU64 holding the sum."""
     r = result or DeviceU64()
     check(lib().ms_select_sum(r.p, cmp, val, col.h, stream))
+    return r
+
+
+def select_sum_noalloc(cmp, val, col, result=None, stream=None):
+    """ms_select_sum_noalloc: the fused select + agg_sum with no device
+    allocation (its temporaries live in the library's per-device array)."""
+    r = result or DeviceU64()
+    check(lib().ms_select_sum_noalloc(r.p, cmp, val, col.h, stream))
     return r
 
 

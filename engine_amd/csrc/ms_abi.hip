@@ -13,6 +13,17 @@
 static thread_local std::string g_err;
 static int g_device = -1;
 
+// Device memory of the entry points that allocate nothing
+// (ms_select_sum_noalloc): SS_MAX_PARTS partial sums followed by a
+// temporary of NOALLOC_TMP_VALUES decoded deltas.  Each device holds its
+// own copy for the life of the process; host code reaches it only through
+// the address hipGetSymbolAddress returned for that device at ms_init.
+static constexpr uint64_t NOALLOC_TMP_VALUES = 1ull << 20;
+static constexpr uint64_t NOALLOC_WORDS = ms::SS_MAX_PARTS + NOALLOC_TMP_VALUES;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_WORDS];
+static constexpr int MAX_DEVICES = 64;
+static uint64_t *g_scratch_ptr[MAX_DEVICES];   // per device, set by ms_init
+
 int ms_fail(int code, const char *fmt, ...)
 {
     char buf[512];
@@ -221,6 +232,13 @@ extern "C" int ms_init(int device)
     MS_HIP(hipDeviceGetDefaultMemPool(&pool, device));
     uint64_t thr = ~0ull;
     MS_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr));
+    if (device >= MAX_DEVICES) return ms_fail(MS_ERR_NODEVICE, "device %d beyond the %d supported", device, MAX_DEVICES);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        MS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ms_fail(MS_ERR_HIP, "no device address for the no-allocation scratch array");
+        g_scratch_ptr[device] = (uint64_t *)p;
+    }
     g_device = device;
     return MS_OK;
 }
@@ -816,6 +834,18 @@ extern "C" int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val, con
     MS_TRY(ms_need_init());
     if (!d_result || !in || cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "bad argument");
     return launch_select_sum(in, make_pred(cmp, val), d_result, false, S(stream));
+}
+
+extern "C" int ms_select_sum_noalloc(uint64_t *d_result, uint32_t cmp, uint64_t val, const ms_col *in, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!d_result || !in || cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "bad argument");
+    int dev = -1;
+    MS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES || !g_scratch_ptr[dev])
+        return ms_fail(MS_ERR_NODEVICE, "ms_init was not called for device %d", dev);
+    const ss_buffers bufs = {g_scratch_ptr[dev], g_scratch_ptr[dev] + SS_MAX_PARTS, NOALLOC_TMP_VALUES};
+    return launch_select_sum(in, make_pred(cmp, val), d_result, false, S(stream), &bufs);
 }
 
 // ------------------------------------------------------------------ project

@@ -516,11 +516,21 @@ static bool delta_shiftable(const ms_colv &v) { return v.g.lg_dstep != 0xff && v
 
 // Deltas of a delta_f column whose inner layout has no fast decoder,
 // decoded per value into a temporary.
-static int delta_src(const ms_col *in, const ms_colv &v, temps &sc, const uint64_t **arr)
+static int delta_src(const ms_col *in, const ms_colv &v, temps &sc, const uint64_t **arr,
+                     const ss_buffers *bufs = nullptr)
 {
     *arr = nullptr;
     if (delta_fast(v)) return MS_OK;
-    uint64_t *tmp = (uint64_t *)sc.get(in->count_bound * 8 + 64);
+    uint64_t *tmp;
+    if (bufs) {
+        if (in->count_bound > bufs->tmp_values)
+            return ms_fail(MS_ERR_CAPACITY, "delta_f column of %llu values without a unit decoder exceeds the "
+                           "%llu-value temporary of the caller's buffers",
+                           (unsigned long long)in->count_bound, (unsigned long long)bufs->tmp_values);
+        tmp = bufs->tmp;
+    } else {
+        tmp = (uint64_t *)sc.get(in->count_bound * 8 + 64);
+    }
     if (!tmp) return ms_fail(MS_ERR_NOMEM, "delta scratch");
     hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(in->count_bound, WG * 8ull)), dim3(WG), 0, sc.s, v, tmp);
     MS_HIP(hipGetLastError());
@@ -581,7 +591,8 @@ static uint32_t stride_grid(const void *kernel, uint64_t items)
     return grid_for(items, WAVES, r < 8192 ? r : 8192);
 }
 
-int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s)
+int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s,
+                      const ss_buffers *bufs)
 {
     (void)accumulate;
     ms_colv v;
@@ -590,7 +601,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     const uint64_t nb = in->count_bound;
     uint32_t grid = 1;
     int with_rest = 1;
-    uint64_t *part = (uint64_t *)sc.get(8192 * 8);
+    uint64_t *part = bufs ? bufs->part : (uint64_t *)sc.get(SS_MAX_PARTS * 8);
     if (!part) return ms_fail(MS_ERR_NOMEM, "select_sum scratch");
     switch (v.g.kind) {
     case MS_UNCOMPR:
@@ -629,7 +640,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         break;
     case MS_DELTA: {
         const uint64_t *arr;
-        MS_TRY(delta_src(in, v, sc, &arr));
+        MS_TRY(delta_src(in, v, sc, &arr, bufs));
         if (!arr && v.g.lg_spb != 0xff) {
             grid = stride_grid((const void *)k_delta_select_sum, nb / DELTA_TILE + 1);
             hipLaunchKernelGGL(k_delta_select_sum, dim3(grid), dim3(WG), 0, s, v, p, part);
@@ -644,6 +655,7 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         return ms_fail(MS_ERR_INVALID, "select_sum: unsupported format");
     }
     MS_HIP(hipGetLastError());
+    static_assert(SS_MAX_PARTS >= 8192, "every grid above is capped at 8192 workgroups");
     hipLaunchKernelGGL(k_sum_finish, dim3(1), dim3(WG), 0, s, v, p, (const uint64_t *)part, grid, with_rest, d_result);
     MS_HIP(hipGetLastError());
     return MS_OK;

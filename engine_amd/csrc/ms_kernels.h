@@ -37,9 +37,20 @@ void dev_free(void *p, hipStream_t s);
 
 // decode side -------------------------------------------------------
 int launch_fill_u64(uint64_t *p, uint64_t n, uint64_t v, hipStream_t s);
+// Caller-provided device buffers of launch_select_sum: `part` holds
+// SS_MAX_PARTS per-workgroup partial sums; `tmp` (tmp_values values) holds
+// the decoded deltas of a delta_f column without a unit decoder.  Every
+// kernel writes what it later reads, so neither needs clearing.
+constexpr uint64_t SS_MAX_PARTS = 8192;
+struct ss_buffers {
+    uint64_t *part;
+    uint64_t *tmp;
+    uint64_t tmp_values;
+};
 // *d_result (+)= sum of x_i with pred(x_i); the result is zeroed first
-// unless `accumulate`.
-int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s);
+// unless `accumulate`.  With `bufs` no device memory is allocated.
+int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s,
+                      const ss_buffers *bufs = nullptr);
 // writes the column's `count` values to d_out (uncompressed)
 int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
 // ascending positions with pred into d_pos (capacity >= count bound);

@@ -223,6 +223,18 @@ int ms_agg_sum(ms_col **out, const ms_col *in, void *stream);
 int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val,
                   const ms_col *in, void *stream);
 
+/* The same as ms_select_sum, allocating nothing: no reference counterpart;
+ * for callers that cannot allocate device memory (code under hipGraph
+ * stream capture, tight serving loops).  Its temporaries live in a fixed
+ * per-device array of 8 MiB + 64 KiB set up by ms_init: 8192 per-workgroup
+ * partial sums and room for 2^20 decoded deltas, which only a delta_f
+ * column whose inner format has no unit decoder needs (such a column of
+ * more than 2^20 values is refused with MS_ERR_CAPACITY).  All calls on
+ * one device share that array: calls that overlap on one device (other
+ * streams or threads) race and must be serialised by the caller. */
+int ms_select_sum_noalloc(uint64_t *d_result, uint32_t cmp, uint64_t val,
+                          const ms_col *in, void *stream);
+
 /* intersect_sorted<ve, out_f, l_f, r_f>(l, r): multiset intersection of two
  * ascending columns (min multiplicity, as scalar/intersect_uncompr.h:69-80),
  * written through the selective write-iterator into out_f. */
