@@ -35,6 +35,12 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 METRIC = "fused select+sum GB/s over compressed cols, %HBM peak; batched vs plain"
 CMP_LT = 2
+# The sweep's batch shape: fixed, so that the profiles and the traffic
+# measurement under profiles/ belong to exactly the timed launch.  The
+# autotuner's own pick and its gain over the library default are reported
+# beside it (autotune).
+SWEEP_CFG = {"wg_per_cu": 4, "units_per_item": 32, "tiles_per_item": 8, "runs_per_item": 4096,
+             "values_per_item": 2048, "waves_per_simd": 0}
 
 
 def log(*a):
@@ -63,22 +69,44 @@ def make_layouts(n, seed, with_uncompr=False):
         yield "uncompr", uncompr_f(), datagen.uniform_bits(seed, n, 32), 1 << 31
 
 
-def cpu_baseline(col_gpu, fmt, val, threads_all):
-    """The oracle's streaming select+sum on the GPU-produced image (host
-    cores of this machine), median of 5 after one warm-up."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cols, threads_all, reps=5):
+    """The same 4-column step on the host cores: the CPU baseline's
+    streaming select + sum (oracle/ms_cpu.c) over the GPU-produced images,
+    in its AVX-512 style (scalar where the CPU lacks AVX-512) and its scalar
+    style, on 1 thread (the reference's execution model) and on all the
+    process's cores; median of `reps` steps after one warm-up."""
     from oracle import oracle as orc
-    img, meta = col_gpu.image()
-    ocol = orc.from_image(fmt, img, *meta)
+    hosts = []
+    for c in cols:
+        img, meta = c["col"].image()
+        hosts.append((orc.from_image(c["fmt"], img, *meta), c["val"], c["sum"]))
+    styles = [("avx512", 1)] if orc.cpu_has_avx512() else []
+    styles.append(("scalar", 0))
     out = {}
-    for threads in (1, threads_all):
-        orc.select_sum_stream(CMP_LT, val, ocol, threads)
-        ts = []
-        for _ in range(5):
-            t0 = time.perf_counter()
-            r = orc.select_sum_stream(CMP_LT, val, ocol, threads)
-            ts.append(time.perf_counter() - t0)
-        out[threads] = (statistics.median(ts), r)
-    return out, meta
+    for style, simd in styles:
+        for threads in (1, threads_all):
+            def step():
+                return [orc.cpu_select_sum(CMP_LT, v, h, threads, simd) for h, v, _ in hosts]
+            got = step()
+            if got != [w for _, _, w in hosts]:
+                raise RuntimeError(f"CPU baseline ({style}, {threads} threads) result mismatch")
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                step()
+                ts.append(time.perf_counter() - t0)
+            out[(style, threads)] = statistics.median(ts)
+    return out, styles[0][0]
 
 
 def timed(fn, steps, warmup, hip):
@@ -93,6 +121,39 @@ def timed(fn, steps, warmup, hip):
         fn()
     e1.record()
     return e0.elapsed_ms(e1) / steps
+
+
+def codec_rates(cols, n, args, ea, hip, datagen):
+    """Column-level morph of each sweep layout: decompression (format ->
+    uncompr_f) and compression (uncompr_f -> format), through ms_morph with
+    the columns resident in HBM.  Algorithmic bytes of either direction:
+    sizeUsed + 8 n.  The decompressed values are checked against the input
+    hash and the recompressed image against the original's hash first."""
+    from engine_amd.formats import uncompr_f
+    out = {}
+    for c in cols:
+        raw = ea.morph(uncompr_f(), c["col"])
+        vals = raw.values()
+        if datagen.column_hash(vals) != c["hash"]:
+            raise RuntimeError(f"{c['name']}: decompressed values differ from the input")
+        del vals
+        again = ea.morph(c["fmt"], raw)
+        if datagen.column_hash(again.image()[0].view(np.uint8)) != c["image_hash"]:
+            raise RuntimeError(f"{c['name']}: recompressed image differs")
+        del again
+        ms_d = timed(lambda: ea.morph(uncompr_f(), c["col"]), args.steps, args.warmup, hip)
+        ms_c = timed(lambda: ea.morph(c["fmt"], raw), args.steps, args.warmup, hip)
+        alg = c["size_used"] + 8 * n
+        out[c["name"]] = {
+            "algorithmic_bytes": alg,
+            "decompress_us": round(ms_d * 1e3, 2), "decompress_GBps": round(alg / ms_d / 1e6, 1),
+            "decompress_frac": round(alg / ms_d / 1e6 / HBM_PEAK_GBS, 4),
+            "compress_us": round(ms_c * 1e3, 2), "compress_GBps": round(alg / ms_c / 1e6, 1),
+            "compress_frac": round(alg / ms_c / 1e6 / HBM_PEAK_GBS, 4)}
+        log(f"{c['name']}: decompress {ms_d * 1e3:.1f} us ({alg / ms_d / 1e6:.1f} GB/s), "
+            f"compress {ms_c * 1e3:.1f} us ({alg / ms_c / 1e6:.1f} GB/s)")
+        del raw
+    return out
 
 
 def c1_workload(args, ea, hip, datagen):
@@ -228,32 +289,35 @@ def c4_workload(args, ea, hip, datagen):
     verify("plain")
     ms_plain = timed(plain, max(2, args.steps // 10), 1, hip)
     # the same per-column calls recorded once and replayed as one hipGraph
-    st = hip.Stream()
-    plan = ea.Plan()
+    # (left out of profiling runs: --no-graph-legs)
+    ms_graph = None
+    if not args.no_graph_legs:
+        st = hip.Stream()
+        plan = ea.Plan()
 
-    def plain_on(stream):
-        for k, (col, val) in enumerate(cols):
-            ea.select_sum(CMP_LT, val, col, slots[k], stream)
+        def plain_on(stream):
+            for k, (col, val) in enumerate(cols):
+                ea.select_sum(CMP_LT, val, col, slots[k], stream)
 
-    with plan.record(st.handle):
-        plain_on(st.handle)
-    with plan.record(st.handle):
-        plain_on(st.handle)
-    res.zero()                 # the replay must write every result
-    plan.launch(st.handle)
-    st.synchronize()
-    verify("plain graph")
-    for _ in range(2):
+        with plan.record(st.handle):
+            plain_on(st.handle)
+        with plan.record(st.handle):
+            plain_on(st.handle)
+        res.zero()                 # the replay must write every result
         plan.launch(st.handle)
-    st.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
-    reps = max(2, args.steps // 4)
-    e0.record(st.handle)
-    for _ in range(reps):
-        plan.launch(st.handle)
-    e1.record(st.handle)
-    ms_graph = e0.elapsed_ms(e1) / reps
-    plan.free()
+        st.synchronize()
+        verify("plain graph")
+        for _ in range(2):
+            plan.launch(st.handle)
+        st.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        reps = max(2, args.steps // 4)
+        e0.record(st.handle)
+        for _ in range(reps):
+            plan.launch(st.handle)
+        e1.record(st.handle)
+        ms_graph = e0.elapsed_ms(e1) / reps
+        plan.free()
     # one batch, autotuned
     b = ea.Batch()
     for col, val in cols:
@@ -268,18 +332,18 @@ def c4_workload(args, ea, hip, datagen):
     verify("batched tuned")
     ms_batch = timed(lambda: b.run(res), args.steps, args.warmup, hip)
     _, items, _ = b.stats()
-    gb = lambda ms: round((bytes_used + 8 * nc) / ms / 1e6, 1)  # noqa: E731
-    log(f"c4: plain {ms_plain:.3f} ms ({gb(ms_plain)} GB/s), plain graph {ms_graph:.3f} ms ({gb(ms_graph)} GB/s), "
+    gb = lambda ms: round((bytes_used + 8 * nc) / ms / 1e6, 1) if ms else None  # noqa: E731
+    log(f"c4: plain {ms_plain:.3f} ms ({gb(ms_plain)} GB/s), plain graph {ms_graph} ms ({gb(ms_graph)} GB/s), "
         f"batched {ms_batch:.3f} ms ({gb(ms_batch)} GB/s), tuned {cfg.as_dict()} in {tune_s:.1f}s")
     return {
         "workload": f"{nc} uint64 columns, lengths log-uniform in [10^3, 10^6], formats {kinds}, "
                     f"select(x<c)+agg_sum per column",
         "values": nvals, "compressed_bytes": bytes_used,
         "plain_ms": round(ms_plain, 4), "plain_GBps": gb(ms_plain),
-        "plain_graph_ms": round(ms_graph, 4), "plain_graph_GBps": gb(ms_graph),
+        "plain_graph_ms": round(ms_graph, 4) if ms_graph else None, "plain_graph_GBps": gb(ms_graph),
         "batched_ms": round(ms_batch, 4), "batched_GBps": gb(ms_batch),
         "batched_pct_hbm_peak": round(100 * gb(ms_batch) / HBM_PEAK_GBS, 2),
-        "speedup_vs_plain": round(ms_plain / ms_batch, 2), "speedup_vs_plain_graph": round(ms_graph / ms_batch, 2),
+        "speedup_vs_plain": round(ms_plain / ms_batch, 2), "speedup_vs_plain_graph": round(ms_graph / ms_batch, 2) if ms_graph else None,
         "work_items": items, "tuned_config": cfg.as_dict(), "tune_seconds": round(tune_s, 1),
     }
 
@@ -299,6 +363,10 @@ def main():
     ap.add_argument("--batch-config", default=None,
                     help="JSON ms_batch_config for the sweep's batch instead of autotuning (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner comparison of the sweep")
+    ap.add_argument("--no-codecs", action="store_true", help="skip the decompression / compression rates")
+    ap.add_argument("--no-graph-legs", action="store_true",
+                    help="skip the per-column calls replayed as one hipGraph (profiling runs)")
     ap.add_argument("--only", default=None, help="only the layouts whose name starts with this (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -329,7 +397,8 @@ def main():
             log(f"MISMATCH {name}: gpu {got} numpy {expect}")
             sys.exit(1)
         cols.append(dict(name=name, fmt=fmt, col=col, val=val, size_used=m.size_used, count=m.count,
-                         hash=datagen.column_hash(x), sum=expect, upload_GBps=8 * n / up_s / 1e9))
+                         hash=datagen.column_hash(x), sum=expect, upload_GBps=8 * n / up_s / 1e9,
+                         image_hash=datagen.column_hash(col.image()[0].view(np.uint8)) if fmt.kind else None))
         log(f"{name}: n={n} sizeUsed={m.size_used} ({8 * n / m.size_used:.2f}x) prepared in "
             f"{time.perf_counter() - t0:.1f}s, sum verified")
         del x
@@ -357,12 +426,26 @@ def main():
     if res.get().tolist() != [c["sum"] for c in cols]:
         log("MISMATCH batched step")
         sys.exit(1)
-    if args.batch_config:
-        from engine_amd.formats import BatchConfig
-        batch.config = BatchConfig(**json.loads(args.batch_config))
-        cfg = batch.config
-    else:
-        cfg = batch.tune(res, reps=5)
+    from engine_amd.formats import BatchConfig
+    # the library default, then the autotuner's pick, then the fixed shape
+    # (a profiling run, --no-tune, launches the fixed shape only)
+    autotune = None
+    if not args.no_tune:
+        ms_default = timed(lambda: batch.run(res), args.steps, args.warmup, hip)
+        t1 = time.perf_counter()
+        tuned = batch.tune(res, reps=5)
+        tune_s = time.perf_counter() - t1
+        batch.run(res)
+        if res.get().tolist() != [c["sum"] for c in cols]:
+            log("MISMATCH batched step (tuned)")
+            sys.exit(1)
+        ms_tuned = timed(lambda: batch.run(res), args.steps, args.warmup, hip)
+        autotune = {"default_us": round(ms_default * 1e3, 2), "tuned_us": round(ms_tuned * 1e3, 2),
+                    "tuned_over_default": round(ms_default / ms_tuned, 3), "tuned_config": tuned.as_dict(),
+                    "tune_seconds": round(tune_s, 2)}
+        log(f"autotune: default {ms_default * 1e3:.1f} us, tuned {ms_tuned * 1e3:.1f} us {tuned.as_dict()}")
+    batch.config = BatchConfig(**(json.loads(args.batch_config) if args.batch_config else SWEEP_CFG))
+    cfg = batch.config
     batch.run(res)
     if res.get().tolist() != [c["sum"] for c in cols]:
         log("MISMATCH batched step (tuned)")
@@ -384,10 +467,14 @@ def main():
     log(f"step: plain {ms_plain * 1e3:.1f} us ({value_plain:.1f} GB/s), batched {ms_step * 1e3:.1f} us "
         f"({value:.1f} GB/s), tuned {cfg.as_dict()}")
     achieved = total_bytes / ms_kernel / 1e6
+    # HBM bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    # profiles/) at this exact shape; null when the shape differs
     traffic = None
     tpath = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic_batch_select_sum.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("bytes_per_launch")
+        t = json.load(open(tpath))
+        if t.get("batch_config") == cfg.as_dict() and t.get("algorithmic_bytes") == total_bytes:
+            traffic = t.get("bytes_per_launch")
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak",
@@ -410,7 +497,14 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_us": round(ms_kernel * 1e3, 2), "algorithmic_bytes": total_bytes},
+        "autotune": autotune,
     }
+    if not args.no_codecs and cols:
+        try:
+            out["codecs"] = codec_rates(cols, n, args, ea, hip, datagen)
+        except Exception as e:     # the sweep's line above stands on its own
+            log(f"codec measurement failed: {e!r}")
+            out["codecs"] = {"error": repr(e)}
     extra = [("static_vbp_query", c1_workload, not args.no_c1), ("plan_query", c3_workload, args.c3_log2n > 0),
              ("uneven_columns", c4_workload, args.c4_cols > 0)]
     for key, fn, on in extra:
@@ -424,21 +518,25 @@ def main():
         from engine_amd import learn
         try:
             out["learned_selection"] = learn.run_selection_workload(ncols=args.c5_cols, ntrain=args.c5_train,
-                                                                    steps=max(2, args.steps // 2), log=log)
+                                                                    steps=max(2, args.steps // 2), log=log,
+                                                                    graph_leg=not args.no_graph_legs)
         except Exception as e:     # the sweep's line above stands on its own
             log(f"learned-selection workload failed: {e!r}")
             out["learned_selection"] = {"error": repr(e)}
     if not args.no_cpu and cols:
-        dom = cols[0]
-        threads_all = len(os.sched_getaffinity(0))
-        r, meta = cpu_baseline(dom["col"], dom["fmt"], dom["val"], threads_all)
-        assert r[1][1] == dom["sum"] and r[threads_all][1] == dom["sum"], "CPU baseline result mismatch"
+        threads_all = min(len(os.sched_getaffinity(0)), 16)
+        r, best = cpu_baseline(cols, threads_all)
+        gbs = lambda sec: round(total_bytes / sec / 1e9, 3)  # noqa: E731
         out["cpu_baseline"] = {
-            "value": round(meta[1] / r[1][0] / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{dom['name']}, 2^{args.log2n} values ({meta[1]} compressed bytes), oracle streaming "
-                      f"select+sum, median of 5",
-            "all_cores": {"cores": threads_all, "value": round(meta[1] / r[threads_all][0] / 1e9, 3)},
+            "value": gbs(r[(best, 1)]), "unit": "GB/s", "cores": 1, "kind": "port",
+            "style": best, "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"the same step: select(x<c)+sum over the {nc} sweep columns of 2^{args.log2n} values "
+                      f"({total_bytes} compressed bytes), streaming restatement of the reference's "
+                      f"decompress_and_process_batch ({best}), median of 5 steps",
+            "all_cores": {"cores": threads_all, "value": gbs(r[(best, threads_all)])},
+            "styles": {f"{st}_{th}t": gbs(sec) for (st, th), sec in r.items()},
         }
+        log(f"cpu baseline ({out['cpu_baseline']['cpu']}): {out['cpu_baseline']['styles']}")
     print(json.dumps(out), flush=True)
 
 

@@ -41,8 +41,12 @@ ABI = {
     "ms_col_count_bound": (U64, [VP]),
     "ms_col_free": (ctypes.c_int, [VP]),
     "ms_col_check": (ctypes.c_int, [VP, VP]),
+    "ms_col_store": (ctypes.c_int, [VP, ctypes.c_char_p, VP]),
+    "ms_col_load": (ctypes.c_int, [_COLP, _FD, ctypes.c_char_p, VP]),
+    "ms_col_prepare_for_random_access": (ctypes.c_int, [VP, P(ctypes.c_int), VP]),
     "ms_morph": (ctypes.c_int, [_COLP, _FD, VP, VP]),
     "ms_select": (ctypes.c_int, [_COLP, U32, U64, _FD, VP, U32, VP]),
+    "ms_between": (ctypes.c_int, [_COLP, U32, U64, U32, U64, _FD, VP, U32, VP]),
     "ms_project": (ctypes.c_int, [_COLP, _FD, VP, VP, U32, VP]),
     "ms_agg_sum": (ctypes.c_int, [_COLP, VP, VP]),
     "ms_select_sum": (ctypes.c_int, [VP, U32, U64, VP, VP]),
@@ -50,6 +54,9 @@ ABI = {
     "ms_intersect_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_merge_sorted": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_calc_binary": (ctypes.c_int, [_COLP, U32, _FD, VP, VP, VP]),
+    "ms_group": (ctypes.c_int, [_COLP, _COLP, _FD, _FD, VP, VP, VP]),
+    "ms_join": (ctypes.c_int, [_COLP, _COLP, _FD, _FD, VP, VP, VP]),
+    "ms_semi_join": (ctypes.c_int, [_COLP, _FD, VP, VP, VP]),
     "ms_plan_create": (ctypes.c_int, [_COLP, U64]),
     "ms_plan_begin": (ctypes.c_int, [VP, VP]),
     "ms_plan_end": (ctypes.c_int, [VP, VP]),
@@ -95,7 +102,9 @@ def lib():
             raise ImportError(f"{LIB_PATH} is missing: run `python -m engine_amd.build` (hipcc, gfx950)")
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in ABI.items():
-            fn = getattr(L, name)
+            fn = getattr(L, name, None)
+            if fn is None:   # a stale build: the call fails loudly when made
+                continue
             fn.restype, fn.argtypes = res, args
         _lib = L
     return _lib
@@ -155,6 +164,16 @@ class Column:
     def check(self, stream=None):
         check(lib().ms_col_check(self.h, stream))
 
+    def store(self, path, stream=None):
+        """binary_io<F>::store."""
+        check(lib().ms_col_store(self.h, os.fsencode(path), stream))
+
+    def prepare_for_random_access(self, stream=None):
+        """column<F>::prepare_for_random_access; True if the column changed."""
+        acted = ctypes.c_int()
+        check(lib().ms_col_prepare_for_random_access(self.h, ctypes.byref(acted), stream))
+        return bool(acted.value)
+
     @property
     def device_ptr(self):
         return lib().ms_col_device_data(self.h)
@@ -186,12 +205,23 @@ def upload_image(fmt, image, count, size_used, size_compr, stream=None):
                 stream)
 
 
+def load(fmt, path, stream=None):
+    """binary_io<F>::load."""
+    init()
+    return _new(lib().ms_col_load, ctypes.byref(fmt), os.fsencode(path), stream)
+
+
 def morph(fmt, col, stream=None):
     return _new(lib().ms_morph, ctypes.byref(fmt), col.h, stream)
 
 
 def select(cmp, val, out_fmt, col, lanes=8, stream=None):
     return _new(lib().ms_select, cmp, val, ctypes.byref(out_fmt), col.h, lanes, stream)
+
+
+def between(cmp_lower, val_lower, cmp_upper, val_upper, out_fmt, col, lanes=8, stream=None):
+    return _new(lib().ms_between, cmp_lower, val_lower, cmp_upper, val_upper, ctypes.byref(out_fmt), col.h, lanes,
+                stream)
 
 
 def project(out_fmt, data, pos, lanes=8, stream=None):
@@ -212,6 +242,29 @@ def merge_sorted(out_fmt, l, r, stream=None):
 
 def calc_binary(op, out_fmt, l, r, stream=None):
     return _new(lib().ms_calc_binary, op, ctypes.byref(out_fmt), l.h, r.h, stream)
+
+
+def _new2(fn, *args):
+    a, b = VP(), VP()
+    check(fn(ctypes.byref(a), ctypes.byref(b), *args))
+    st = next((x for x in args if isinstance(x, Stream)), None)
+    return Column(a, st), Column(b, st)
+
+
+def group(gr_fmt, ext_fmt, data, in_gr=None, stream=None):
+    """group (unary without in_gr, binary with it): (group ids, extents)."""
+    return _new2(lib().ms_group, ctypes.byref(gr_fmt), ctypes.byref(ext_fmt), in_gr.h if in_gr else None, data.h,
+                 stream)
+
+
+def join(l_fmt, r_fmt, l, r, stream=None):
+    """nested_loop_join: (left positions, right positions) of equal pairs."""
+    return _new2(lib().ms_join, ctypes.byref(l_fmt), ctypes.byref(r_fmt), l.h, r.h, stream)
+
+
+def semi_join(out_fmt, l, r, stream=None):
+    """left_semi_nto1_nested_loop_join: left positions with a match."""
+    return _new(lib().ms_semi_join, ctypes.byref(out_fmt), l.h, r.h, stream)
 
 
 class DeviceU64:

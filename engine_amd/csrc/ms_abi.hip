@@ -310,20 +310,25 @@ static uint64_t wit_size_max(const ms_format_desc &f, uint64_t m, uint64_t K)
 
 static uint64_t out_capacity(const ms_format_desc &f, uint64_t count_bound, uint64_t K)
 {
-    // plus room for the raw appends of select/project behind the rest
-    return wit_size_max(f, count_bound, K) + 8 * 64 + 256;
+    // plus room for the raw appends of select/project behind the rest, and
+    // for the zero padding of the rest to two blocks that a preparation
+    // for random access writes
+    const ms_geom g = ms_make_geom(f);
+    return wit_size_max(f, count_bound, K) + 8 * 64 + 256 + 16 * g.bs;
 }
 
 static hipStream_t S(void *stream) { return (hipStream_t)stream; }
 
 // Builds the block index of an image on the host (the producer-side kernels
 // build it on the device).
-static int host_index(const ms_format_desc &f, const ms_geom &g, const uint8_t *img, const ms_col_meta &m,
-                      std::vector<uint64_t> &idx, std::vector<uint64_t> &refs)
+static int host_index_walk(const ms_format_desc &f, const ms_geom &g, const uint8_t *img, const ms_col_meta &m,
+                           std::vector<uint64_t> &idx, std::vector<uint64_t> &refs, uint64_t *walked)
 {
+    struct set_end { uint64_t *w; const uint64_t &p; ~set_end() { if (w) *w = p; } };
     const ms_parts pt = ms_parts_of(g, m.count, m.size_used, m.size_compr);
     const uint64_t end = m.size_compr;
     uint64_t pos = 0;
+    const set_end report{walked, pos};   // the byte the walk ended at
     auto leaf_call = [&](uint64_t n) -> int {
         // one morph_batch call of the leaf layout over n values at pos
         if (g.leaf == MS_UNCOMPR) {
@@ -379,6 +384,12 @@ static int host_index(const ms_format_desc &f, const ms_geom &g, const uint8_t *
     }
 }
 
+static int host_index(const ms_format_desc &f, const ms_geom &g, const uint8_t *img, const ms_col_meta &m,
+                      std::vector<uint64_t> &idx, std::vector<uint64_t> &refs, uint64_t *walked = nullptr)
+{
+    return host_index_walk(f, g, img, m, idx, refs, walked);
+}
+
 static int check_meta(const ms_format_desc &f, const ms_col_meta &m)
 {
     const ms_geom g = ms_make_geom(f);
@@ -415,7 +426,8 @@ extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *
     MS_TRY(host_index(*f, g, (const uint8_t *)image, *meta, idx, refs));
     ms_col *c;
     uint64_t alloc = meta->size_used;
-    if (f->kind != MS_UNCOMPR && f->kind != MS_RLE) alloc = std::max(alloc, size_max_any_len(*f, meta->count));
+    if (f->kind != MS_UNCOMPR && f->kind != MS_RLE)   // room for a preparation for random access too
+        alloc = std::max(alloc, size_max_any_len(*f, meta->count)) + 16 * g.bs + 64;
     MS_TRY(new_col(*f, meta->count, alloc, S(stream), &c));
     if (idx.size() > c->index_cap || refs.size() > c->refs_cap) {
         ms_col_free(c);
@@ -502,10 +514,13 @@ extern "C" int ms_col_download(const ms_col *c, void *image, uint64_t capacity, 
     if (c->f.kind == MS_UNCOMPR || c->f.kind == MS_RLE || m.size_used == m.size_compr) {
         MS_HIP(hipMemcpyAsync(image, c->data, m.size_used, hipMemcpyDeviceToHost, S(stream)));
     } else {
-        // the gap between the compressed part and the rest reads as zeros
+        // the gap between the compressed part and the rest reads as zeros,
+        // but for the compressed rest a preparation for random access put
+        // at its start
         uint8_t *h = (uint8_t *)image;
-        memset(h + m.size_compr, 0, pt.rest_off - m.size_compr);
-        if (m.size_compr) MS_HIP(hipMemcpyAsync(h, c->data, m.size_compr, hipMemcpyDeviceToHost, S(stream)));
+        const uint64_t head = m.size_compr + c->prepared_bytes;
+        memset(h + head, 0, pt.rest_off - head);
+        if (head) MS_HIP(hipMemcpyAsync(h, c->data, head, hipMemcpyDeviceToHost, S(stream)));
         MS_HIP(hipMemcpyAsync(h + pt.rest_off, c->data + pt.rest_off, m.size_used - pt.rest_off,
                               hipMemcpyDeviceToHost, S(stream)));
     }
@@ -535,6 +550,125 @@ extern "C" int ms_col_free(ms_col *c)
     }
     delete c;
     return MS_OK;
+}
+
+static int emit_new(const ms_format_desc &f, const uint64_t *vals, const uint64_t *d_m, uint64_t m_bound,
+                    uint64_t chunk, uint64_t extra_values, temps &sc, ms_col **out, uint64_t **d_append);
+
+// column<F>::prepare_for_random_access (column.h:241-268): the rest,
+// zero-padded to a multiple of the block size, compressed by one fresh
+// morph_batch call to the end of the compressed part (inside the gap).
+extern "C" int ms_col_prepare_for_random_access(ms_col *c, int *acted, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!c) return ms_fail(MS_ERR_INVALID, "null column");
+    if (acted) *acted = 0;
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "prepare_for_random_access synchronises: not in a plan");
+    if (c->f.kind == MS_RLE) return ms_fail(MS_ERR_INVALID, "rle_f has no block size");
+    if (c->prepared || c->f.kind == MS_UNCOMPR) return MS_OK;
+    ms_col_meta m;
+    MS_TRY(ms_col_get_meta(c, &m, stream));
+    const ms_parts pt = ms_parts_of(c->g, m.count, m.size_used, m.size_compr);
+    if (!pt.n_rest) return MS_OK;
+    const uint64_t bs = c->g.bs;
+    const uint64_t ru = (pt.n_rest + bs - 1) / bs * bs;
+    if (pt.rest_off + 8 * ru > c->size_alloc)
+        return ms_fail(MS_ERR_CAPACITY, "no room to pad the rest of %llu values to %llu",
+                       (unsigned long long)pt.n_rest, (unsigned long long)ru);
+    temps sc(S(stream));
+    uint64_t *rest = (uint64_t *)(c->data + pt.rest_off);
+    MS_TRY(launch_fill_u64(rest + pt.n_rest, ru - pt.n_rest, 0, S(stream)));
+    uint64_t *d_m = (uint64_t *)sc.get(64);
+    if (!d_m) return ms_fail(MS_ERR_NOMEM, "prepare scratch");
+    MS_TRY(launch_fill_u64(d_m, 1, ru, S(stream)));
+    ms_col *t;
+    uint64_t *app;
+    MS_TRY(emit_new(c->f, rest, d_m, ru, 0, 0, sc, &t, &app));
+    ms_col_meta tm;
+    int rc = ms_col_get_meta(t, &tm, stream);
+    if (!rc && (tm.size_compr != tm.size_used || m.size_compr + tm.size_compr > pt.rest_off))
+        rc = ms_fail(MS_ERR_CAPACITY, "compressed rest does not fit the gap");
+    if (!rc && tm.size_compr &&
+        hipMemcpyAsync(c->data + m.size_compr, t->data, tm.size_compr, hipMemcpyDeviceToDevice, S(stream)) !=
+            hipSuccess)
+        rc = ms_fail(MS_ERR_HIP, "prepare copy");
+    if (!rc && hipStreamSynchronize(S(stream)) != hipSuccess) rc = ms_fail(MS_ERR_HIP, "prepare sync");
+    ms_col_free(t);
+    if (rc) return rc;
+    c->prepared = true;
+    c->prepared_bytes = tm.size_compr;
+    if (acted) *acted = 1;
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------ binary_io
+// binary_io<F>::store / load (binary_io.h:43-87): the file holds the count
+// and sizeUsed as two little-endian uint64, then the sizeUsed bytes of the
+// image.  The file has no sizeCompr: the reference's loaded column keeps the
+// constructor's 0 (right for uncompr_f only); here the loader derives it
+// from the image as a column-level morph of `count` values lays it out
+// (floor(count / block) values compressed, the rest raw behind the gap;
+// rle_f: all of it) and refuses an image that does not fit that layout.
+extern "C" int ms_col_store(const ms_col *c, const char *path, void *stream)
+{
+    if (!c || !path) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "binary_io synchronises: not in a plan");
+    ms_col_meta m;
+    MS_TRY(ms_col_get_meta(c, &m, stream));
+    std::vector<uint8_t> img(m.size_used + 1);
+    MS_TRY(ms_col_download(c, img.data(), img.size(), nullptr, stream));
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return ms_fail(MS_ERR_INVALID, "could open the file for writing: %s", path);
+    const uint64_t hdr[2] = {m.count, m.size_used};
+    const bool ok = fwrite(hdr, 8, 2, fp) == 2 && fwrite(img.data(), 1, m.size_used, fp) == m.size_used;
+    const bool closed = fclose(fp) == 0;
+    if (!ok || !closed) return ms_fail(MS_ERR_INVALID, "could write the column meta data and data: %s", path);
+    return MS_OK;
+}
+
+extern "C" int ms_col_load(ms_col **out, const ms_format_desc *f, const char *path, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out || !path) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (!valid_format(f)) return ms_fail(MS_ERR_INVALID, "invalid format");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return ms_fail(MS_ERR_INVALID, "could not open the file for reading: %s", path);
+    uint64_t hdr[2];
+    if (fread(hdr, 8, 2, fp) != 2) {
+        fclose(fp);
+        return ms_fail(MS_ERR_INVALID, "could not read the column meta data: %s", path);
+    }
+    std::vector<uint8_t> img;
+    try {
+        img.resize(hdr[1] + 1);
+    } catch (...) {
+        fclose(fp);
+        return ms_fail(MS_ERR_NOMEM, "%llu bytes for the column data", (unsigned long long)hdr[1]);
+    }
+    const bool ok = fread(img.data(), 1, hdr[1], fp) == hdr[1];
+    fclose(fp);
+    if (!ok) return ms_fail(MS_ERR_INVALID, "could not read the column data: %s", path);
+    ms_col_meta m = {hdr[0], hdr[1], 0, hdr[1]};
+    const ms_geom g = ms_make_geom(*f);
+    if (f->kind == MS_RLE) {
+        m.size_compr = m.size_used;
+    } else if (f->kind != MS_UNCOMPR) {
+        const uint64_t nc = m.count / g.bs * g.bs, rest = m.count - nc;
+        uint64_t end = 0;
+        std::vector<uint64_t> idx, refs;
+        if (f->kind == MS_STATIC_VBP) {
+            end = nc * f->bw / 8;
+        } else {
+            const ms_col_meta walk = {nc, m.size_used, m.size_used, m.size_used};
+            MS_TRY(host_index(*f, g, img.data(), walk, idx, refs, &end));
+        }
+        const uint64_t want = rest ? ms_rest_offset(g, end) + 8 * rest : end;
+        if (want != m.size_used)
+            return ms_fail(MS_ERR_INVALID, "the image does not have a column-level morph's layout of %llu values",
+                           (unsigned long long)m.count);
+        m.size_compr = end;
+    }
+    return ms_col_upload(out, f, img.data(), &m, stream);
 }
 
 // ------------------------------------------------------------------ plans
@@ -778,16 +912,23 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
         MS_TRY(new_col(in->f, nb, in->size_alloc, S(stream), &c));
         const uint64_t bytes = (uint64_t)(c->data - (uint8_t *)c->meta) + in->size_alloc;
         MS_HIP(hipMemcpyAsync(c->meta, in->meta, bytes, hipMemcpyDeviceToDevice, S(stream)));
+        c->prepared = in->prepared;
+        c->prepared_bytes = in->prepared_bytes;
+        *out = c;
+        return MS_OK;
+    }
+    ms_col *c;
+    if (dst->kind == MS_UNCOMPR) {
+        // decompression straight into the new column's buffer
+        MS_TRY(new_col(*dst, nb, 8 * nb, S(stream), &c));
+        int rc = launch_decompress(in, (uint64_t *)c->data, S(stream));
+        if (!rc) rc = launch_set_meta(c->meta, &in->meta->count, 8, 0, S(stream));
+        if (rc) { ms_col_free(c); return rc; }
         *out = c;
         return MS_OK;
     }
     const uint64_t *vals, *d_n;
     MS_TRY(values_of(in, sc, &vals, &d_n));
-    ms_col *c;
-    if (dst->kind == MS_UNCOMPR) {
-        uint64_t *app;
-        return emit_new(*dst, vals, d_n, nb, 0, 0, sc, out, &app);
-    }
     if (dst->kind == MS_RLE) {
         MS_TRY(new_col(*dst, nb, 16 * nb + 64, S(stream), &c));
         int rc = launch_rle_compress(c, vals, d_n, nb, sc);
@@ -803,15 +944,17 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
 }
 
 // ------------------------------------------------------------------ select
-extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_format_desc *out_f, const ms_col *in,
-                         uint32_t lanes, void *stream)
+// Positions with p(x) through the selective write-iterator into out_f, the
+// input rest's last (rest mod lanes) values processed "scalar" and their
+// positions appended raw (select_compr.h:100-260, between_compr.h:107-270).
+static int select_pred(ms_col **out, const ms_pred &p, const ms_format_desc *out_f, const ms_col *in, uint32_t lanes,
+                       void *stream)
 {
     MS_TRY(ms_need_init());
-    if (!out || !in || cmp > MS_CMP_GE || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!out || !in || lanes == 0 || lanes > 64) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
     temps sc(S(stream));
     const uint64_t nb = in->count_bound;
-    const ms_pred p = make_pred(cmp, val);
     uint64_t *pos = (uint64_t *)sc.get(nb * 8 + 64);
     uint64_t *d = (uint64_t *)sc.get(64);
     if (!pos || !d) return ms_fail(MS_ERR_NOMEM, "select scratch");
@@ -827,6 +970,28 @@ extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_form
     if (rc) { ms_col_free(c); return rc; }
     *out = c;
     return MS_OK;
+}
+
+extern "C" int ms_select(ms_col **out, uint32_t cmp, uint64_t val, const ms_format_desc *out_f, const ms_col *in,
+                         uint32_t lanes, void *stream)
+{
+    if (cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "comparison %u", cmp);
+    return select_pred(out, make_pred(cmp, val), out_f, in, lanes, stream);
+}
+
+// between_t<t_op_lower, t_op_upper>: x passes when both comparisons hold;
+// for the interval comparisons (all but NE) that is one interval.
+extern "C" int ms_between(ms_col **out, uint32_t cmp_lower, uint64_t val_lower, uint32_t cmp_upper,
+                          uint64_t val_upper, const ms_format_desc *out_f, const ms_col *in, uint32_t lanes,
+                          void *stream)
+{
+    if (cmp_lower > MS_CMP_GE || cmp_upper > MS_CMP_GE || cmp_lower == MS_CMP_NE || cmp_upper == MS_CMP_NE)
+        return ms_fail(MS_ERR_INVALID, "between takes the interval comparisons EQ, LT, LE, GT, GE");
+    const ms_pred a = make_pred(cmp_lower, val_lower), b = make_pred(cmp_upper, val_upper);
+    ms_pred p = {a.lo > b.lo ? a.lo : b.lo, a.hi < b.hi ? a.hi : b.hi, 0, 0};
+    p.empty = a.empty || b.empty || p.lo > p.hi;
+    if (p.empty) p.lo = 0, p.hi = ~0ull;
+    return select_pred(out, p, out_f, in, lanes, stream);
 }
 
 extern "C" int ms_select_sum(uint64_t *d_result, uint32_t cmp, uint64_t val, const ms_col *in, void *stream)
@@ -918,6 +1083,105 @@ extern "C" int ms_agg_sum(ms_col **out, const ms_col *in, void *stream)
     if (rc) { ms_col_free(c); return rc; }
     *out = c;
     return MS_OK;
+}
+
+// ------------------------------------------------------------------ group, join
+static int host_count(const ms_col *c, void *stream, uint64_t *n)
+{
+    ms_col_meta m;
+    MS_TRY(ms_col_get_meta(c, &m, stream));
+    *n = m.count;
+    return MS_OK;
+}
+
+extern "C" int ms_group(ms_col **out_gr, ms_col **out_ext, const ms_format_desc *gr_f, const ms_format_desc *ext_f,
+                        const ms_col *in_gr, const ms_col *in_data, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out_gr || !out_ext || !in_data) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(gr_f) || !wit_ok(gr_f) || !valid_format(ext_f) || !wit_ok(ext_f))
+        return ms_fail(MS_ERR_INVALID, "invalid output format");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "group synchronises: not in a plan");
+    uint64_t n, ng;
+    MS_TRY(host_count(in_data, stream, &n));
+    if (in_gr) {
+        MS_TRY(host_count(in_gr, stream, &ng));
+        if (ng != n)
+            return ms_fail(MS_ERR_SIZE, "binary group: inGrCol and inDataCol must contain the same number of data "
+                                        "elements");
+    }
+    temps sc(S(stream));
+    const uint64_t *x, *g = nullptr, *d_n, *d_ng;
+    MS_TRY(values_of(in_data, sc, &x, &d_n));
+    if (in_gr) MS_TRY(values_of(in_gr, sc, &g, &d_ng));
+    uint64_t *og = (uint64_t *)sc.get(8 * n + 64), *oe = (uint64_t *)sc.get(8 * n + 64);
+    uint64_t *d_cnt = (uint64_t *)sc.get(64);
+    if (!og || !oe || !d_cnt) return ms_fail(MS_ERR_NOMEM, "group scratch");
+    MS_TRY(launch_group(x, g, n, og, oe, d_cnt, sc));
+    ms_col *cg, *ce;
+    uint64_t *app;
+    MS_TRY(emit_new(*gr_f, og, d_n, n, wit_chunk(*gr_f), 0, sc, &cg, &app));
+    int rc = emit_new(*ext_f, oe, d_cnt, n, wit_chunk(*ext_f), 0, sc, &ce, &app);
+    if (rc) { ms_col_free(cg); return rc; }
+    *out_gr = cg;
+    *out_ext = ce;
+    return MS_OK;
+}
+
+extern "C" int ms_join(ms_col **out_l, ms_col **out_r, const ms_format_desc *l_f, const ms_format_desc *r_f,
+                       const ms_col *l, const ms_col *r, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out_l || !out_r || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(l_f) || !wit_ok(l_f) || !valid_format(r_f) || !wit_ok(r_f))
+        return ms_fail(MS_ERR_INVALID, "invalid output format");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "join synchronises: not in a plan");
+    uint64_t na, nb;
+    MS_TRY(host_count(l, stream, &na));
+    MS_TRY(host_count(r, stream, &nb));
+    temps sc(S(stream));
+    const uint64_t *a, *b, *d_na, *d_nb;
+    MS_TRY(values_of(l, sc, &a, &d_na));
+    MS_TRY(values_of(r, sc, &b, &d_nb));
+    // the outer loop runs over the longer input (scalar/join_uncompr.h:53-66)
+    const bool swap = na < nb;
+    uint64_t *d_total = (uint64_t *)sc.get(64);
+    if (!d_total) return ms_fail(MS_ERR_NOMEM, "join scratch");
+    uint64_t *po = nullptr, *pi = nullptr, total = 0;
+    MS_TRY(launch_join(swap ? b : a, swap ? nb : na, swap ? a : b, swap ? na : nb, false, &po, &pi, d_total, &total,
+                       sc));
+    const uint64_t *pl = swap ? pi : po, *pr = swap ? po : pi;
+    ms_col *cl, *cr;
+    uint64_t *app;
+    MS_TRY(emit_new(*l_f, pl, d_total, total, wit_chunk(*l_f), 0, sc, &cl, &app));
+    int rc = emit_new(*r_f, pr, d_total, total, wit_chunk(*r_f), 0, sc, &cr, &app);
+    if (rc) { ms_col_free(cl); return rc; }
+    *out_l = cl;
+    *out_r = cr;
+    return MS_OK;
+}
+
+extern "C" int ms_semi_join(ms_col **out, const ms_format_desc *out_f, const ms_col *l, const ms_col *r,
+                            void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out || !l || !r) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "join synchronises: not in a plan");
+    uint64_t na, nb;
+    MS_TRY(host_count(l, stream, &na));
+    MS_TRY(host_count(r, stream, &nb));
+    temps sc(S(stream));
+    const uint64_t *a, *b, *d_na, *d_nb;
+    MS_TRY(values_of(l, sc, &a, &d_na));
+    MS_TRY(values_of(r, sc, &b, &d_nb));
+    uint64_t *d_total = (uint64_t *)sc.get(64);
+    if (!d_total) return ms_fail(MS_ERR_NOMEM, "semi join scratch");
+    uint64_t *po = nullptr, *pi = nullptr, total = 0;
+    MS_TRY(launch_join(a, na, b, nb, true, &po, &pi, d_total, &total, sc));
+    if (!po) po = d_total;   // no left value: nothing is read
+    uint64_t *app;
+    return emit_new(*out_f, po, d_total, na, wit_chunk(*out_f), 0, sc, out, &app);
 }
 
 // ------------------------------------------------------------------ set ops

@@ -311,23 +311,76 @@ __global__ __launch_bounds__(WG) void k_delta_carries(const uint64_t *scan, uint
 // ------------------------------------------------------------------
 // Decompression kernels.
 // ------------------------------------------------------------------
-template <bool IS_FOR>
-__global__ __launch_bounds__(WG) void k_decomp_blocks(ms_colv c, uint64_t *out)
+// 8-lane indexed layouts (dynamic_vbp_f, for_f's inner blocks) and
+// static_vbp_f<vbp_l<bw, 8>> (LAYOUT 2: unit u's payload is 64 bw bytes at
+// u * 64 bw, no index): one resident wave of workgroups, grid-stride over
+// groups of 4 units; a group's index entries (and for_f references) are
+// loaded lane-parallel one group ahead, and its 4 units' words are all
+// requested before the first is decoded.
+enum { DL_BLOCKS = 0, DL_FOR = 1, DL_STATIC8 = 2 };
+
+template <int LAYOUT, bool NT>
+__global__ __launch_bounds__(WG) void k_decomp_units(ms_colv c, uint64_t *out)
+{
+    const ms_parts pt = parts_of(c);
+    const uint32_t lg_spb = LAYOUT == DL_STATIC8 ? 0 : c.g.lg_spb;
+    const uint32_t lg_ipo = c.g.lg_ipo;
+    const uint64_t n = pt.n_compr >> 9;
+    const uint64_t nq = (n + 3) >> 2;
+    const uint32_t t = threadIdx.x & 63;
+    const uint32_t j = t & 7, gl = t >> 3;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    const uint64_t mask = (1ull << lg_spb) - 1ull;
+    const uint64_t bw8 = c.g.bw;
+    uint64_t q = (uint64_t)blockIdx.x * WAVES + wave_index();
+    const auto entry = [&](uint64_t u, uint64_t &e, uint64_t &r) {
+        e = r = 0;
+        if (LAYOUT == DL_STATIC8) return;
+        e = gload(c.index + (u >> lg_spb));
+        if (LAYOUT == DL_FOR) r = gload(c.refs + ((u >> lg_spb) >> lg_ipo));
+    };
+    uint64_t my_e = 0, my_r = 0;
+    if (q < nq && t < 4 && q * 4 + t < n) entry(q * 4 + t, my_e, my_r);
+    for (; q < nq; q += W) {
+        const uint64_t u0 = q * 4;
+        const uint32_t cnt = n - u0 < 4 ? (uint32_t)(n - u0) : 4u;
+        uint64_t e[4], r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (LAYOUT == DL_STATIC8) {
+                e[u] = (((u0 + u) * 64ull * bw8) << 8) | bw8;
+                r[u] = 0;
+            } else {
+                e[u] = readlane64(my_e, u);
+                r[u] = LAYOUT == DL_FOR ? readlane64(my_r, u) : 0ull;
+            }
+        }
+        const uint64_t un = u0 + 4 * W + t;
+        my_e = my_r = 0;
+        if (q + W < nq && t < 4 && un < n) entry(un, my_e, my_r);
+        dec_units<NT>(c.data, e, r, cnt, u0, mask, j, gl, out);
+    }
+}
+
+// delta_f with 8 lanes over indexed 8-lane inner blocks: one wave per
+// 2048-value tile (lane-major threads, 8 per delta lane), the tile's index
+// entries requested one tile ahead.
+template <bool NT>
+__global__ __launch_bounds__(WG) void k_delta_decomp(ms_colv c, uint64_t *out)
 {
     const ms_parts pt = parts_of(c);
     const uint64_t nunits = pt.n_compr >> 9;
+    const uint64_t ntiles = (nunits + 3) >> 2;
     const uint32_t t = threadIdx.x & 63;
-    const uint32_t j = t & 7, gl = t >> 3;
-    const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
-    for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
-        const uint64_t b = u >> c.g.lg_spb, sub = u & ((1ull << c.g.lg_spb) - 1);
-        const uint64_t e = uniform64(c.index[b]);
-        const uint64_t ref = IS_FOR ? uniform64(c.refs[b >> c.g.lg_ipo]) : 0;
-        uint64_t d[8];
-        dec_dispatch((uint32_t)(e & 0xff), (const uint64_t *)(c.data + (e >> 8)), 8, j, sub * 8 + gl, d);
-        uint64_t *o = out + u * 512;
-#pragma unroll
-        for (int m = 0; m < 8; m++) o[8 * (8 * gl + m) + j] = d[m] + ref;
+    const uint32_t j = t >> 3, gl = t & 7;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    uint64_t tile = (uint64_t)blockIdx.x * WAVES + wave_index();
+    uint64_t my_e = tile < ntiles ? delta_tile_entries(c.index, c.g.lg_spb, tile, nunits, t) : 0ull;
+    for (; tile < ntiles; tile += W) {
+        const uint64_t e_next =
+            tile + W < ntiles ? delta_tile_entries(c.index, c.g.lg_spb, tile + W, nunits, t) : 0ull;
+        delta_tile_store<NT>(c.data, my_e, c.refs, c.g.lg_spb, tile, nunits, t, j, gl, out);
+        my_e = e_next;
     }
 }
 
@@ -369,30 +422,69 @@ __global__ __launch_bounds__(WG) void k_copy_rest(ms_colv c, uint64_t *out)
         out[pt.n_compr + i] = x[i];
 }
 
-// rle_f expansion: run starts are the exclusive prefix sums of the lengths;
-// each workgroup expands 2048 output positions.
-__global__ __launch_bounds__(WG) void k_rle_lengths(ms_colv c, uint64_t *len, uint64_t runs_bound)
+// rle_f expansion in chunks of RLE_XCH runs: the chunks' value counts
+// (k_rle_chunk_lens), their exclusive scan, then one wave per chunk
+// (k_rle_expand_chunks): 64 runs at a time are loaded lane-parallel, their
+// starts come from a wave scan of the lengths, and the wave writes each
+// run's copies with contiguous stores.
+static constexpr uint64_t RLE_XCH = 1024;
+
+__global__ __launch_bounds__(WG) void k_rle_chunk_lens(ms_colv c, uint64_t *clen, uint64_t nch_bound)
 {
     const uint64_t runs = c.meta->size_used / 16;
-    const uint64_t *w = (const uint64_t *)c.data;
-    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs_bound; r += (uint64_t)gridDim.x * WG)
-        len[r] = r < runs ? w[2 * r + 1] : 0;
+    const u64x2 *w = (const u64x2 *)c.data;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t ch = (uint64_t)blockIdx.x * WAVES + wave_index(); ch < nch_bound; ch += W) {
+        uint64_t acc = 0;
+        const uint64_t r0 = ch * RLE_XCH;
+        uint64_t len[RLE_XCH / 64];
+#pragma unroll
+        for (uint32_t i = 0; i < RLE_XCH / 64; i++) {
+            const uint64_t r = r0 + 64 * i + t;
+            len[i] = r < runs ? gload(w + r).y : 0ull;
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < RLE_XCH / 64; i++) acc += len[i];
+        acc = wave_sum_u64(acc);
+        if (t == 0) clen[ch] = acc;
+    }
 }
 
-__global__ __launch_bounds__(WG) void k_rle_expand(ms_colv c, const uint64_t *starts, uint64_t *out)
+template <bool NT>
+__global__ __launch_bounds__(WG) void k_rle_expand_chunks(ms_colv c, const uint64_t *cstart, uint64_t *out)
 {
     const uint64_t runs = c.meta->size_used / 16;
-    const uint64_t n = c.meta->count;
-    const uint64_t *w = (const uint64_t *)c.data;
-    for (uint64_t t0 = (uint64_t)blockIdx.x * 2048; t0 < n; t0 += (uint64_t)gridDim.x * 2048) {
-        for (uint64_t i = t0 + threadIdx.x; i < t0 + 2048 && i < n; i += WG) {
-            // last run with start <= i
-            uint64_t lo = 0, hi = runs;
-            while (hi - lo > 1) {
-                const uint64_t mid = (lo + hi) / 2;
-                if (starts[mid] <= i) lo = mid; else hi = mid;
+    const uint64_t n = c.meta->count;   // writes stay below it whatever the lengths say
+    const uint64_t nch = (runs + RLE_XCH - 1) / RLE_XCH;
+    const u64x2 *w = (const u64x2 *)c.data;
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t W = (uint64_t)gridDim.x * WAVES;
+    for (uint64_t ch = (uint64_t)blockIdx.x * WAVES + wave_index(); ch < nch; ch += W) {
+        uint64_t base = uniform64(gload(cstart + ch));
+        const uint64_t r0 = ch * RLE_XCH;
+        const uint64_t r1 = r0 + RLE_XCH < runs ? r0 + RLE_XCH : runs;
+        u64x2 nx = r0 + t < r1 ? gload(w + r0 + t) : u64x2{0, 0};
+        for (uint64_t g = r0; g < r1; g += 64) {
+            const u64x2 vl = nx;
+            nx = g + 64 + t < r1 ? gload(w + g + 64 + t) : u64x2{0, 0};
+            // inclusive scan of the 64 lengths
+            uint64_t incl = vl.y;
+#pragma unroll
+            for (uint32_t o = 1; o < 64; o <<= 1) {
+                const uint64_t y = __shfl_up(incl, o, MS_WAVE);
+                if (t >= o) incl += y;
             }
-            out[i] = w[2 * lo];
+            const uint64_t start = base + incl - vl.y;
+            base += readlane64(incl, 63);
+            const uint32_t nr = r1 - g < 64 ? (uint32_t)(r1 - g) : 64u;
+            for (uint32_t k = 0; k < nr; k++) {
+                const uint64_t v = readlane64(vl.x, k);
+                const uint64_t s0 = readlane64(start, k);
+                const uint64_t len = readlane64(vl.y, k);
+                const uint64_t e = s0 + len < n && s0 + len >= s0 ? s0 + len : n;
+                for (uint64_t i = s0 + t; i < e; i += 64) st64<NT>(out + i, v);
+            }
         }
     }
 }
@@ -661,6 +753,16 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
     return MS_OK;
 }
 
+// Decompressed values are written once, far more of them than any cache
+// holds: the unit and delta decoders store them non-temporally (measured:
+// dynamic_vbp 334 -> 247 us, delta 400 -> 232 us for 2^27 values with the
+// paired 16-byte stores).  rle expansion measured slower that way
+// (339 -> 421 us) and stores normally.
+#ifndef MS_DECOMP_NT
+#define MS_DECOMP_NT 1
+#endif
+static constexpr bool DECOMP_NT = MS_DECOMP_NT != 0;
+
 int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
 {
     ms_colv v;
@@ -668,30 +770,39 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
     const uint64_t nb = in->count_bound;
     if (nb == 0) return MS_OK;
     const uint32_t grid_units = grid_for(nb / 512 + 1, WAVES * 4ull, 8192);
+    const uint64_t quads = nb / 2048 + 1;   // groups of 4 units
     switch (v.g.kind) {
     case MS_UNCOMPR:
         break;
     case MS_RLE: {
         temps sc(s);
-        const uint64_t runs_bound = in->size_alloc / 16 + 1;
-        uint64_t *len = (uint64_t *)sc.get(runs_bound * 8);
-        uint64_t *st = (uint64_t *)sc.get(runs_bound * 8);
-        if (!len || !st) return ms_fail(MS_ERR_NOMEM, "rle scratch");
-        hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(runs_bound, WG * 4ull)), dim3(WG), 0, s, v, len, runs_bound);
-        MS_TRY(launch_exclusive_scan(len, st, runs_bound, sc));
-        hipLaunchKernelGGL(k_rle_expand, dim3(grid_for(nb, 2048, 8192)), dim3(WG), 0, s, v, st, d_out);
+        const uint64_t nch_bound = (in->size_alloc / 16 + 1) / RLE_XCH + 1;
+        uint64_t *clen = (uint64_t *)sc.get(nch_bound * 8);
+        uint64_t *cst = (uint64_t *)sc.get(nch_bound * 8);
+        if (!clen || !cst) return ms_fail(MS_ERR_NOMEM, "rle scratch");
+        hipLaunchKernelGGL(k_rle_chunk_lens, dim3(stride_grid((const void *)k_rle_chunk_lens, nch_bound)), dim3(WG), 0,
+                           s, v, clen, nch_bound);
+        MS_TRY(launch_exclusive_scan(clen, cst, nch_bound, sc));
+        auto k = k_rle_expand_chunks<false>;
+        hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nch_bound)), dim3(WG), 0, s, v, cst, d_out);
         MS_HIP(hipGetLastError());
         return MS_OK;
     }
     case MS_STATIC_VBP:
-        if (fast_static(v)) hipLaunchKernelGGL(static_dec_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, d_out);
-        else hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
+        if (fast_static(v) && v.g.lanes == 8) {
+            auto k = k_decomp_units<DL_STATIC8, DECOMP_NT>;
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out);
+        } else if (fast_static(v)) {
+            hipLaunchKernelGGL(static_dec_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, d_out);
+        } else {
+            hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
+        }
         break;
     case MS_DYNAMIC_VBP:
     case MS_FOR:
         if (fast_blocks(v)) {
-            if (v.g.kind == MS_FOR) hipLaunchKernelGGL(k_decomp_blocks<true>, dim3(grid_units), dim3(WG), 0, s, v, d_out);
-            else hipLaunchKernelGGL(k_decomp_blocks<false>, dim3(grid_units), dim3(WG), 0, s, v, d_out);
+            auto k = v.g.kind == MS_FOR ? k_decomp_units<DL_FOR, DECOMP_NT> : k_decomp_units<DL_BLOCKS, DECOMP_NT>;
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out);
         } else {
             hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
         }
@@ -700,6 +811,11 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         temps sc(s);
         const uint64_t *arr;
         MS_TRY(delta_src(in, v, sc, &arr));
+        if (!arr && v.g.lg_spb != 0xff) {
+            auto k = k_delta_decomp<DECOMP_NT>;
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nb / DELTA_TILE + 1)), dim3(WG), 0, s, v, d_out);
+            break;
+        }
         const dim3 grid(grid_for(nb / DELTA_TILE + 1, 1, 65536));
         if (arr) hipLaunchKernelGGL((k_delta_tiles<0, 1>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);
         else hipLaunchKernelGGL((k_delta_tiles<0, 0>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);

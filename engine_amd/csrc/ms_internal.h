@@ -174,4 +174,6 @@ struct ms_col {
     uint64_t count_bound;    // host-known upper bound of count
     hipStream_t stream;      // stream of creation (frees are ordered on it)
     struct ms_plan *arena;   // non-null: memory belongs to this plan's arena
+    uint64_t prepared_bytes; // prepared for random access: compressed rest bytes at sizeCompr
+    bool prepared;
 };

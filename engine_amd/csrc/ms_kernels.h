@@ -84,6 +84,14 @@ int launch_merge(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound, con
                  const uint64_t *d_nb, uint64_t nb_bound, uint64_t *d_out, uint64_t *d_cnt, temps &sc);
 int launch_calc(uint32_t op, const uint64_t *a, const uint64_t *b, const uint64_t *d_n, uint64_t n_bound,
                 uint64_t *d_out, hipStream_t s);
+// group ids in order of first appearance and the groups' first positions
+// of n values (gr: the ids of a previous grouping, or null)
+int launch_group(const uint64_t *data, const uint64_t *gr, uint64_t n, uint64_t *out_gr, uint64_t *out_ext,
+                 uint64_t *d_ngroups, temps &sc);
+// equal pairs of outer x inner positions in nested-loop order (semi: only
+// the outer positions with a match, into *out_outer); synchronises
+int launch_join(const uint64_t *outer, uint64_t na, const uint64_t *inner, uint64_t nb, bool semi,
+                uint64_t **out_outer, uint64_t **out_inner, uint64_t *d_total, uint64_t *total, temps &sc);
 // small device-side bookkeeping kernels
 int launch_copy_u64(uint64_t *dst, const uint64_t *src, hipStream_t s);
 int launch_set_meta(ms_dev_meta *m, const uint64_t *d_count, uint64_t size_per_value, uint64_t size_compr_is_used,

@@ -683,4 +683,213 @@ __device__ __forceinline__ void delta_tile_ss(const uint8_t *data, uint64_t my_e
     delta_units_ss<WIDE_OUT_OF_LINE>(data, e, cnt, u0, (1ull << lg_spb) - 1ull, p, t, j, gl, carry, acc);
 }
 
+// ------------------------------------------------------------------
+// Decompression of 512-value units of an 8-lane indexed layout (and of
+// delta tiles) into an uncompressed array.  As for select + sum, the words
+// of up to four units are requested before the first is decoded, so a wave
+// keeps four units' loads in flight; thread (gl, j) writes its 8 values to
+// out[u * 512 + 64 gl + 8 m + j].
+// ------------------------------------------------------------------
+template <bool NT>
+__device__ __forceinline__ void st64(uint64_t *p, uint64_t v)
+{
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Stores of a thread's 8 values v[m] to o[8 m]: with STORE16 the values
+// of two neighbouring lanes j, j + 1 (in threads t and t ^ XOR) are paired
+// through a DPP swap, so each thread writes 4 aligned 16-byte pairs (the
+// even lane the even m, the odd lane the odd m) and a wave's store
+// instruction covers whole 128-byte lines.
+#ifndef MS_DECOMP_STORE16
+#define MS_DECOMP_STORE16 1
+#endif
+
+template <int XOR>
+__device__ __forceinline__ uint64_t swap_pair(uint64_t x)
+{
+    // XOR 1: quad_perm [1,0,3,2]; XOR 8: row_ror:8 (halves of a 16-lane row)
+    constexpr int ctl = XOR == 1 ? 0xb1 : 0x128;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, ctl, 0xf, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), ctl, 0xf, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// o points at lane j's slot of row 0 (o = base + 64 gl + j); odd = j & 1.
+template <bool NT, int XOR>
+__device__ __forceinline__ void store8(uint64_t *o, const uint64_t (&v)[8], uint32_t odd)
+{
+    if (!MS_DECOMP_STORE16) {
+#pragma unroll
+        for (int m = 0; m < 8; m++) st64<NT>(o + 8 * m, v[m]);
+        return;
+    }
+    u64x2 *p = (u64x2 *)(o - odd);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint64_t r = swap_pair<XOR>(odd ? v[2 * k] : v[2 * k + 1]);
+        u64x2 pr;
+        pr.x = odd ? r : v[2 * k];
+        pr.y = odd ? v[2 * k + 1] : r;
+        u64x2 *q = (u64x2 *)((uint64_t *)p + 8 * (2 * k + odd));
+        if (NT) __builtin_nontemporal_store(pr, q);
+        else *q = pr;
+    }
+}
+
+__device__ __forceinline__ void cut32_dispatch(uint32_t bw, const uint64_t (&w)[5], uint32_t g, uint32_t (&c)[8])
+{
+    switch (bw) {
+#define MS_CASE(B) \
+    case B: vbp_cut32<B>(w, ((g * B) & 7) * 8, c); break;
+        MS_BW32_CASES(MS_CASE)
+#undef MS_CASE
+    default:
+#pragma unroll
+        for (int m = 0; m < 8; m++) c[m] = 0;
+        break;
+    }
+}
+
+// Codes of more than 32 bits: rare, out of line (keeps the callers'
+// register budget).
+template <bool NT>
+__device__ __attribute__((noinline)) void dec_wide_store(const uint64_t *base, uint32_t bw, uint32_t j, uint32_t g,
+                                                         uint64_t ref, uint64_t *o)
+{
+    uint64_t d[8];
+    dec_dispatch(bw, base, 8, j, g, d);
+#pragma unroll
+    for (int m = 0; m < 8; m++) d[m] += ref;
+    store8<NT, 1>(o, d, j & 1);
+}
+
+__device__ __forceinline__ void dec_unit_prep(const uint8_t *data, uint64_t e, bool valid, uint64_t u, uint64_t mask,
+                                              uint32_t j, uint32_t gl, uint64_t (&w)[5])
+{
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    if (!valid || bw > 32) return;
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));   // keeps per-width offsets of g from being hoisted
+    unit_words_dispatch(bw, (const uint64_t *)(data + (e >> 8)), j, g, w);
+}
+
+template <bool NT>
+__device__ __forceinline__ void dec_unit_finish(const uint8_t *data, uint64_t e, uint64_t ref, bool valid, uint64_t u,
+                                                uint64_t mask, uint32_t j, uint32_t gl, const uint64_t (&w)[5],
+                                                uint64_t *out)
+{
+    if (!valid) return;
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));
+    uint64_t *o = out + u * 512 + 64 * gl + j;
+    if (bw <= 32) {
+        uint32_t c[8];
+        cut32_dispatch(bw, w, g, c);
+        uint64_t v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = (uint64_t)c[m] + ref;
+        store8<NT, 1>(o, v, j & 1);
+    } else {
+        dec_wide_store<NT>((const uint64_t *)(data + (e >> 8)), bw, j, g, ref, o);
+    }
+}
+
+// Units u0 .. u0+cnt-1 (cnt <= 4); e[u] = their index entries, r[u] their
+// reference values (0 but for for_f).
+template <bool NT>
+__device__ __forceinline__ void dec_units(const uint8_t *data, const uint64_t (&e)[4], const uint64_t (&r)[4],
+                                          uint32_t cnt, uint64_t u0, uint64_t mask, uint32_t j, uint32_t gl,
+                                          uint64_t *out)
+{
+    uint64_t w0[5], w1[5], w2[5], w3[5];   // named arrays: they stay in registers
+    dec_unit_prep(data, e[0], cnt > 0, u0, mask, j, gl, w0);
+    dec_unit_prep(data, e[1], cnt > 1, u0 + 1, mask, j, gl, w1);
+    dec_unit_prep(data, e[2], cnt > 2, u0 + 2, mask, j, gl, w2);
+    dec_unit_prep(data, e[3], cnt > 3, u0 + 3, mask, j, gl, w3);
+    dec_unit_finish<NT>(data, e[0], r[0], cnt > 0, u0, mask, j, gl, w0, out);
+    dec_unit_finish<NT>(data, e[1], r[1], cnt > 1, u0 + 1, mask, j, gl, w1, out);
+    dec_unit_finish<NT>(data, e[2], r[2], cnt > 2, u0 + 2, mask, j, gl, w2, out);
+    dec_unit_finish<NT>(data, e[3], r[3], cnt > 3, u0 + 3, mask, j, gl, w3, out);
+}
+
+// delta_f: thread (j = t / 8, gl = t % 8) holds 8 deltas (<= 26 bits) of
+// lane j; writes lane j's running sums from `carry` and advances it.
+template <bool NT>
+__device__ __forceinline__ void delta_codes_store(uint32_t (&d)[8], uint32_t t, uint64_t &carry, uint64_t *o)
+{
+#pragma unroll
+    for (int m = 1; m < 8; m++) d[m] += d[m - 1];
+    const uint32_t tot = d[7];
+    const uint32_t incl = group8_scan(tot, t);
+    const uint64_t bt = carry + (incl - tot);
+    uint64_t v[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) v[m] = bt + d[m];
+    store8<NT, 8>(o, v, (t >> 3) & 1);
+    carry += group8_sum(tot);
+}
+
+template <bool NT>
+__device__ __attribute__((noinline)) uint64_t delta_wide_store(const uint64_t *base, uint32_t bw, uint32_t j,
+                                                               uint32_t g, uint32_t t, uint64_t carry, uint64_t *o)
+{
+    uint64_t d[8];
+    dec_dispatch(bw, base, 8, j, g, d);
+#pragma unroll
+    for (int m = 1; m < 8; m++) d[m] += d[m - 1];
+    const uint64_t tot = d[7];
+    const uint64_t incl = group8_scan64(tot, t);
+    const uint64_t bt = carry + incl - tot;
+#pragma unroll
+    for (int m = 0; m < 8; m++) d[m] += bt;
+    store8<NT, 8>(o, d, (t >> 3) & 1);
+    return carry + __shfl(incl, (int)((t & ~7u) + 7), MS_WAVE);
+}
+
+template <bool NT>
+__device__ __forceinline__ void delta_unit_store(const uint8_t *data, uint64_t e, bool valid, uint64_t u,
+                                                 uint64_t mask, uint32_t t, uint32_t j, uint32_t gl,
+                                                 const uint64_t (&w)[5], uint64_t &carry, uint64_t *out)
+{
+    if (!valid) return;
+    const uint32_t bw = (uint32_t)(e & 0xff);
+    uint32_t g = (uint32_t)(u & mask) * 8 + gl;
+    asm volatile("" : "+v"(g));
+    uint64_t *o = out + u * 512 + 64 * gl + j;
+    if (bw <= 26) {
+        uint32_t d[8];
+        dec32_cut_dispatch(bw, w, g, d);
+        delta_codes_store<NT>(d, t, carry, o);
+    } else {
+        carry = delta_wide_store<NT>((const uint64_t *)(data + (e >> 8)), bw, j, g, t, carry, o);
+    }
+}
+
+// A delta tile (4 units) into out: entries in lanes 0..3 of my_e.
+template <bool NT>
+__device__ __forceinline__ void delta_tile_store(const uint8_t *data, uint64_t my_e, const uint64_t *refs,
+                                                 uint32_t lg_spb, uint64_t tile, uint64_t nunits, uint32_t t,
+                                                 uint32_t j, uint32_t gl, uint64_t *out)
+{
+    uint64_t carry = gload(refs + tile * 8 + j);
+    const uint64_t u0 = tile * 4;
+    const uint32_t cnt = nunits - u0 < 4 ? (uint32_t)(nunits - u0) : 4u;
+    const uint64_t mask = (1ull << lg_spb) - 1ull;
+    uint64_t e[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) e[u] = readlane64(my_e, u);
+    uint64_t w0[5], w1[5], w2[5], w3[5];
+    delta_unit_prep(data, e[0], cnt > 0, u0, mask, j, gl, w0);
+    delta_unit_prep(data, e[1], cnt > 1, u0 + 1, mask, j, gl, w1);
+    delta_unit_prep(data, e[2], cnt > 2, u0 + 2, mask, j, gl, w2);
+    delta_unit_prep(data, e[3], cnt > 3, u0 + 3, mask, j, gl, w3);
+    delta_unit_store<NT>(data, e[0], cnt > 0, u0, mask, t, j, gl, w0, carry, out);
+    delta_unit_store<NT>(data, e[1], cnt > 1, u0 + 1, mask, t, j, gl, w1, carry, out);
+    delta_unit_store<NT>(data, e[2], cnt > 2, u0 + 2, mask, t, j, gl, w2, carry, out);
+    delta_unit_store<NT>(data, e[3], cnt > 3, u0 + 3, mask, t, j, gl, w3, carry, out);
+}
+
 }  // namespace ms

@@ -168,12 +168,35 @@ int ms_col_download(const ms_col *c, void *image, uint64_t capacity,
 /* The column's format descriptor. */
 int ms_col_format(const ms_col *c, ms_format_desc *f);
 
+/* column<F>::prepare_for_random_access (column.h:241-268): the
+ * uncompressed rest, zero-padded to a multiple of the block size, is
+ * compressed by one morph_batch call (fresh pages / references / delta
+ * carry) to the end of the compressed part, inside the gap; the meta data
+ * are unchanged and the downloaded image then shows those bytes.  *acted
+ * (may be NULL) = 1 if the column changed: not for uncompr_f, an empty rest
+ * or a second call.  rle_f (no block size) is refused.  Synchronises
+ * `stream`. */
+int ms_col_prepare_for_random_access(ms_col *c, int *acted, void *stream);
+
 /* Device address of the byte image (64-byte aligned). */
 const void *ms_col_device_data(const ms_col *c);
 
 /* An upper bound of the column's count that is known without a device
  * round trip (exact for uploaded columns). */
 uint64_t ms_col_count_bound(const ms_col *c);
+
+/* binary_io<F>::store (binary_io.h:70-85): writes count and sizeUsed as
+ * two little-endian uint64, then the sizeUsed bytes of the image (the gap
+ * reads as zeros).  Synchronises `stream`. */
+int ms_col_store(const ms_col *c, const char *path, void *stream);
+
+/* binary_io<F>::load (binary_io.h:43-68): a column of format f from such a
+ * file.  The file holds no sizeCompr (the reference's loaded column keeps
+ * 0); the loader derives it as a column-level morph of `count` values lays
+ * the image out (floor(count / block) values compressed, the rest raw
+ * behind the gap; rle_f: the whole image) and refuses (MS_ERR_INVALID) an
+ * image of another size.  Synchronises `stream`. */
+int ms_col_load(ms_col **out, const ms_format_desc *f, const char *path, void *stream);
 
 /* Releases the column, stream ordered on the stream it was created on:
  * that stream must still exist (destroy a stream only after freeing the
@@ -205,6 +228,16 @@ int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in,
 int ms_select(ms_col **out, uint32_t cmp, uint64_t val,
               const ms_format_desc *out_f, const ms_col *in, uint32_t lanes,
               void *stream);
+
+/* my_between_wit_t<cmp_lower, cmp_upper, ve, out_f, in_f>::apply(in,
+ * val_lower, val_upper) (between_compr.h:107-270, interface between.h):
+ * ascending positions i with cmp_lower(x_i, val_lower) and
+ * cmp_upper(x_i, val_upper), written and laid out as ms_select does.  The
+ * comparisons are unsigned; NE is refused (the two would not form one
+ * interval). */
+int ms_between(ms_col **out, uint32_t cmp_lower, uint64_t val_lower,
+               uint32_t cmp_upper, uint64_t val_upper, const ms_format_desc *out_f,
+               const ms_col *in, uint32_t lanes, void *stream);
 
 /* my_project_wit_t<ve, out_f, data_f, pos_f>::apply(data, pos):
  * out_i = data[pos_i].  Any data format is accepted (the reference needs
@@ -245,6 +278,31 @@ int ms_intersect_sorted(ms_col **out, const ms_format_desc *out_f,
  * columns (an equal pair is emitted once, scalar/merge_uncompr.h:69-96). */
 int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f,
                     const ms_col *l, const ms_col *r, void *stream);
+
+/* group<ve, gr_f, ext_f, [in_gr_f,] data_f>([in_gr,] data) (interfaces/
+ * group.h:38-99, scalar/group_uncompr.h:44-113): *out_gr holds a group id
+ * per value, numbered in order of first appearance of the value (in_gr
+ * NULL: unary group) or of the pair (in_gr_i, data_i) (binary group);
+ * *out_ext holds each group's first position.  A binary group of inputs
+ * of different counts is MS_ERR_SIZE.  Synchronises `stream`. */
+int ms_group(ms_col **out_gr, ms_col **out_ext, const ms_format_desc *gr_f,
+             const ms_format_desc *ext_f, const ms_col *in_gr,
+             const ms_col *in_data, void *stream);
+
+/* nested_loop_join<ve, l_f, r_f>(l, r) (interfaces/join.h:38-75,
+ * scalar/join_uncompr.h:44-96): the positions of every equal pair, in
+ * the reference's nested-loop order: the outer loop runs over the longer
+ * input (the left one when both are equally long), both ascending.
+ * Synchronises `stream` (the output size is read back). */
+int ms_join(ms_col **out_l, ms_col **out_r, const ms_format_desc *l_f,
+            const ms_format_desc *r_f, const ms_col *l, const ms_col *r,
+            void *stream);
+
+/* left_semi_nto1_nested_loop_join<ve, out_f>(l, r) (interfaces/join.h:
+ * 77-110, scalar/join_uncompr.h:98-128): ascending left positions whose
+ * value occurs in r.  Synchronises `stream`. */
+int ms_semi_join(ms_col **out, const ms_format_desc *out_f, const ms_col *l,
+                 const ms_col *r, void *stream);
 
 /* calc_binary<op, ve, out_f, l_f, r_f>(l, r): out_i = op(l_i, r_i) mod 2^64.
  * Counts must match (MS_ERR_SIZE otherwise, scalar/calc_uncompr.h:84-88). */

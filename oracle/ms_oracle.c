@@ -573,6 +573,32 @@ int oc_morph_decompress(const oc_col *in, uint64_t *out)
     return MS_OK;
 }
 
+/* column.h:241-268: the uncompressed rest, zero-padded to a multiple of
+ * the block size, is compressed by one morph_batch call (fresh state: a
+ * new dynamic_vbp page, for_f reference page, zero delta carry) to the end
+ * of the compressed part, inside the gap; the meta data stay as they are.
+ * No action for uncompr_f or without a rest (rle_f has no block size and
+ * is refused). */
+int oc_prepare_for_random_access(oc_col *c, int *acted)
+{
+    uint64_t rest, bs, ru, r, b;
+    *acted = 0;
+    if (c->f.kind == MS_RLE) return MS_ERR_INVALID;
+    if (c->f.kind == MS_UNCOMPR) return MS_OK;
+    rest = oc_count_values_uncompr(c);
+    if (!rest) return MS_OK;
+    oc_block_size(&c->f, &bs);
+    ru = round_up(rest, bs);
+    oc_rest_offset(&c->f, c->size_compr, &r);
+    if (r + 8 * ru > c->size_alloc) return MS_ERR_CAPACITY;
+    memset(c->data + r + 8 * rest, 0, 8 * (ru - rest));
+    memset(c->data + c->size_compr, 0, r - c->size_compr);       /* packing ORs into zeroed words */
+    oc_morph_batch_compress(&c->f, (const uint64_t *)(c->data + r), ru, c->data + c->size_compr, &b);
+    if (c->size_compr + b > r) return MS_ERR_CAPACITY;
+    *acted = 1;
+    return MS_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* Write-iterator output (write_iterator.h:66-196; delta.h:369-490 for */
 /* the delta carry; uncompr.h:125-221 for the unbuffered uncompr one)  */
@@ -680,6 +706,158 @@ int oc_select(oc_col *out, uint32_t cmp, uint64_t val, const ms_format_desc *out
     free(x);
     free(pos);
     return rc;
+}
+
+/* between_compr.h:107-270: select with two comparisons, same write-
+ * iterator output and scalar-remainder append as select. */
+int oc_between(oc_col *out, uint32_t cmp_lo, uint64_t val_lo, uint32_t cmp_hi, uint64_t val_hi,
+               const ms_format_desc *out_f, const oc_col *in, uint32_t lanes)
+{
+    uint64_t *x, *pos, na = 0, nb = 0, app;
+    uint64_t n = in->count, rest = oc_count_values_uncompr(in), nvec;
+    int has_rest = in->size_compr != in->size_used, rc;
+    if (cmp_lo > MS_CMP_GE || cmp_hi > MS_CMP_GE || lanes == 0 || lanes > 64) return MS_ERR_INVALID;
+    rc = decode_all(in, &x);
+    if (rc) { free(x); return rc; }
+    pos = (uint64_t *)malloc(8 * (n + 1));
+    nvec = has_rest ? (n - rest) + round_down(rest, lanes) : n;
+    for (uint64_t i = 0; i < nvec; i++)
+        if (cmp_apply(cmp_lo, x[i], val_lo) && cmp_apply(cmp_hi, x[i], val_hi)) pos[na++] = i;
+    rc = oc_wit_emit(out, out_f, pos, na, 8 * lanes + 64, &app);
+    if (!rc && has_rest && rest % lanes) {
+        uint64_t *dst = (uint64_t *)(out->data + app);
+        for (uint64_t i = nvec; i < n; i++)
+            if (cmp_apply(cmp_lo, x[i], val_lo) && cmp_apply(cmp_hi, x[i], val_hi)) dst[nb++] = i;
+        out->size_used = app + 8 * nb;
+        out->count = na + nb;
+    }
+    free(x);
+    free(pos);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* group and join (scalar/group_uncompr.h:44-113,                       */
+/* scalar/join_uncompr.h:44-128): the reference's scalar algorithms,    */
+/* a hash map for group ids in order of first appearance and nested     */
+/* loops for the joins.                                                 */
+/* ------------------------------------------------------------------ */
+typedef struct gmap {
+    uint64_t *ka, *kb, *id;   /* keys (ka, kb) -> id + 1 (0: empty) */
+    uint64_t cap;
+} gmap;
+
+static uint64_t mix64(uint64_t a, uint64_t b)
+{
+    uint64_t z = a * 0x9E3779B97F4A7C15ull ^ (b + 0x632BE59BD9B4E019ull + (a << 6) + (a >> 2));
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* groupIds[key]: the slot of key, inserted with id 0 when new */
+static uint64_t *gmap_slot(gmap *m, uint64_t a, uint64_t b)
+{
+    uint64_t h = mix64(a, b) & (m->cap - 1);
+    while (m->id[h] && (m->ka[h] != a || m->kb[h] != b)) h = (h + 1) & (m->cap - 1);
+    if (!m->id[h]) { m->ka[h] = a; m->kb[h] = b; }
+    return &m->id[h];
+}
+
+int oc_group(oc_col *out_gr, oc_col *out_ext, const oc_col *in_gr, const oc_col *in_data)
+{
+    uint64_t *x, *g = NULL, n = in_data->count, next = 0, next_ext = 0;
+    ms_format_desc u;
+    gmap m;
+    int rc;
+    if (in_gr && in_gr->count != n) return MS_ERR_SIZE;   /* group_uncompr.h:52-57 */
+    rc = decode_all(in_data, &x);
+    if (rc) { free(x); return rc; }
+    if (in_gr && (rc = decode_all(in_gr, &g))) { free(x); free(g); return rc; }
+    memset(&u, 0, sizeof u);
+    u.kind = MS_UNCOMPR;
+    m.cap = 16;
+    while (m.cap < 2 * n + 16) m.cap <<= 1;
+    m.ka = (uint64_t *)calloc(m.cap, 8);
+    m.kb = (uint64_t *)calloc(m.cap, 8);
+    m.id = (uint64_t *)calloc(m.cap, 8);
+    if (col_alloc(out_gr, &u, 8 * n + 8) || col_alloc(out_ext, &u, 8 * n + 8)) {
+        rc = MS_ERR_NOMEM;
+    } else {
+        uint64_t *og = (uint64_t *)out_gr->data, *oe = (uint64_t *)out_ext->data;
+        for (uint64_t i = 0; i < n; i++) {
+            uint64_t *id = gmap_slot(&m, g ? g[i] : 0, x[i]);
+            if (!*id) {                 /* the key was not found */
+                *id = ++next;
+                oe[next_ext++] = i;
+            }
+            og[i] = *id - 1;
+        }
+        out_gr->count = n;
+        out_gr->size_used = 8 * n;
+        out_ext->count = next_ext;
+        out_ext->size_used = 8 * next_ext;
+    }
+    free(m.ka); free(m.kb); free(m.id); free(x); free(g);
+    return rc;
+}
+
+/* nested_loop_join: every equal pair, the outer loop over the larger input
+ * (the left one when both are equally long), positions ascending */
+int oc_join(oc_col *out_l, oc_col *out_r, const oc_col *l, const oc_col *r)
+{
+    uint64_t *a, *b, na = l->count, nb = r->count, k = 0;
+    ms_format_desc u;
+    int rc, swap = na < nb;
+    rc = decode_all(l, &a);
+    if (rc) { free(a); return rc; }
+    rc = decode_all(r, &b);
+    if (rc) { free(a); free(b); return rc; }
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < na; i++) for (uint64_t j = 0; j < nb; j++) total += a[i] == b[j];
+    memset(&u, 0, sizeof u);
+    u.kind = MS_UNCOMPR;
+    if (col_alloc(out_l, &u, 8 * total + 8) || col_alloc(out_r, &u, 8 * total + 8)) {
+        free(a); free(b);
+        return MS_ERR_NOMEM;
+    }
+    uint64_t *ol = (uint64_t *)out_l->data, *orr = (uint64_t *)out_r->data;
+    if (!swap) {
+        for (uint64_t i = 0; i < na; i++)
+            for (uint64_t j = 0; j < nb; j++)
+                if (a[i] == b[j]) { ol[k] = i; orr[k] = j; k++; }
+    } else {                               /* join(r, l) with the outputs swapped back */
+        for (uint64_t j = 0; j < nb; j++)
+            for (uint64_t i = 0; i < na; i++)
+                if (a[i] == b[j]) { ol[k] = i; orr[k] = j; k++; }
+    }
+    out_l->count = out_r->count = k;
+    out_l->size_used = out_r->size_used = 8 * k;
+    free(a); free(b);
+    return MS_OK;
+}
+
+/* left_semi_nto1_nested_loop_join: ascending left positions whose value
+ * occurs in the right input */
+int oc_semi_join(oc_col *out, const oc_col *l, const oc_col *r)
+{
+    uint64_t *a, *b, na = l->count, nb = r->count, k = 0;
+    ms_format_desc u;
+    int rc = decode_all(l, &a);
+    if (rc) { free(a); return rc; }
+    rc = decode_all(r, &b);
+    if (rc) { free(a); free(b); return rc; }
+    memset(&u, 0, sizeof u);
+    u.kind = MS_UNCOMPR;
+    if (col_alloc(out, &u, 8 * na + 8)) { free(a); free(b); return MS_ERR_NOMEM; }
+    uint64_t *o = (uint64_t *)out->data;
+    for (uint64_t i = 0; i < na; i++)
+        for (uint64_t j = 0; j < nb; j++)
+            if (a[i] == b[j]) { o[k++] = i; break; }
+    out->count = k;
+    out->size_used = 8 * k;
+    free(a); free(b);
+    return MS_OK;
 }
 
 /* project_compr.h:80-261 (random access replaced by decompression) */

@@ -57,6 +57,9 @@ int oc_morph_compress(oc_col *out, const ms_format_desc *dst,
                       const uint64_t *in, uint64_t n);
 int oc_morph_decompress(const oc_col *in, uint64_t *out);
 uint64_t oc_count_values_uncompr(const oc_col *c);
+/* column<F>::prepare_for_random_access (column.h:241-268); *acted = 1 if
+ * the column changed. */
+int oc_prepare_for_random_access(oc_col *c, int *acted);
 void oc_col_free(oc_col *c);
 
 /* Write-iterator output of m values in format f (write_iterator.h:66-306,
@@ -68,6 +71,8 @@ int oc_wit_emit(oc_col *out, const ms_format_desc *f, const uint64_t *v,
 /* Operators. */
 int oc_select(oc_col *out, uint32_t cmp, uint64_t val,
               const ms_format_desc *out_f, const oc_col *in, uint32_t lanes);
+int oc_between(oc_col *out, uint32_t cmp_lo, uint64_t val_lo, uint32_t cmp_hi, uint64_t val_hi,
+               const ms_format_desc *out_f, const oc_col *in, uint32_t lanes);
 int oc_project(oc_col *out, const ms_format_desc *out_f, const oc_col *data,
                const oc_col *pos, uint32_t lanes);
 int oc_agg_sum(oc_col *out, const oc_col *in);
@@ -75,6 +80,11 @@ int oc_intersect_sorted(oc_col *out, const ms_format_desc *out_f,
                         const oc_col *l, const oc_col *r);
 int oc_merge_sorted(oc_col *out, const ms_format_desc *out_f,
                     const oc_col *l, const oc_col *r);
+/* scalar/group_uncompr.h:44-113 (in_gr NULL: unary group), scalar/
+ * join_uncompr.h:44-128 (nested loops; semi join); uncompressed outputs. */
+int oc_group(oc_col *out_gr, oc_col *out_ext, const oc_col *in_gr, const oc_col *in_data);
+int oc_join(oc_col *out_l, oc_col *out_r, const oc_col *l, const oc_col *r);
+int oc_semi_join(oc_col *out, const oc_col *l, const oc_col *r);
 int oc_calc_binary(oc_col *out, uint32_t op, const ms_format_desc *out_f,
                    const oc_col *l, const oc_col *r);
 
@@ -84,6 +94,14 @@ int oc_calc_binary(oc_col *out, uint32_t op, const ms_format_desc *out_f,
 int oc_select_sum(uint64_t *result, uint32_t cmp, uint64_t val, const oc_col *in);
 int oc_select_sum_stream(uint64_t *result, uint32_t cmp, uint64_t val,
                          const oc_col *in, int threads);
+
+/* The CPU baseline (ms_cpu.c): streaming select + sum over dynamic_vbp,
+ * delta+dynamic_vbp, for+dynamic_vbp (8 lanes), rle and uncompr columns
+ * with `threads` threads; simd 0 scalar, 1 AVX-512 (MS_ERR_INVALID where
+ * the CPU lacks it or the layout is not streamed). */
+int oc_cpu_has_avx512(void);
+int oc_cpu_select_sum(uint64_t *result, uint32_t cmp, uint64_t val, const oc_col *in,
+                      int threads, int simd);
 
 #ifdef __cplusplus
 }

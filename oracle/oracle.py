@@ -54,12 +54,19 @@ def lib():
             "oc_wit_emit": (ctypes.c_int, [C, FD, ctypes.c_void_p, U64, U64, P(U64)]),
             "oc_select": (ctypes.c_int, [C, U32, U64, FD, C, U32]),
             "oc_project": (ctypes.c_int, [C, FD, C, C, U32]),
+            "oc_between": (ctypes.c_int, [C, U32, U64, U32, U64, FD, C, U32]),
+            "oc_group": (ctypes.c_int, [C, C, C, C]),
+            "oc_join": (ctypes.c_int, [C, C, C, C]),
+            "oc_semi_join": (ctypes.c_int, [C, C, C]),
             "oc_agg_sum": (ctypes.c_int, [C, C]),
             "oc_intersect_sorted": (ctypes.c_int, [C, FD, C, C]),
             "oc_merge_sorted": (ctypes.c_int, [C, FD, C, C]),
             "oc_calc_binary": (ctypes.c_int, [C, U32, FD, C, C]),
             "oc_select_sum": (ctypes.c_int, [P(U64), U32, U64, C]),
             "oc_select_sum_stream": (ctypes.c_int, [P(U64), U32, U64, C, ctypes.c_int]),
+            "oc_cpu_has_avx512": (ctypes.c_int, []),
+            "oc_prepare_for_random_access": (ctypes.c_int, [C, P(ctypes.c_int)]),
+            "oc_cpu_select_sum": (ctypes.c_int, [P(U64), U32, U64, C, ctypes.c_int, ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -120,6 +127,17 @@ class Column:
 
     def count_uncompr(self):
         return lib().oc_count_values_uncompr(ctypes.byref(self.c))
+
+    def prepare_for_random_access(self):
+        """column<F>::prepare_for_random_access; True if the column changed."""
+        acted = ctypes.c_int()
+        _check(lib().oc_prepare_for_random_access(ctypes.byref(self.c), ctypes.byref(acted)),
+               "prepare_for_random_access")
+        return bool(acted.value)
+
+    def raw(self, nbytes):
+        """The first nbytes of the buffer (past sizeUsed too)."""
+        return np.ctypeslib.as_array(self.c.data, shape=(nbytes,)).copy()
 
 
 def _u64(a):
@@ -186,6 +204,33 @@ def select(cmp, val, out_f, col, lanes):
     return c
 
 
+def between(cmp_lo, val_lo, cmp_hi, val_hi, out_f, col, lanes):
+    c = Column()
+    _check(lib().oc_between(ctypes.byref(c.c), cmp_lo, val_lo, cmp_hi, val_hi, ctypes.byref(out_f),
+                            ctypes.byref(col.c), lanes), "between")
+    return c
+
+
+def group(data, in_gr=None):
+    """(group ids, extents) as uncompressed columns."""
+    g, e = Column(), Column()
+    _check(lib().oc_group(ctypes.byref(g.c), ctypes.byref(e.c), ctypes.byref(in_gr.c) if in_gr else None,
+                          ctypes.byref(data.c)), "group")
+    return g, e
+
+
+def join(l, r):
+    a, b = Column(), Column()
+    _check(lib().oc_join(ctypes.byref(a.c), ctypes.byref(b.c), ctypes.byref(l.c), ctypes.byref(r.c)), "join")
+    return a, b
+
+
+def semi_join(l, r):
+    c = Column()
+    _check(lib().oc_semi_join(ctypes.byref(c.c), ctypes.byref(l.c), ctypes.byref(r.c)), "semi_join")
+    return c
+
+
 def project(out_f, data, pos, lanes):
     c = Column()
     _check(lib().oc_project(ctypes.byref(c.c), ctypes.byref(out_f), ctypes.byref(data.c),
@@ -231,6 +276,18 @@ def select_sum_stream(cmp, val, col, threads=1):
     r = ctypes.c_uint64()
     _check(lib().oc_select_sum_stream(ctypes.byref(r), cmp, val, ctypes.byref(col.c), threads),
            "select_sum_stream")
+    return r.value
+
+
+def cpu_has_avx512():
+    return bool(lib().oc_cpu_has_avx512())
+
+
+def cpu_select_sum(cmp, val, col, threads=1, simd=0):
+    """The CPU baseline's streaming select + sum (simd 0 scalar, 1 AVX-512)."""
+    r = ctypes.c_uint64()
+    _check(lib().oc_cpu_select_sum(ctypes.byref(r), cmp, val, ctypes.byref(col.c), threads, simd),
+           "cpu_select_sum")
     return r.value
 
 

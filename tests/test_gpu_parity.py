@@ -330,3 +330,64 @@ def test_plan_graph_replays_match_eager(oracle):
         assert other == rm.masked_sum(ms, MS_CMP_LT, 100)
     del keep, res
     plan.free()
+
+
+PRA_FORMATS = [static_vbp_f(12, 1), static_vbp_f(7, 8), static_vbp_f(40, 8), dynamic_vbp_f(512, 2048, 8),
+               default_dynamic_vbp_f(8), default_delta_dynamic_vbp_f(8), default_for_dynamic_vbp_f(8),
+               delta_f(1024, 8, static_vbp_f(40, 8)), for_f(1024, 8, uncompr_f())]
+
+
+@pytest.mark.parametrize("f", PRA_FORMATS, ids=repr)
+@pytest.mark.parametrize("n", [0, 5, 4096, 4096 + 77, 100_000 + 333])
+def test_prepare_for_random_access(oracle, f, n):
+    """column<F>::prepare_for_random_access on the device: the image
+    (including the compressed rest it puts into the gap) equals the
+    oracle's after the same call; a second call, uncompr_f and a column
+    without a rest do nothing; values, sums and gathers are unchanged."""
+    x = np.ascontiguousarray(data_for(f, n, 90 + n % 11), dtype=np.uint64)
+    g = ea.morph(f, ea.upload_values(x))
+    o = oracle.morph(f, x)
+    want = o.prepare_for_random_access()
+    assert g.prepare_for_random_access() == want
+    assert_same_column(g, o)
+    assert not g.prepare_for_random_access()
+    assert np.array_equal(g.values(), x)
+    if n:
+        val = int(np.median(x))
+        assert int(ea.select_sum(MS_CMP_LT, val, g).get()[0]) == rm.masked_sum(x, MS_CMP_LT, val)
+        pos = np.array([0, n - 1, n // 2, max(0, n - 3)], dtype=np.uint64)
+        got = ea.project(uncompr_f(), g, ea.upload_values(pos)).values()
+        assert np.array_equal(got, x[pos.astype(np.int64)])
+    # a same-format morph keeps the prepared image
+    assert_same_column(ea.morph(f, g), o)
+
+
+def test_prepare_for_random_access_uncompr_and_rle():
+    u = ea.upload_values(np.arange(10, dtype=np.uint64))
+    assert not u.prepare_for_random_access()
+    r = ea.morph(rle_f(), u)
+    with pytest.raises(ea.MsError):
+        r.prepare_for_random_access()
+
+
+@pytest.mark.parametrize("in_f", [uncompr_f(), static_vbp_f(12, 1), dynamic_vbp_f(512, 2048, 8),
+                                  default_for_dynamic_vbp_f(8), default_delta_dynamic_vbp_f(8)], ids=repr)
+@pytest.mark.parametrize("out_f", [uncompr_f(), default_delta_dynamic_vbp_f(8), dynamic_vbp_f(512, 2048, 8)],
+                         ids=repr)
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_between_parity(oracle, in_f, out_f, lanes):
+    """between (two comparisons) on the device: byte images equal the
+    oracle's for interval, empty and one-sided ranges."""
+    n = 30_000 + 77
+    x = np.ascontiguousarray(data_for(in_f, n, 33), dtype=np.uint64)
+    g = ea.morph(in_f, ea.upload_values(x)) if in_f.kind else ea.upload_values(x)
+    o = oracle.morph(in_f, x) if in_f.kind else oracle.from_values(x)
+    q = np.quantile(x.astype(np.float64), [0.2, 0.7]).astype(np.uint64)
+    for lo_cmp, lo, hi_cmp, hi in [(MS_CMP_GE, int(q[0]), MS_CMP_LE, int(q[1])),
+                                   (MS_CMP_GT, int(q[0]), MS_CMP_LT, int(q[1])),
+                                   (MS_CMP_GE, int(q[1]), MS_CMP_LE, int(q[0])),
+                                   (MS_CMP_EQ, int(x[n // 2]), MS_CMP_GE, 0)]:
+        assert_same_column(ea.between(lo_cmp, lo, hi_cmp, hi, out_f, g, lanes),
+                           oracle.between(lo_cmp, lo, hi_cmp, hi, out_f, o, lanes))
+    with pytest.raises(ea.MsError):
+        ea.between(MS_CMP_NE, 1, MS_CMP_LE, 5, out_f, g, lanes)

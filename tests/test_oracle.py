@@ -317,3 +317,114 @@ def test_calc_all_ops(oracle):
         out = oracle.calc_binary(op, uncompr_f(), oracle.from_values(a), oracle.from_values(b))
         assert np.array_equal(out.values(), rm.calc(op, a, b)), op
     assert oracle.calc_binary(0, uncompr_f(), oracle.from_values(a), oracle.from_values(b[:10])) == 4
+
+
+# ------------------------------------------- prepare_for_random_access
+PRA_FORMATS = [static_vbp_f(12, 1), static_vbp_f(7, 8), dynamic_vbp_f(512, 2048, 8), default_dynamic_vbp_f(8),
+               default_delta_dynamic_vbp_f(8), default_for_dynamic_vbp_f(8), for_f(1024, 8, uncompr_f())]
+
+
+@pytest.mark.parametrize("f", PRA_FORMATS, ids=repr)
+@pytest.mark.parametrize("n", [0, 5, 512, 3000, 4096 + 77])
+def test_prepare_for_random_access(oracle, f, n):
+    """column.h:241-268 by its definition: the zero-padded rest, compressed
+    by an independent morph_batch call, sits at sizeCompr; the main part,
+    the raw rest and the meta data are unchanged."""
+    bits = f.bw if f.kind == 1 else 20
+    v = np.sort(rng_vals(n + 3, n, bits)) if f.kind == 3 else rng_vals(n + 3, n, bits)
+    col = oracle.morph(f, v)
+    before, meta = col.image(), col.meta()
+    bs = f.block_size()
+    rest = n % bs
+    acted = col.prepare_for_random_access()
+    assert acted == (rest > 0)
+    assert col.meta() == meta
+    after = col.image()
+    if not rest:
+        assert after.tobytes() == before.tobytes()
+        return
+    ru = -(-rest // bs) * bs
+    padded = np.concatenate([v[n - rest:], np.zeros(ru - rest, np.uint64)])
+    want = oracle.morph_batch_compress(f, padded)
+    c = col.size_compr
+    assert after[:c].tobytes() == before[:c].tobytes()
+    assert after[c:c + len(want)].tobytes() == want.tobytes()
+    r = oracle.rest_offset(f, c)
+    assert after[r:].tobytes() == before[r:].tobytes()
+    assert c + len(want) <= r
+    # the padding behind the rest is zero, and the compressed rest decodes
+    # to the padded rest
+    raw = col.raw(r + 8 * ru)
+    assert not raw[r + 8 * rest:].any()
+    dec, used = oracle.morph_batch_decompress(f, after[c:c + len(want)], ru)
+    assert used == len(want) and np.array_equal(dec, padded)
+    assert np.array_equal(col.values(), v)
+
+
+def test_prepare_for_random_access_uncompr_and_rle(oracle):
+    assert not oracle.from_values(np.arange(10, dtype=np.uint64)).prepare_for_random_access()
+    with pytest.raises(RuntimeError):
+        oracle.morph(rle_f(), np.arange(10, dtype=np.uint64)).prepare_for_random_access()
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_between_brute_force(oracle, lanes):
+    """between_compr.h: positions with both comparisons, laid out as select
+    lays them out (the two equal a select whose predicate is the same)."""
+    x = rng_vals(7, 5000 + 3, 12)
+    col = oracle.morph(dynamic_vbp_f(512, 2048, 8), x)
+    for lo_cmp, lo, hi_cmp, hi in [(MS_CMP_GE, 100, MS_CMP_LE, 900), (MS_CMP_GT, 100, MS_CMP_LT, 900),
+                                   (MS_CMP_GE, 900, MS_CMP_LE, 100), (MS_CMP_EQ, 7, MS_CMP_LE, 4095)]:
+        o = oracle.between(lo_cmp, lo, hi_cmp, hi, uncompr_f(), col, lanes)
+        want = np.flatnonzero(rm.CMP[lo_cmp](x, np.uint64(lo)) & rm.CMP[hi_cmp](x, np.uint64(hi)))
+        assert np.array_equal(o.values(), want.astype(np.uint64))
+    # a one-sided between is a select
+    a = oracle.between(MS_CMP_GE, 0, MS_CMP_LT, 2048, default_delta_dynamic_vbp_f(8), col, lanes)
+    b = oracle.select(MS_CMP_LT, 2048, default_delta_dynamic_vbp_f(8), col, lanes)
+    assert a.image().tobytes() == b.image().tobytes() and a.meta() == b.meta()
+
+
+# ------------------------------------------------------------ group, join
+def test_kat_group_join(oracle):
+    k = KAT["group_unary"]
+    g, e = oracle.group(oracle.from_values(k["data"]))
+    assert g.values().tolist() == k["gr"] and e.values().tolist() == k["ext"]
+    k = KAT["group_binary"]
+    g, e = oracle.group(oracle.from_values(k["data"]), oracle.from_values(k["in_gr"]))
+    assert g.values().tolist() == k["gr"] and e.values().tolist() == k["ext"]
+    k = KAT["join"]
+    a, b = oracle.join(oracle.from_values(k["l"]), oracle.from_values(k["r"]))
+    assert a.values().tolist() == k["out_l"] and b.values().tolist() == k["out_r"]
+    k = KAT["semi_join"]
+    assert oracle.semi_join(oracle.from_values(k["l"]), oracle.from_values(k["r"])).values().tolist() == k["out"]
+
+
+@pytest.mark.parametrize("n", [0, 1, 1000, 20_000])
+def test_group_brute_force(oracle, n):
+    x = rng_vals(n + 11, n, 6)
+    y = rng_vals(n + 12, n, 2)
+    for gin in (None, y):
+        keys = list(zip(gin.tolist(), x.tolist())) if gin is not None else x.tolist()
+        ids, ext = {}, []
+        want = []
+        for i, kk in enumerate(keys):
+            if kk not in ids:
+                ids[kk] = len(ids)
+                ext.append(i)
+            want.append(ids[kk])
+        g, e = oracle.group(oracle.from_values(x), oracle.from_values(gin) if gin is not None else None)
+        assert g.values().tolist() == want and e.values().tolist() == ext
+
+
+@pytest.mark.parametrize("na,nb", [(0, 5), (7, 0), (300, 200), (200, 300), (256, 256)])
+def test_join_brute_force(oracle, na, nb):
+    a = rng_vals(na + 1, na, 5)
+    b = rng_vals(nb + 2, nb, 5)
+    l, r = oracle.join(oracle.from_values(a), oracle.from_values(b))
+    if na >= nb:
+        want = [(i, j) for i in range(na) for j in range(nb) if a[i] == b[j]]
+    else:   # the outer loop over the longer right input
+        want = [(i, j) for j in range(nb) for i in range(na) if a[i] == b[j]]
+    assert list(zip(l.values().tolist(), r.values().tolist())) == want
+    s = oracle.semi_join(oracle.from_values(a), oracle.from_values(b))
+    assert s.values().tolist() == [i for i in range(na) if a[i] in set(b.tolist())]
