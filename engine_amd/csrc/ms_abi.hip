@@ -149,6 +149,9 @@ static bool valid_format(const ms_format_desc *fp)
     case MS_STATIC_VBP:
     case MS_DYNAMIC_VBP: return valid_leaf(f);
     case MS_RLE: return true;
+    case MS_STATIC_VBP_PAD: return f.bw >= 1 && f.bw <= 64 && f.step >= 1 && f.step <= 64;
+    case MS_K_WISE_NS: return f.block == 2;   // the reference implements 2-wise null suppression only
+    case MS_GROUP_SIMPLE: return f.step >= 1 && f.step <= 64;
     case MS_DELTA:
     case MS_FOR: {
         ms_format_desc in = {};
@@ -182,9 +185,16 @@ static uint64_t size_max(const ms_format_desc &f, uint64_t n)
                (n / f.block) * ms_leaf_size_max(f.inner_kind, f.inner_bw, f.inner_block, f.inner_page, f.block);
     }
     case MS_RLE: return 16 * n;
+    case MS_STATIC_VBP_PAD: return ms_pad_size_max(f.bw, f.step, n);
+    case MS_K_WISE_NS: return ms_kwns_size_max(f.block, n);
+    case MS_GROUP_SIMPLE: return ms_gs_size_max(f.step, n);
     default: (void)g; return 0;
     }
 }
+
+// Formats of the extra codecs (ms_xfmt.hip): column-level morph and
+// decompression only.
+static bool xfmt(uint32_t kind) { return kind == MS_STATIC_VBP_PAD || kind == MS_K_WISE_NS || kind == MS_GROUP_SIMPLE; }
 
 static uint64_t size_max_any_len(const ms_format_desc &f, uint64_t n)
 {
@@ -252,7 +262,9 @@ int ms_need_init()
 // ------------------------------------------------------------------ columns
 static uint64_t idx_entries(const ms_geom &g, uint64_t count)
 {
-    if (g.kind == MS_DYNAMIC_VBP || g.kind == MS_DELTA || g.kind == MS_FOR) return count / g.inner_bs + 2;
+    if (g.kind == MS_DYNAMIC_VBP || g.kind == MS_DELTA || g.kind == MS_FOR || g.kind == MS_K_WISE_NS ||
+        g.kind == MS_GROUP_SIMPLE)
+        return count / g.inner_bs + 2;
     return 0;
 }
 
@@ -267,7 +279,8 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     c->stream = s;
     c->index_cap = idx_entries(c->g, count_bound);
     c->refs_cap = f.kind == MS_FOR ? count_bound / f.block + 2
-                : f.kind == MS_DELTA ? (count_bound / MS_DELTA_TILE + 2) * 8 : 0;
+                : f.kind == MS_DELTA ? (count_bound / MS_DELTA_TILE + 2) * 8
+                : f.kind == MS_GROUP_SIMPLE ? count_bound / f.step + 2 : 0;
     c->size_alloc = size_alloc;
     const uint64_t idx_bytes = (c->index_cap * 8 + 255) / 256 * 256;
     const uint64_t refs_bytes = (c->refs_cap * 8 + 255) / 256 * 256;
@@ -361,6 +374,47 @@ static int host_index_walk(const ms_format_desc &f, const ms_geom &g, const uint
         return MS_OK;
     };
     switch (f.kind) {
+    case MS_K_WISE_NS: {   // a byte offset per 32 pairs
+        for (uint64_t i = 0; i < pt.n_compr / 2; i++) {
+            if (pos >= end) return ms_fail(MS_ERR_INVALID, "k_wise_ns pair past sizeCompr");
+            if (i % (MS_KWNS_PER_ENTRY / 2) == 0) idx.push_back(pos);
+            const uint32_t d = img[pos];
+            pos += 17 - (d & 7) - ((d >> 3) & 7);
+        }
+        return pos <= end ? MS_OK : ms_fail(MS_ERR_INVALID, "k_wise_ns data past sizeCompr");
+    }
+    case MS_GROUP_SIMPLE: {   // per block (data offset << 8) | code and its first value
+        static const uint8_t num[15] = {64, 32, 21, 16, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1};
+        const uint64_t E = f.step, a = 8 * E;
+        uint64_t done = 0;
+        while (done < pt.n_compr) {
+            if (pos + 9 > end) return ms_fail(MS_ERR_INVALID, "group_simple header past sizeCompr");
+            uint64_t cs;
+            memcpy(&cs, img + pos, 8);
+            const uint32_t R = img[pos + 8];
+            if (R >= 64 || cs > pt.n_compr || pos + 9 + (cs + 1) / 2 > end)
+                return ms_fail(MS_ERR_INVALID, "group_simple header");
+            uint64_t w = (pos + 9 + (cs + 1) / 2 + a - 1) / a * a;
+            for (uint64_t k = 0; k < cs; k++) {
+                const uint32_t sl = (img[pos + 9 + k / 2] >> (4 * (k % 2))) & 15;
+                if (sl > 14) return ms_fail(MS_ERR_INVALID, "group_simple selector %u", sl);
+                idx.push_back((w << 8) | sl);
+                refs.push_back(done);
+                done += num[sl] * E;
+                w += a;
+            }
+            if (R) {
+                idx.push_back((w << 8) | (16 + R));
+                refs.push_back(done);
+                done += R * E;
+                w += a;
+            }
+            if (!cs && !R) return ms_fail(MS_ERR_INVALID, "group_simple call without blocks");
+            if (w > end || done > pt.n_compr) return ms_fail(MS_ERR_INVALID, "group_simple blocks past sizeCompr");
+            pos = w;
+        }
+        return MS_OK;
+    }
     case MS_DYNAMIC_VBP: return leaf_call(pt.n_compr);
     case MS_DELTA:
         for (uint64_t b = 0; b < pt.n_compr / f.block; b++) MS_TRY(leaf_call(f.block));
@@ -411,6 +465,8 @@ static int check_meta(const ms_format_desc &f, const ms_col_meta &m)
     if (pt.n_compr % g.bs) return ms_fail(MS_ERR_INVALID, "compressed part not a multiple of the block size");
     if (f.kind == MS_STATIC_VBP && m.size_compr != pt.n_compr * f.bw / 8)
         return ms_fail(MS_ERR_INVALID, "static_vbp: sizeCompr does not match count");
+    if (f.kind == MS_STATIC_VBP_PAD && m.size_compr != pt.n_compr / g.bs * f.step * 8)
+        return ms_fail(MS_ERR_INVALID, "static_vbp<vbp_padding_l>: sizeCompr does not match count");
     return MS_OK;
 }
 
@@ -437,6 +493,7 @@ extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *
     dm.count = meta->count;
     dm.size_used = meta->size_used;
     dm.size_compr = meta->size_compr;
+    if (f->kind == MS_GROUP_SIMPLE) dm.aux[2] = idx.size();   // its blocks
     if (meta->size_used) MS_HIP(hipMemcpyAsync(c->data, image, meta->size_used, hipMemcpyHostToDevice, S(stream)));
     if (!idx.empty()) MS_HIP(hipMemcpyAsync(c->index, idx.data(), idx.size() * 8, hipMemcpyHostToDevice, S(stream)));
     if (!refs.empty()) MS_HIP(hipMemcpyAsync(c->refs, refs.data(), refs.size() * 8, hipMemcpyHostToDevice, S(stream)));
@@ -785,7 +842,7 @@ static uint64_t wit_chunk(const ms_format_desc &f)
 static bool wit_ok(const ms_format_desc *f)
 {
     if (f->kind == MS_UNCOMPR) return true;
-    if (f->kind == MS_RLE) return false;
+    if (f->kind == MS_RLE || xfmt(f->kind)) return false;
     const uint64_t bs = ms_make_geom(*f).bs;
     return wit_chunk(*f) % bs == 0;
 }
@@ -824,6 +881,9 @@ static int emit_new(const ms_format_desc &f, const uint64_t *vals, const uint64_
         hipLaunchKernelGGL(k_emit_uncompr, dim3(grid_of(m_bound)), dim3(256), 0, sc.s, vals, d_m, c->meta,
                            (uint64_t *)c->data);
         rc = hipGetLastError() == hipSuccess ? MS_OK : ms_fail(MS_ERR_HIP, "emit uncompr");
+    } else if (xfmt(f.kind)) {
+        rc = chunk == 0 ? launch_xfmt_compress(c, vals, d_m, m_bound, 0, sc)
+                        : ms_fail(MS_ERR_INVALID, "no write-iterator for format kind %u", f.kind);
     } else {
         rc = launch_emit(c, vals, d_m, m_bound, chunk, nullptr, sc);
     }
@@ -937,7 +997,7 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
         return MS_OK;
     }
     MS_TRY(new_col(*dst, nb, out_capacity(*dst, nb, 0), S(stream), &c));
-    int rc = launch_emit(c, vals, d_n, nb, 0, nullptr, sc);
+    int rc = xfmt(dst->kind) ? launch_xfmt_compress(c, vals, d_n, nb, 0, sc) : launch_emit(c, vals, d_n, nb, 0, nullptr, sc);
     if (rc) { ms_col_free(c); return rc; }
     *out = c;
     return MS_OK;
@@ -1024,10 +1084,11 @@ extern "C" int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_co
     const uint64_t np = pos->count_bound;
     const uint64_t *pv, *d_np;
     MS_TRY(values_of(pos, sc, &pv, &d_np));
-    // data without random access is decompressed first
+    // data without random access (delta_f, rle_f and the formats of the
+    // extra codecs) is decompressed first
     ms_col tmpdata;
     const ms_col *dsrc = data;
-    if (data->f.kind == MS_DELTA || data->f.kind == MS_RLE) {
+    if (data->f.kind == MS_DELTA || data->f.kind == MS_RLE || xfmt(data->f.kind)) {
         const uint64_t *dv, *d_dn;
         MS_TRY(values_of(data, sc, &dv, &d_dn));
         ms_dev_meta *dm = (ms_dev_meta *)sc.get(sizeof(ms_dev_meta));

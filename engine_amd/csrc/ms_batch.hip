@@ -311,6 +311,8 @@ extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uin
     if (!b || !in) return ms_fail(MS_ERR_INVALID, "null argument");
     MS_TRY(check_unheld(b));
     if (cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "comparison %u", cmp);
+    if (in->g.kind == MS_STATIC_VBP_PAD || in->g.kind == MS_K_WISE_NS || in->g.kind == MS_GROUP_SIMPLE)
+        return ms_fail(MS_ERR_INVALID, "no batched decoder for format kind %u: decompress it first", in->g.kind);
     if (in->g.kind == MS_DELTA && !delta_batchable(in->g))
         return ms_fail(MS_ERR_INVALID, "batched delta_f needs step 8 over an 8-lane dynamic_vbp inner format");
     ms_bentry e = {};

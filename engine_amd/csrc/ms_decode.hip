@@ -122,6 +122,17 @@ __global__ __launch_bounds__(WG) void k_select_sum_uncompr(ms_colv c, ms_pred p,
     wg_partial(acc, res);
 }
 
+// *d_n decompressed values.
+__global__ __launch_bounds__(WG) void k_select_sum_values(const uint64_t *x, const uint64_t *d_n, ms_pred p,
+                                                          uint64_t *res)
+{
+    const uint64_t n = *d_n;
+    uint64_t acc = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < n; i += (uint64_t)gridDim.x * WG)
+        acc += pred_eval(p, x[i]) ? x[i] : 0ull;
+    wg_partial(acc, res);
+}
+
 // Any layout readable by read_compr, one value per thread.
 __global__ __launch_bounds__(WG) void k_select_sum_generic(ms_colv c, ms_pred p, uint64_t *res)
 {
@@ -743,6 +754,27 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
         else hipLaunchKernelGGL((k_delta_tiles<1, 0>), dim3(grid), dim3(WG), 0, s, v, arr, nullptr, p, part);
         break;
     }
+    case MS_STATIC_VBP_PAD:
+    case MS_K_WISE_NS:
+    case MS_GROUP_SIMPLE: {
+        // these layouts are read through decompression into a temporary
+        uint64_t *tmp;
+        if (bufs) {
+            if (nb > bufs->tmp_values)
+                return ms_fail(MS_ERR_CAPACITY, "a column of %llu values exceeds the %llu-value temporary of the "
+                               "caller's buffers", (unsigned long long)nb, (unsigned long long)bufs->tmp_values);
+            tmp = bufs->tmp;
+        } else {
+            tmp = (uint64_t *)sc.get(nb * 8 + 64);
+        }
+        if (!tmp) return ms_fail(MS_ERR_NOMEM, "select_sum temporary");
+        MS_TRY(launch_decompress(in, tmp, s));
+        grid = grid_for(nb, WG * 32ull, 2048);
+        hipLaunchKernelGGL(k_select_sum_values, dim3(grid), dim3(WG), 0, s, (const uint64_t *)tmp, &in->meta->count,
+                           p, part);
+        with_rest = 0;
+        break;
+    }
     default:
         return ms_fail(MS_ERR_INVALID, "select_sum: unsupported format");
     }
@@ -821,6 +853,11 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         else hipLaunchKernelGGL((k_delta_tiles<0, 0>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);
         break;
     }
+    case MS_STATIC_VBP_PAD:
+    case MS_K_WISE_NS:
+    case MS_GROUP_SIMPLE:
+        MS_TRY(launch_xfmt_decompress(in, d_out, s));
+        break;
     default:
         return ms_fail(MS_ERR_INVALID, "decompress: unsupported format");
     }

@@ -70,6 +70,25 @@ MS_HD uint64_t ms_leaf_size_max(uint32_t kind, uint32_t bw, uint64_t block, uint
     return 8 * n + (n / block) * page;
 }
 
+// get_size_max_byte of the formats below (vbp_padding.h:60-63,
+// k_wise_ns.h:126-130, group_simple.h:93-121) for n a multiple of the
+// block size.
+MS_HD uint64_t ms_pad_size_max(uint32_t bw, uint32_t step, uint64_t n)
+{
+    const uint64_t bs = (uint64_t)(64 / bw) * step;
+    return n / bs * step * 8 * 10;
+}
+MS_HD uint64_t ms_kwns_size_max(uint64_t block, uint64_t n) { return 8 * n + n / block; }
+MS_HD uint64_t ms_gs_size_max(uint32_t E, uint64_t n)
+{
+    const uint64_t calls = (n + 1023) / 1024;   // the reference's fixed 1024-value call bound
+    const uint64_t sels = n / E / 2;
+    return calls * (8 + 1) + (sels + 1) / 2 * 2 + calls * (8ull * E - 1) + 8 * n;
+}
+
+// Values of a k_wise_ns_f column per block-index entry (32 pairs).
+#define MS_KWNS_PER_ENTRY 64ull
+
 MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
 {
     ms_geom g = {};
@@ -105,6 +124,18 @@ MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
         }
         break;
     }
+    case MS_STATIC_VBP_PAD:
+        g.leaf = MS_STATIC_VBP_PAD; g.bw = f.bw; g.lanes = f.step; g.bs = (uint64_t)(64 / f.bw) * f.step;
+        g.inner_bs = g.bs; g.gap = 2 * ms_pad_size_max(f.bw, f.step, g.bs);
+        break;
+    case MS_K_WISE_NS:
+        g.leaf = MS_K_WISE_NS; g.lanes = f.block; g.bs = f.block; g.inner_bs = MS_KWNS_PER_ENTRY;
+        g.gap = 2 * ms_kwns_size_max(f.block, f.block);
+        break;
+    case MS_GROUP_SIMPLE:
+        g.leaf = MS_GROUP_SIMPLE; g.lanes = f.step; g.bs = f.step; g.inner_bs = f.step;
+        g.gap = 2 * ms_gs_size_max(f.step, f.step);
+        break;
     default:   // MS_RLE: no block size, no rest
         g.leaf = MS_RLE; g.lanes = 1; g.bs = 0; g.inner_bs = 0; g.gap = 0;
         break;

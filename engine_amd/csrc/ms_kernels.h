@@ -75,6 +75,13 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
 // rle_f compression of d_v[0 .. *d_m)
 int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc);
 
+// static_vbp<vbp_padding_l>, k_wise_ns, group_simple (ms_xfmt.hip): the
+// column-level morph of d_v[0 .. *d_m) into `out` (the call starts base0
+// bytes past the image base), and decompression of the compressed part
+int launch_xfmt_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t base0,
+                         temps &sc);
+int launch_xfmt_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
+
 // operators ---------------------------------------------------------
 int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_np, uint64_t np_bound,
                   uint64_t *d_out, uint64_t *d_status, temps &sc);

@@ -83,8 +83,13 @@ int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_n
 {
     ms_colv v;
     MS_TRY(colv_of(data, v));
-    if (v.g.kind == MS_DELTA || v.g.kind == MS_RLE)
-        return ms_fail(MS_ERR_INVALID, "gather: no random access into this format");
+    // gather_one reads uncompr_f, static_vbp_f and the block index of
+    // dynamic_vbp_f / for_f; every other layout has no random access
+    const uint32_t k = v.g.kind;
+    if (k != MS_UNCOMPR && k != MS_STATIC_VBP && k != MS_DYNAMIC_VBP && k != MS_FOR)
+        return ms_fail(MS_ERR_INVALID, "gather: no random access into format kind %u", k);
+    if ((k == MS_DYNAMIC_VBP || k == MS_FOR) && !v.index)
+        return ms_fail(MS_ERR_INVALID, "gather: column without a block index");
     hipLaunchKernelGGL(k_gather, dim3(grid_for(np_bound, WG * 4ull)), dim3(WG), 0, sc.s, v, d_pos, d_np, d_out,
                        d_status);
     MS_HIP(hipGetLastError());

@@ -7,9 +7,11 @@ Here the same arguments fill an ``ms_format_desc`` (include/morphstore_hip.h).
 """
 import ctypes
 
-MS_UNCOMPR, MS_STATIC_VBP, MS_DYNAMIC_VBP, MS_DELTA, MS_FOR, MS_RLE = range(6)
+(MS_UNCOMPR, MS_STATIC_VBP, MS_DYNAMIC_VBP, MS_DELTA, MS_FOR, MS_RLE, MS_STATIC_VBP_PAD, MS_K_WISE_NS,
+ MS_GROUP_SIMPLE) = range(9)
 KIND_NAMES = {MS_UNCOMPR: "uncompr_f", MS_STATIC_VBP: "static_vbp_f", MS_DYNAMIC_VBP: "dynamic_vbp_f",
-              MS_DELTA: "delta_f", MS_FOR: "for_f", MS_RLE: "rle_f"}
+              MS_DELTA: "delta_f", MS_FOR: "for_f", MS_RLE: "rle_f", MS_STATIC_VBP_PAD: "static_vbp_f",
+              MS_K_WISE_NS: "k_wise_ns_f", MS_GROUP_SIMPLE: "group_simple_f"}
 
 MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE = range(6)
 (MS_OP_ADD, MS_OP_SUB, MS_OP_MUL, MS_OP_DIV, MS_OP_MOD, MS_OP_MIN, MS_OP_MAX,
@@ -34,6 +36,12 @@ class FormatDesc(ctypes.Structure):
             return f"dynamic_vbp_f<{self.block},{self.page},{self.step}>"
         if k == MS_RLE:
             return "rle_f"
+        if k == MS_STATIC_VBP_PAD:
+            return f"static_vbp_f<vbp_padding_l<{self.bw},{self.step}>>"
+        if k == MS_K_WISE_NS:
+            return f"k_wise_ns_f<{self.block}>"
+        if k == MS_GROUP_SIMPLE:
+            return f"group_simple_f<{self.step},uint64_t,{8 * self.step}>"
         inner = FormatDesc(self.inner_kind, self.inner_bw, self.inner_block, self.inner_page,
                            self.inner_step).name()
         if k == MS_DELTA:
@@ -50,6 +58,10 @@ class FormatDesc(ctypes.Structure):
             return 64 * self.step
         if self.kind == MS_RLE:
             return None
+        if self.kind == MS_STATIC_VBP_PAD:
+            return 64 // self.bw * self.step
+        if self.kind == MS_GROUP_SIMPLE:
+            return self.step
         return self.block
 
 
@@ -75,6 +87,21 @@ def uncompr_f():
 def static_vbp_f(bw, step):
     """static_vbp_f<vbp_l<bw, step>>."""
     return FormatDesc(MS_STATIC_VBP, bw, 0, 0, step)
+
+
+def static_vbp_padding_f(bw, step):
+    """static_vbp_f<vbp_padding_l<bw, step>>."""
+    return FormatDesc(MS_STATIC_VBP_PAD, bw, 0, 0, step)
+
+
+def k_wise_ns_f(block=2):
+    """k_wise_ns_f<block> (the reference implements block 2)."""
+    return FormatDesc(MS_K_WISE_NS, 0, block, 0, 0)
+
+
+def group_simple_f(lanes=8):
+    """group_simple_f<lanes, uint64_t, 8 * lanes> (DEFAULT_GROUP_SIMPLE_F)."""
+    return FormatDesc(MS_GROUP_SIMPLE, 0, 0, 0, lanes)
 
 
 def dynamic_vbp_f(block, page, step):

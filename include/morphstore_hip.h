@@ -57,7 +57,10 @@ enum ms_format_kind {
     MS_DYNAMIC_VBP = 2,  /* dynamic_vbp_f<block,page,step>  dynamic_vbp.h:83-159  */
     MS_DELTA = 3,        /* delta_f<block,step,inner>       delta.h:78-157        */
     MS_FOR = 4,          /* for_f<block,page,inner>         for.h:84-164          */
-    MS_RLE = 5           /* rle_f                           rle.h:42-49           */
+    MS_RLE = 5,          /* rle_f                           rle.h:42-49           */
+    MS_STATIC_VBP_PAD = 6, /* static_vbp_f<vbp_padding_l<bw,step>> vbp_padding.h:52-66 */
+    MS_K_WISE_NS = 7,    /* k_wise_ns_f<block>              k_wise_ns.h:43-135    */
+    MS_GROUP_SIMPLE = 8  /* group_simple_f<step, uint64_t, 8*step> group_simple.h:64-131 */
 };
 
 /* Meta byte of a non-existent block in the last page of a dynamic_vbp
@@ -68,6 +71,12 @@ enum ms_format_kind {
  * A format descriptor is the run-time form of the reference's format
  * template arguments.
  *   static_vbp_f<vbp_l<bw,step>>     : bw, step
+ *   static_vbp_f<vbp_padding_l<bw,step>> (MS_STATIC_VBP_PAD): bw, step; a
+ *       block is 64/bw codes per lane, one 64-bit word per lane
+ *   k_wise_ns_f<block>               : block (2: the reference implements
+ *       only the SSE variant, 2-wise null suppression)
+ *   group_simple_f<step, uint64_t, 8*step> : step (values per group; the
+ *       data area is aligned to 8*step bytes from the image base)
  *   dynamic_vbp_f<block,page,step>   : block, page, step.  `step` is the
  *       number of interleaved lanes the payload is packed with.  The
  *       reference ignores t_Step and packs with the vector extension's

@@ -28,6 +28,7 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -174,6 +175,58 @@ struct rle_f : format {
     }
 };
 
+/* vbp_padding_l<bw, step> (vbp_padding.h:45-66): the layout of
+ * static_vbp_f<vbp_padding_l<bw, step>>. */
+template <unsigned t_bw, unsigned t_step>
+struct vbp_padding_l {
+    static constexpr unsigned m_Bw = t_bw;
+    static constexpr unsigned m_Step = t_step;
+};
+
+template <unsigned t_bw, unsigned t_step>
+struct static_vbp_f<vbp_padding_l<t_bw, t_step>> : format {
+    static_assert(t_bw >= 1 && t_bw <= 64, "vbp_padding_l: bit width in [1, 64]");
+    static constexpr size_t m_BlockSize = (64 / t_bw) * t_step;
+    static ms_format_desc desc()
+    {
+        ms_format_desc d = {};
+        d.kind = MS_STATIC_VBP_PAD;
+        d.bw = t_bw;
+        d.step = t_step;
+        return d;
+    }
+};
+
+/* k_wise_ns_f<t_BlockSizeLog> (k_wise_ns.h:43-135; implemented for 2). */
+template <size_t t_BlockSizeLog>
+struct k_wise_ns_f : format {
+    static_assert(t_BlockSizeLog == 2, "k_wise_ns_f: the reference implements 2-wise null suppression");
+    static constexpr size_t m_BlockSize = t_BlockSizeLog;
+    static ms_format_desc desc()
+    {
+        ms_format_desc d = {};
+        d.kind = MS_K_WISE_NS;
+        d.block = (uint32_t)t_BlockSizeLog;
+        return d;
+    }
+};
+
+/* group_simple_f<t_GroupSizeLog, t_base_t, t_AlignmentBytes>
+ * (group_simple.h:64-131) with 64-bit values and vector-sized alignment. */
+template <size_t t_GroupSizeLog, typename t_base_t, size_t t_AlignmentBytes>
+struct group_simple_f : format {
+    static_assert(std::is_same<t_base_t, uint64_t>::value, "group_simple_f: uint64_t values");
+    static_assert(t_AlignmentBytes == 8 * t_GroupSizeLog, "group_simple_f: aligned to one vector of the group");
+    static constexpr size_t m_BlockSize = t_GroupSizeLog;
+    static ms_format_desc desc()
+    {
+        ms_format_desc d = {};
+        d.kind = MS_GROUP_SIMPLE;
+        d.step = (uint32_t)t_GroupSizeLog;
+        return d;
+    }
+};
+
 // ------------------------------------------------------------------ column
 /* column<F> (column.h:45-287): a device column in format F. */
 template <class F>
@@ -216,6 +269,15 @@ class column {
 
     ms_col *handle() const { return m_Col; }
 
+    /* column.h:241-268: true if the column changed. */
+    template <class t_vector_extension>
+    bool prepare_for_random_access() const
+    {
+        int acted = 0;
+        ms_check(ms_col_prepare_for_random_access(m_Col, &acted, current_stream()));
+        return acted != 0;
+    }
+
   private:
     ms_col *m_Col;
     ms_col_meta meta() const
@@ -247,6 +309,23 @@ const column<F> *upload_column(const void *image, size_t count, size_t size_used
     ms_check(ms_col_upload(&c, &d, image, &m, current_stream()));
     return new column<F>(c);
 }
+
+/* binary_io<F> (binary_io.h:38-87). */
+template <class F>
+class binary_io {
+  public:
+    static const column<F> *load(const std::string &p_Filename)
+    {
+        const ms_format_desc d = F::desc();
+        ms_col *c = nullptr;
+        ms_check(ms_col_load(&c, &d, p_Filename.c_str(), current_stream()));
+        return new column<F>(c);
+    }
+    static void store(const column<F> *p_Column, const std::string &p_FileName)
+    {
+        ms_check(ms_col_store(p_Column->handle(), p_FileName.c_str(), current_stream()));
+    }
+};
 
 // ------------------------------------------------------------------ operators
 template <class t_ve, class t_dst_f, class t_src_f>
@@ -299,6 +378,71 @@ struct my_select_wit_t {
         return new column<t_out_pos_f>(c);
     }
 };
+
+/* my_between_wit_t (between_compr.h:107-270). */
+template <template <class, int> class t_compare_lower, template <class, int> class t_compare_upper, class t_ve,
+          class t_out_pos_f, class t_in_data_f>
+struct my_between_wit_t {
+    static const column<t_out_pos_f> *apply(const column<t_in_data_f> *const inDataCol, const uint64_t val_lower,
+                                            const uint64_t val_upper, const size_t outPosCountEstimate = 0)
+    {
+        (void)outPosCountEstimate;
+        const ms_format_desc d = t_out_pos_f::desc();
+        ms_col *c = nullptr;
+        ms_check(ms_between(&c, t_compare_lower<t_ve, 64>::code, val_lower, t_compare_upper<t_ve, 64>::code,
+                            val_upper, &d, inDataCol->handle(), t_ve::lanes, current_stream()));
+        return new column<t_out_pos_f>(c);
+    }
+};
+
+/* group (interfaces/group.h:38-99): unary, then binary. */
+template <class t_ve, class t_out_gr_f, class t_out_ext_f, class t_in_data_f>
+const std::tuple<const column<t_out_gr_f> *, const column<t_out_ext_f> *>
+group(const column<t_in_data_f> *const inDataCol, const size_t outExtCountEstimate = 0)
+{
+    (void)outExtCountEstimate;
+    const ms_format_desc dg = t_out_gr_f::desc(), de = t_out_ext_f::desc();
+    ms_col *g = nullptr, *e = nullptr;
+    ms_check(ms_group(&g, &e, &dg, &de, nullptr, inDataCol->handle(), current_stream()));
+    return std::make_tuple(new column<t_out_gr_f>(g), new column<t_out_ext_f>(e));
+}
+
+template <class t_ve, class t_out_gr_f, class t_out_ext_f, class t_in_gr_f, class t_in_data_f>
+const std::tuple<const column<t_out_gr_f> *, const column<t_out_ext_f> *>
+group(const column<t_in_gr_f> *const inGrCol, const column<t_in_data_f> *const inDataCol,
+      const size_t outExtCountEstimate = 0)
+{
+    (void)outExtCountEstimate;
+    const ms_format_desc dg = t_out_gr_f::desc(), de = t_out_ext_f::desc();
+    ms_col *g = nullptr, *e = nullptr;
+    ms_check(ms_group(&g, &e, &dg, &de, inGrCol->handle(), inDataCol->handle(), current_stream()));
+    return std::make_tuple(new column<t_out_gr_f>(g), new column<t_out_ext_f>(e));
+}
+
+/* nested_loop_join and left_semi_nto1_nested_loop_join (interfaces/join.h). */
+template <class t_ve, class t_out_pos_l_f, class t_out_pos_r_f, class t_in_data_l_f, class t_in_data_r_f>
+const std::tuple<const column<t_out_pos_l_f> *, const column<t_out_pos_r_f> *>
+nested_loop_join(const column<t_in_data_l_f> *const inDataLCol, const column<t_in_data_r_f> *const inDataRCol,
+                 const size_t outCountEstimate = 0)
+{
+    (void)outCountEstimate;
+    const ms_format_desc dl = t_out_pos_l_f::desc(), dr = t_out_pos_r_f::desc();
+    ms_col *l = nullptr, *r = nullptr;
+    ms_check(ms_join(&l, &r, &dl, &dr, inDataLCol->handle(), inDataRCol->handle(), current_stream()));
+    return std::make_tuple(new column<t_out_pos_l_f>(l), new column<t_out_pos_r_f>(r));
+}
+
+template <class t_ve, class t_out_pos_l_f, class t_in_data_l_f, class t_in_data_r_f>
+const column<t_out_pos_l_f> *left_semi_nto1_nested_loop_join(const column<t_in_data_l_f> *const inDataLCol,
+                                                             const column<t_in_data_r_f> *const inDataRCol,
+                                                             const size_t outCountEstimate = 0)
+{
+    (void)outCountEstimate;
+    const ms_format_desc d = t_out_pos_l_f::desc();
+    ms_col *c = nullptr;
+    ms_check(ms_semi_join(&c, &d, inDataLCol->handle(), inDataRCol->handle(), current_stream()));
+    return new column<t_out_pos_l_f>(c);
+}
 
 template <class t_ve, class t_out_data_f, class t_in_data_f, class t_in_pos_f>
 struct my_project_wit_t {

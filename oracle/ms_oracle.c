@@ -76,6 +76,12 @@ static int valid(const ms_format_desc *f)
         return valid_leaf(f);
     case MS_RLE:
         return 1;
+    case MS_STATIC_VBP_PAD:   /* vbp_padding.h:45-66 */
+        return f->bw >= 1 && f->bw <= 64 && f->step >= 1 && f->step <= 64;
+    case MS_K_WISE_NS:        /* k_wise_ns.h: the SSE implementation, 2 values per block */
+        return f->block == 2;
+    case MS_GROUP_SIMPLE:     /* group_simple.h:64-131 with uint64_t, 8 E-byte alignment */
+        return f->step >= 1 && f->step <= 64;
     case MS_DELTA:
     case MS_FOR:
         inner_of(f, &in);
@@ -105,6 +111,9 @@ int oc_block_size(const ms_format_desc *f, uint64_t *out)
     case MS_DYNAMIC_VBP:
     case MS_DELTA:
     case MS_FOR: *out = f->block; return MS_OK;
+    case MS_STATIC_VBP_PAD: *out = (uint64_t)(64 / f->bw) * f->step; return MS_OK;   /* vbp_padding.h:64-66 */
+    case MS_K_WISE_NS: *out = f->block; return MS_OK;
+    case MS_GROUP_SIMPLE: *out = f->step; return MS_OK;                              /* t_GroupSizeLog */
     default: return MS_ERR_INVALID;                              /* rle_f has no m_BlockSize */
     }
 }
@@ -129,6 +138,15 @@ int oc_size_max_byte(const ms_format_desc *f, uint64_t n, uint64_t *out)
         return MS_OK;
     }
     case MS_RLE: *out = 16 * n; return MS_OK;
+    case MS_STATIC_VBP_PAD:   /* vbp_padding.h:60-63 */
+        *out = n / ((uint64_t)(64 / f->bw) * f->step) * f->step * 8 * 10;
+        return MS_OK;
+    case MS_K_WISE_NS: *out = 8 * n + n / f->block; return MS_OK;   /* k_wise_ns.h:126-130 */
+    case MS_GROUP_SIMPLE: {   /* group_simple.h:93-121 (the fixed 1024-value call bound) */
+        const uint64_t calls = (n + 1023) / 1024, sels = n / f->step / 2;
+        *out = calls * 9 + round_up(sels, 2) + calls * (8ull * f->step - 1) + 8 * n;
+        return MS_OK;
+    }
     default: return MS_ERR_INVALID;
     }
 }
@@ -433,10 +451,206 @@ static void rle_decompress(const uint8_t *in8, uint64_t words, uint64_t *out)
 }
 
 /* ------------------------------------------------------------------ */
+/* static_vbp_f<vbp_padding_l<bw, S>> (vbp_padding.h:52-205): a block  */
+/* is 64 / bw codes per lane; lane j's word = OR_k v[k S + j] << k bw,  */
+/* codes never cross words (unmasked on pack, masked on unpack).        */
+/* ------------------------------------------------------------------ */
+static uint64_t vbpp_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out8)
+{
+    const unsigned bw = f->bw, S = f->step, per = 64 / bw;
+    const uint64_t bs = (uint64_t)per * S;
+    uint64_t *out = (uint64_t *)out8;
+    for (uint64_t b = 0; b < n / bs; b++)
+        for (unsigned j = 0; j < S; j++) {
+            uint64_t w = 0;
+            for (unsigned k = 0; k < per; k++) w |= in[b * bs + (uint64_t)k * S + j] << (k * bw);
+            out[b * S + j] = w;
+        }
+    return n / bs * S * 8;
+}
+
+static uint64_t vbpp_decompress(const ms_format_desc *f, const uint8_t *in8, uint64_t n, uint64_t *out)
+{
+    const unsigned bw = f->bw, S = f->step, per = 64 / bw;
+    const uint64_t bs = (uint64_t)per * S, mask = bw_max(bw);
+    const uint64_t *in = (const uint64_t *)in8;
+    for (uint64_t b = 0; b < n / bs; b++)
+        for (unsigned j = 0; j < S; j++)
+            for (unsigned k = 0; k < per; k++) out[b * bs + (uint64_t)k * S + j] = (in[b * S + j] >> (k * bw)) & mask;
+    return n / bs * S * 8;
+}
+
+/* ------------------------------------------------------------------ */
+/* k_wise_ns_f<2> (k_wise_ns.h:43-262, the SSE implementation): per     */
+/* pair a descriptor byte (zero_bytes(x1) << 3 | zero_bytes(x0), with   */
+/* zero_bytes = leading zero bytes of (x | 1), math.h:115-120), then    */
+/* the low 8 - zb bytes of x0 and of x1.                                */
+/* ------------------------------------------------------------------ */
+static unsigned zero_bytes(uint64_t x) { return (unsigned)__builtin_clzll(x | 1) / 8; }
+
+static uint64_t kwns_compress(const uint64_t *in, uint64_t n, uint8_t *out)
+{
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i + 1 < n; i += 2) {
+        const unsigned z0 = zero_bytes(in[i]), z1 = zero_bytes(in[i + 1]);
+        out[pos++] = (uint8_t)((z1 << 3) | z0);
+        for (unsigned b = 0; b < 8 - z0; b++) out[pos++] = (uint8_t)(in[i] >> (8 * b));
+        for (unsigned b = 0; b < 8 - z1; b++) out[pos++] = (uint8_t)(in[i + 1] >> (8 * b));
+    }
+    return pos;
+}
+
+static uint64_t kwns_decompress(const uint8_t *in, uint64_t n, uint64_t *out)
+{
+    uint64_t pos = 0;
+    for (uint64_t i = 0; i < n; i += 2) {
+        const unsigned d = in[pos++], z0 = d & 7, z1 = (d >> 3) & 7;
+        uint64_t x0 = 0, x1 = 0;
+        for (unsigned b = 0; b < 8 - z0; b++) x0 |= (uint64_t)in[pos++] << (8 * b);
+        for (unsigned b = 0; b < 8 - z1; b++) x1 |= (uint64_t)in[pos++] << (8 * b);
+        out[i] = x0;
+        out[i + 1] = x1;
+    }
+    return pos;
+}
+
+/* ------------------------------------------------------------------ */
+/* group_simple_f<E, uint64_t, 8 E> (group_simple.h:64-365; packing of  */
+/* group_simple_routine_gen.py:101-168).  Groups of E values, a group's */
+/* pseudo maximum is their OR.  Greedily, the first mode whose next     */
+/* min(n, groups left, 64) groups all fit its width takes n groups; a   */
+/* mode that does not get n groups ends the selectors and the groups    */
+/* left go to one incomplete block of width 64 / count.  A call writes  */
+/* the selector count (uint64), the left groups (uint8), the selectors  */
+/* (4 bit, low nibble first), padding to 8 E bytes from the image base  */
+/* and one 8 E-byte vector per block.                                   */
+/* ------------------------------------------------------------------ */
+static const unsigned gs_num[15] = {64, 32, 21, 16, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1};
+static const unsigned gs_bits[15] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16, 21, 32, 64};
+
+static uint64_t align_to(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+/* base = the call's offset from the image base (the padding aligns to it) */
+static uint64_t gs_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out, uint64_t base)
+{
+    const unsigned E = f->step;
+    const uint64_t G = n / E;
+    uint64_t *gmax = (uint64_t *)malloc(8 * (G + 1));
+    uint8_t *sel = (uint8_t *)malloc(G + 1);
+    uint64_t p = 0, cs = 0, data, w;
+    for (uint64_t g = 0; g < G; g++) {
+        uint64_t o = 0;
+        for (unsigned j = 0; j < E; j++) o |= in[g * E + j];
+        gmax[g] = o;
+    }
+    while (p < G) {
+        const uint64_t avail = G - p < 64 ? G - p : 64;
+        unsigned i, off = 0, nn = 0;
+        for (i = 0; i < 15; i++) {
+            const uint64_t mask = bw_max(gs_bits[i]);
+            const uint64_t maxpos = gs_num[i] < avail ? gs_num[i] : avail;
+            nn = gs_num[i];
+            off = 0;
+            while (off < maxpos && gmax[p + off] <= mask) off++;
+            if (off == maxpos) break;
+        }
+        if (off != nn) break;
+        sel[cs++] = (uint8_t)i;
+        p += nn;
+    }
+    *(uint64_t *)out = cs;
+    out[8] = (uint8_t)(G - p);
+    for (uint64_t k = 0; k < cs; k++) {
+        if (k % 2 == 0) out[9 + k / 2] = sel[k];
+        else out[9 + k / 2] |= (uint8_t)(sel[k] << 4);
+    }
+    data = align_to(base + 9 + (cs + 1) / 2, 8ull * E) - base;
+    w = data;
+    p = 0;
+    for (uint64_t k = 0; k < cs; k++) {
+        const unsigned nn = gs_num[sel[k]], b = gs_bits[sel[k]];
+        for (unsigned j = 0; j < E; j++) {
+            uint64_t v = 0;
+            for (unsigned q = 0; q < nn; q++) v |= in[(p + q) * E + j] << (q * b);
+            memcpy(out + w + 8 * j, &v, 8);
+        }
+        p += nn;
+        w += 8ull * E;
+    }
+    if (G - p) {
+        const unsigned R = (unsigned)(G - p), bw = 64 / R;
+        for (unsigned j = 0; j < E; j++) {
+            uint64_t v = in[p * E + j];
+            for (unsigned q = 1; q < R; q++) v |= in[(p + q) * E + j] << (q * bw);
+            memcpy(out + w + 8 * j, &v, 8);
+        }
+        w += 8ull * E;
+    }
+    free(gmax);
+    free(sel);
+    return w;
+}
+
+static int gs_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t base,
+                         uint64_t *used)
+{
+    const unsigned E = f->step;
+    uint64_t done = 0, pos = 0;
+    while (done < n) {
+        uint64_t cs;
+        memcpy(&cs, in + pos, 8);
+        const unsigned R = in[pos + 8];
+        const uint8_t *sel = in + pos + 9;
+        uint64_t w = align_to(base + pos + 9 + (cs + 1) / 2, 8ull * E) - base;
+        if (R >= 64) return MS_ERR_INVALID;
+        for (uint64_t k = 0; k < cs; k++) {
+            const unsigned s = (sel[k / 2] >> (4 * (k % 2))) & 15;
+            if (s > 14) return MS_ERR_INVALID;
+            const unsigned nn = gs_num[s], b = gs_bits[s];
+            if (done + (uint64_t)nn * E > n) return MS_ERR_INVALID;
+            for (unsigned j = 0; j < E; j++) {
+                uint64_t v;
+                memcpy(&v, in + w + 8 * j, 8);
+                for (unsigned q = 0; q < nn; q++)   /* the last group is not masked */
+                    out[done + (uint64_t)q * E + j] = q + 1 < nn ? (v >> (q * b)) & bw_max(b) : v >> (q * b);
+            }
+            done += (uint64_t)nn * E;
+            w += 8ull * E;
+        }
+        if (R) {
+            const unsigned bw = 64 / R;
+            if (done + (uint64_t)R * E > n) return MS_ERR_INVALID;
+            for (unsigned j = 0; j < E; j++) {
+                uint64_t v;
+                memcpy(&v, in + w + 8 * j, 8);
+                for (unsigned q = 0; q < R; q++) out[done + (uint64_t)q * E + j] = (v >> (q * bw)) & bw_max(bw);
+            }
+            done += (uint64_t)R * E;
+            w += 8ull * E;
+        }
+        if (!cs && !R) return MS_ERR_INVALID;   /* an empty call: no progress */
+        pos = w;
+    }
+    *used = pos;
+    return MS_OK;
+}
+
+/* ------------------------------------------------------------------ */
 /* morph_batch                                                          */
 /* ------------------------------------------------------------------ */
 
+static int morph_batch_compress_at(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out,
+                                   uint64_t base, uint64_t *out_bytes);
+
 int oc_morph_batch_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out, uint64_t *out_bytes)
+{
+    return morph_batch_compress_at(f, in, n, out, 0, out_bytes);
+}
+
+/* base: the call's offset from the image base (group_simple aligns its
+ * data area to it) */
+static int morph_batch_compress_at(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out,
+                                   uint64_t base, uint64_t *out_bytes)
 {
     uint64_t bs, last[64] = {0};
     if (!valid(f)) return MS_ERR_INVALID;
@@ -451,14 +665,22 @@ int oc_morph_batch_compress(const ms_format_desc *f, const uint64_t *in, uint64_
     case MS_DELTA: *out_bytes = delta_compress(f, in, n, out, last); return MS_OK;
     case MS_FOR: *out_bytes = for_compress(f, in, n, out); return MS_OK;
     case MS_RLE: *out_bytes = rle_compress(in, n, out); return MS_OK;
+    case MS_STATIC_VBP_PAD: *out_bytes = vbpp_compress(f, in, n, out); return MS_OK;
+    case MS_K_WISE_NS: *out_bytes = kwns_compress(in, n, out); return MS_OK;
+    case MS_GROUP_SIMPLE: *out_bytes = gs_compress(f, in, n, out, base); return MS_OK;
     default: return MS_ERR_INVALID;
     }
 }
 
 int oc_morph_batch_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t *in_bytes)
 {
+    uint64_t bs;
     if (!valid(f)) return MS_ERR_INVALID;
+    if (f->kind != MS_RLE && (oc_block_size(f, &bs) || n % bs)) return MS_ERR_INVALID;
     switch (f->kind) {
+    case MS_STATIC_VBP_PAD: *in_bytes = vbpp_decompress(f, in, n, out); return MS_OK;
+    case MS_K_WISE_NS: *in_bytes = kwns_decompress(in, n, out); return MS_OK;
+    case MS_GROUP_SIMPLE: return gs_decompress(f, in, n, out, 0, in_bytes);
     case MS_UNCOMPR:
     case MS_STATIC_VBP:
     case MS_DYNAMIC_VBP: return leaf_decompress(f, in, n, out, in_bytes);
@@ -590,10 +812,12 @@ int oc_prepare_for_random_access(oc_col *c, int *acted)
     oc_block_size(&c->f, &bs);
     ru = round_up(rest, bs);
     oc_rest_offset(&c->f, c->size_compr, &r);
-    if (r + 8 * ru > c->size_alloc) return MS_ERR_CAPACITY;
+    /* column.h:251-262 writes without a check; the buffer holds size_alloc
+     * plus the 64 bytes of alignment slack a column allocates */
+    if (r + 8 * ru > c->size_alloc + 64) return MS_ERR_CAPACITY;
     memset(c->data + r + 8 * rest, 0, 8 * (ru - rest));
     memset(c->data + c->size_compr, 0, r - c->size_compr);       /* packing ORs into zeroed words */
-    oc_morph_batch_compress(&c->f, (const uint64_t *)(c->data + r), ru, c->data + c->size_compr, &b);
+    morph_batch_compress_at(&c->f, (const uint64_t *)(c->data + r), ru, c->data + c->size_compr, c->size_compr, &b);
     if (c->size_compr + b > r) return MS_ERR_CAPACITY;
     *acted = 1;
     return MS_OK;

@@ -76,6 +76,38 @@ int main()
     const column<rle_f> *rl = morph<ve, rle_f>(make_column(std::vector<uint64_t>{4, 4, 4, 9, 9, 1}));
     CHECK(rl->get_size_used_byte() == 48);
     CHECK(values_of(rl) == (std::vector<uint64_t>{4, 4, 4, 9, 9, 1}));
+    // between, group, joins: the known answers of between/group/join tests
+    const column<uncompr_f> *bt =
+        my_between_wit_t<greaterequal, lessequal, ve, uncompr_f, sf>::apply(c, 100, 200);
+    std::vector<uint64_t> wbt;
+    for (size_t i = 0; i < n; i++)
+        if (x[i] >= 100 && x[i] <= 200) wbt.push_back(i);
+    CHECK(values_of(bt) == wbt);
+    auto gu = group<ve, uncompr_f, uncompr_f>(make_column(std::vector<uint64_t>{333, 333, 111, 333, 222, 111}));
+    CHECK(values_of(std::get<0>(gu)) == (std::vector<uint64_t>{0, 0, 1, 0, 2, 1}));
+    CHECK(values_of(std::get<1>(gu)) == (std::vector<uint64_t>{0, 2, 4}));
+    auto gb = group<ve, uncompr_f, uncompr_f>(std::get<0>(gu),
+                                              make_column(std::vector<uint64_t>{77, 88, 77, 77, 88, 88}));
+    CHECK(values_of(std::get<0>(gb)) == (std::vector<uint64_t>{0, 1, 2, 0, 3, 4}));
+    auto jn = nested_loop_join<ve, uncompr_f, uncompr_f>(make_column(std::vector<uint64_t>{22, 44, 11, 22, 55, 77}),
+                                                         make_column(std::vector<uint64_t>{33, 22, 22, 11}));
+    CHECK(values_of(std::get<0>(jn)) == (std::vector<uint64_t>{0, 0, 2, 3, 3}));
+    CHECK(values_of(std::get<1>(jn)) == (std::vector<uint64_t>{1, 2, 3, 1, 2}));
+    const column<uncompr_f> *sj = left_semi_nto1_nested_loop_join<ve, uncompr_f>(
+        make_column(std::vector<uint64_t>{11, 22, 33, 11, 44, 55}), make_column(std::vector<uint64_t>{22, 22, 33, 44, 33}));
+    CHECK(values_of(sj) == (std::vector<uint64_t>{1, 2, 4}));
+    // the other candidate formats and binary_io
+    using pf = static_vbp_f<vbp_padding_l<12, 8>>;
+    using gf = group_simple_f<8, uint64_t, 64>;
+    using kf = k_wise_ns_f<2>;
+    CHECK(values_of(morph<ve, pf>(base)) == x);
+    CHECK(values_of(morph<ve, gf>(base)) == x);
+    CHECK(values_of(morph<ve, kf>(base)) == x);
+    const column<sf> *c2 = morph<ve, sf>(base);
+    CHECK(c2->prepare_for_random_access<ve>() == (n % 64 != 0));
+    binary_io<sf>::store(c2, "/tmp/ms_template_api_col.bin");
+    const column<sf> *c3 = binary_io<sf>::load("/tmp/ms_template_api_col.bin");
+    CHECK(values_of(c3) == x);
     // errors surface as exceptions
     bool threw = false;
     try {

@@ -428,3 +428,131 @@ def test_join_brute_force(oracle, na, nb):
     assert list(zip(l.values().tolist(), r.values().tolist())) == want
     s = oracle.semi_join(oracle.from_values(a), oracle.from_values(b))
     assert s.values().tolist() == [i for i in range(na) if a[i] in set(b.tolist())]
+
+
+# ---------------------------------- vbp_padding, k_wise_ns, group_simple
+from engine_amd.formats import group_simple_f, k_wise_ns_f, static_vbp_padding_f  # noqa: E402
+
+
+@pytest.mark.parametrize("bw,S", [(1, 8), (3, 8), (7, 4), (12, 1), (21, 8), (33, 2), (64, 8)])
+def test_vbp_padding_closed_form(oracle, bw, S):
+    """vbp_padding.h:97-140: lane j's word of a block is the OR of its
+    64 / bw codes shifted by k bw; size = blocks x S words."""
+    f = static_vbp_padding_f(bw, S)
+    per = 64 // bw
+    bs = per * S
+    v = rng_vals(bw * 7 + S, 5 * bs, bw)
+    img = oracle.morph_batch_compress(f, v)
+    want = np.zeros(5 * S, np.uint64)
+    for b in range(5):
+        for j in range(S):
+            w = 0
+            for k in range(per):
+                w |= int(v[b * bs + k * S + j]) << (k * bw)
+            want[b * S + j] = w
+    assert img.tobytes() == want.tobytes()
+    dec, used = oracle.morph_batch_decompress(f, img, 5 * bs)
+    assert used == len(img) and np.array_equal(dec, v)
+
+
+def test_k_wise_ns_closed_form(oracle):
+    """k_wise_ns.h:178-212 with the shuffle tables of :49-115: per pair a
+    descriptor (leading zero bytes of x1 << 3 | of x0) and the low bytes."""
+    v = np.array([0, 1, 255, 256, 2**63, 2**40 + 5, 7, 2**64 - 1], dtype=np.uint64)
+    img = oracle.morph_batch_compress(k_wise_ns_f(2), v)
+    want = bytearray()
+    for a, b in zip(v[0::2].tolist(), v[1::2].tolist()):
+        za, zb = [(64 - (x | 1).bit_length()) // 8 for x in (a, b)]
+        want.append((zb << 3) | za)
+        want += a.to_bytes(8, "little")[:8 - za] + b.to_bytes(8, "little")[:8 - zb]
+    assert img.tobytes() == bytes(want)
+    v = rng_vals(5, 4000, 37) >> (rng_vals(6, 4000, 6) % np.uint64(37))
+    img = oracle.morph_batch_compress(k_wise_ns_f(2), v)
+    dec, used = oracle.morph_batch_decompress(k_wise_ns_f(2), img, 4000)
+    assert used == len(img) and np.array_equal(dec, v)
+
+
+def _gs_reference_model(v, E, base=0):
+    """group_simple.h:246-365 written out independently in Python: the ring
+    buffer of pseudo group maxima, the greedy mode choice, 4-bit selectors,
+    aligned data area, incomplete block."""
+    nums = [64, 32, 21, 16, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1]
+    bits = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 16, 21, 32, 64]
+    G = len(v) // E
+    gmax = [0] * G
+    for g in range(G):
+        for j in range(E):
+            gmax[g] |= int(v[g * E + j])
+    sels, p = [], 0
+    while p < G:
+        rb = gmax[p:p + 64]
+        for i in range(15):
+            n, mask = nums[i], (1 << bits[i]) - 1
+            maxpos = min(n, len(rb))
+            off = 0
+            while off < maxpos and rb[off] <= mask:
+                off += 1
+            if off == maxpos:
+                break
+        if off != n:
+            break
+        sels.append(i)
+        p += n
+    out = bytearray(len(sels).to_bytes(8, "little")) + bytes([G - p])
+    for k in range(0, len(sels), 2):
+        out.append(sels[k] | ((sels[k + 1] << 4) if k + 1 < len(sels) else 0))
+    while (base + len(out)) % (8 * E):
+        out.append(0)
+    q = 0
+    for i in sels:
+        for j in range(E):
+            w = 0
+            for k in range(nums[i]):
+                w |= int(v[(q + k) * E + j]) << (k * bits[i])
+            out += w.to_bytes(8, "little")
+        q += nums[i]
+    if G - q:
+        R = G - q
+        for j in range(E):
+            w = 0
+            for k in range(R):
+                w |= int(v[(q + k) * E + j]) << (k * (64 // R))
+            out += (w & (2**64 - 1)).to_bytes(8, "little")
+    return bytes(out), sels
+
+
+@pytest.mark.parametrize("E", [8, 4])
+@pytest.mark.parametrize("kind", ["narrow", "mixed", "wide", "zeros", "short"])
+def test_group_simple_against_model(oracle, E, kind):
+    n = {"short": 5 * E, "zeros": 300 * E}.get(kind, 2000 * E)
+    if kind == "narrow":
+        v = rng_vals(1, n, 3)
+    elif kind == "mixed":
+        v = rng_vals(2, n, 30) >> (rng_vals(3, n, 5) % np.uint64(30))
+    elif kind == "wide":
+        v = rng_vals(4, n, 64)
+    elif kind == "zeros":
+        v = np.zeros(n, np.uint64)
+    else:
+        v = rng_vals(5, n, 10)
+    f = group_simple_f(E)
+    img = oracle.morph_batch_compress(f, v)
+    want, sels = _gs_reference_model(v, E)
+    assert img.tobytes() == want
+    dec, used = oracle.morph_batch_decompress(f, img, n)
+    assert used == len(img) and np.array_equal(dec, v)
+
+
+@pytest.mark.parametrize("f", [static_vbp_padding_f(7, 8), k_wise_ns_f(2), group_simple_f(8)], ids=repr)
+@pytest.mark.parametrize("n", [0, 1, 167, 4096 + 77])
+def test_extra_formats_column_layout(oracle, f, n):
+    v = rng_vals(n + 9, n, 7)
+    col = oracle.morph(f, v)
+    bs = f.block_size()
+    nc = n // bs * bs
+    main = oracle.morph_batch_compress(f, v[:nc]) if nc else np.zeros(0, np.uint8)
+    if f.kind != 8 or nc:
+        assert col.size_compr == len(main)
+    assert np.array_equal(col.values(), v)
+    if n - nc:
+        assert col.count_uncompr() == n - nc
