@@ -45,6 +45,8 @@ ABI = {
     "ms_col_load": (ctypes.c_int, [_COLP, _FD, ctypes.c_char_p, VP]),
     "ms_col_prepare_for_random_access": (ctypes.c_int, [VP, P(ctypes.c_int), VP]),
     "ms_morph": (ctypes.c_int, [_COLP, _FD, VP, VP]),
+    "ms_morph_blockwise": (ctypes.c_int, [_COLP, _FD, VP, VP, U64, VP]),
+    "ms_col_block_codes": (ctypes.c_int, [VP, VP, U64, P(U64)]),
     "ms_select": (ctypes.c_int, [_COLP, U32, U64, _FD, VP, U32, VP]),
     "ms_between": (ctypes.c_int, [_COLP, U32, U64, U32, U64, _FD, VP, U32, VP]),
     "ms_project": (ctypes.c_int, [_COLP, _FD, VP, VP, U32, VP]),
@@ -174,6 +176,14 @@ class Column:
         check(lib().ms_col_prepare_for_random_access(self.h, ctypes.byref(acted), stream))
         return bool(acted.value)
 
+    def block_codes(self):
+        """blockwise_individual_f: the format code of every block."""
+        n = ctypes.c_uint64()
+        check(lib().ms_col_block_codes(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint8)
+        check(lib().ms_col_block_codes(self.h, out.ctypes.data, n.value, ctypes.byref(n)))
+        return out[:n.value]
+
     @property
     def device_ptr(self):
         return lib().ms_col_device_data(self.h)
@@ -213,6 +223,14 @@ def load(fmt, path, stream=None):
 
 def morph(fmt, col, stream=None):
     return _new(lib().ms_morph, ctypes.byref(fmt), col.h, stream)
+
+
+def morph_blockwise(fmt, col, codes, stream=None):
+    """A blockwise_individual_f column of col's values with the given format
+    code per block (ms_format_code) instead of the format's selector."""
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    return _new(lib().ms_morph_blockwise, ctypes.byref(fmt), col.h, codes.ctypes.data if len(codes) else None,
+                len(codes), stream)
 
 
 def select(cmp, val, out_fmt, col, lanes=8, stream=None):

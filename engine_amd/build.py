@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmorphstore_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["ms_abi.hip", "ms_decode.hip", "ms_encode.hip", "ms_ops.hip", "ms_batch.hip", "ms_learn.hip", "ms_rel.hip", "ms_xfmt.hip"]
+SOURCES = ["ms_abi.hip", "ms_decode.hip", "ms_encode.hip", "ms_ops.hip", "ms_batch.hip", "ms_learn.hip", "ms_rel.hip", "ms_xfmt.hip", "ms_bwi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

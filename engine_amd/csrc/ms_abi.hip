@@ -100,7 +100,7 @@ void dev_free(void *p, hipStream_t s)
 
 void *temps::get(size_t bytes)
 {
-    if (n >= 64) return nullptr;
+    if (n >= (int)(sizeof ptrs / sizeof ptrs[0])) return nullptr;
     void *p = dev_alloc(bytes, s);
     if (!p) return nullptr;
     ptrs[n++] = p;
@@ -152,6 +152,9 @@ static bool valid_format(const ms_format_desc *fp)
     case MS_STATIC_VBP_PAD: return f.bw >= 1 && f.bw <= 64 && f.step >= 1 && f.step <= 64;
     case MS_K_WISE_NS: return f.block == 2;   // the reference implements 2-wise null suppression only
     case MS_GROUP_SIMPLE: return f.step >= 1 && f.step <= 64;
+    case MS_BLOCKWISE_INDIVIDUAL:   // blockwise_individual.h:125-139: a multiple of every default format's block
+        return f.block > 0 && f.block % 1024 == 0 && f.block <= (1u << 20) && f.step == 8 &&
+               f.bw <= MS_BWI_SEL_STATIC && (f.bw != MS_BWI_SEL_STATIC || bwi_code_ok(f.page));
     case MS_DELTA:
     case MS_FOR: {
         ms_format_desc in = {};
@@ -188,6 +191,7 @@ static uint64_t size_max(const ms_format_desc &f, uint64_t n)
     case MS_STATIC_VBP_PAD: return ms_pad_size_max(f.bw, f.step, n);
     case MS_K_WISE_NS: return ms_kwns_size_max(f.block, n);
     case MS_GROUP_SIMPLE: return ms_gs_size_max(f.step, n);
+    case MS_BLOCKWISE_INDIVIDUAL: return ms_bwi_size_max(f.block, n);
     default: (void)g; return 0;
     }
 }
@@ -268,8 +272,15 @@ static uint64_t idx_entries(const ms_geom &g, uint64_t count)
     return 0;
 }
 
-// One allocation: [meta 256 B][index][refs][image]
-static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_alloc, hipStream_t s, ms_col **out)
+// Capacities of a blockwise_individual_f column: its index entries,
+// reference slots and view-table bytes.
+struct col_caps {
+    uint64_t index_cap, refs_cap, tab_bytes;
+};
+
+// One allocation: [meta 256 B][index][refs][view table][image]
+static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_alloc, hipStream_t s, ms_col **out,
+                   const col_caps *caps = nullptr)
 {
     ms_col *c = new (std::nothrow) ms_col();
     if (!c) return ms_fail(MS_ERR_NOMEM, "host allocation");
@@ -277,15 +288,17 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     c->g = ms_make_geom(f);
     c->count_bound = count_bound;
     c->stream = s;
-    c->index_cap = idx_entries(c->g, count_bound);
-    c->refs_cap = f.kind == MS_FOR ? count_bound / f.block + 2
+    c->index_cap = caps ? caps->index_cap : idx_entries(c->g, count_bound);
+    c->refs_cap = caps ? caps->refs_cap
+                : f.kind == MS_FOR ? count_bound / f.block + 2
                 : f.kind == MS_DELTA ? (count_bound / MS_DELTA_TILE + 2) * 8
                 : f.kind == MS_GROUP_SIMPLE ? count_bound / f.step + 2 : 0;
     c->size_alloc = size_alloc;
     const uint64_t idx_bytes = (c->index_cap * 8 + 255) / 256 * 256;
     const uint64_t refs_bytes = (c->refs_cap * 8 + 255) / 256 * 256;
+    const uint64_t tab_bytes = caps ? (caps->tab_bytes + 255) / 256 * 256 : 0;
     const uint64_t data_bytes = (size_alloc + 1024 + 255) / 256 * 256;
-    uint8_t *base = (uint8_t *)dev_alloc(256 + idx_bytes + refs_bytes + data_bytes, s);
+    uint8_t *base = (uint8_t *)dev_alloc(256 + idx_bytes + refs_bytes + tab_bytes + data_bytes, s);
     if (!base) {
         delete c;
         return ms_fail(MS_ERR_NOMEM, "device allocation of %llu bytes", (unsigned long long)(size_alloc + 1024));
@@ -294,7 +307,8 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     c->meta = (ms_dev_meta *)base;
     c->index = c->index_cap ? (uint64_t *)(base + 256) : nullptr;
     c->refs = c->refs_cap ? (uint64_t *)(base + 256 + idx_bytes) : nullptr;
-    c->data = base + 256 + idx_bytes + refs_bytes;
+    c->vmeta = tab_bytes ? (ms_dev_meta *)(base + 256 + idx_bytes + refs_bytes) : nullptr;
+    c->data = base + 256 + idx_bytes + refs_bytes + tab_bytes;
     int rc = launch_fill_u64((uint64_t *)base, 256 / 8, 0, s);
     if (rc) {
         dev_free(base, s);
@@ -444,6 +458,15 @@ static int host_index(const ms_format_desc &f, const ms_geom &g, const uint8_t *
     return host_index_walk(f, g, img, m, idx, refs, walked);
 }
 
+// host_index_walk over one blockwise_individual_f view: `count` values in
+// at most `avail` bytes; *walked = the view's bytes.
+static int view_walk(const ms_format_desc &d, const uint8_t *img, uint64_t count, uint64_t avail,
+                     std::vector<uint64_t> &idx, std::vector<uint64_t> &refs, uint64_t *walked)
+{
+    const ms_col_meta m = {count, avail, avail, avail};
+    return host_index_walk(d, ms_make_geom(d), img, m, idx, refs, walked);
+}
+
 static int check_meta(const ms_format_desc &f, const ms_col_meta &m)
 {
     const ms_geom g = ms_make_geom(f);
@@ -478,6 +501,38 @@ extern "C" int ms_col_upload(ms_col **out, const ms_format_desc *f, const void *
     if (!valid_format(f)) return ms_fail(MS_ERR_INVALID, "invalid format");
     MS_TRY(check_meta(*f, *meta));
     const ms_geom g = ms_make_geom(*f);
+    if (f->kind == MS_BLOCKWISE_INDIVIDUAL) {
+        bwi_host hb;
+        MS_TRY(bwi_walk_image(*f, (const uint8_t *)image, meta->count, meta->size_compr, hb, view_walk));
+        if (hb.h.C != meta->size_compr)
+            return ms_fail(MS_ERR_INVALID, "blockwise_individual_f: sizeCompr %llu, the views end at %llu",
+                           (unsigned long long)meta->size_compr, (unsigned long long)hb.h.C);
+        const uint64_t alloc = std::max(meta->size_used, ms_rest_offset(g, meta->size_compr) + 8 * g.bs) + 64;
+        const col_caps caps = {hb.idx.size() + 1, bwi_refs_entries(hb.h) + 8, bwi_tab_bytes(hb.h.nb)};
+        ms_col *c;
+        MS_TRY(new_col(*f, meta->count, alloc, S(stream), &c, &caps));
+        ms_dev_meta dm = {};
+        dm.count = meta->count;
+        dm.size_used = meta->size_used;
+        dm.size_compr = meta->size_compr;
+        temps sc(S(stream));
+        int rc = MS_OK;
+        if (meta->size_used && hipMemcpyAsync(c->data, image, meta->size_used, hipMemcpyHostToDevice, S(stream)))
+            rc = ms_fail(MS_ERR_HIP, "upload");
+        if (!rc && !hb.idx.empty() &&
+            hipMemcpyAsync(c->index, hb.idx.data(), hb.idx.size() * 8, hipMemcpyHostToDevice, S(stream)))
+            rc = ms_fail(MS_ERR_HIP, "upload");
+        if (!rc && !hb.refs.empty() &&
+            hipMemcpyAsync(c->refs, hb.refs.data(), hb.refs.size() * 8, hipMemcpyHostToDevice, S(stream)))
+            rc = ms_fail(MS_ERR_HIP, "upload");
+        if (!rc && hipMemcpyAsync(c->meta, &dm, sizeof dm, hipMemcpyHostToDevice, S(stream)))
+            rc = ms_fail(MS_ERR_HIP, "upload");
+        if (!rc) rc = bwi_finish_upload(c, hb, sc);
+        if (!rc && hipStreamSynchronize(S(stream))) rc = ms_fail(MS_ERR_HIP, "upload");
+        if (rc) { ms_col_free(c); return rc; }
+        *out = c;
+        return MS_OK;
+    }
     std::vector<uint64_t> idx, refs;
     MS_TRY(host_index(*f, g, (const uint8_t *)image, *meta, idx, refs));
     ms_col *c;
@@ -605,6 +660,7 @@ extern "C" int ms_col_free(ms_col *c)
     } else {
         (void)hipFreeAsync((void *)c->meta, c->stream);
     }
+    delete c->bwi;
     delete c;
     return MS_OK;
 }
@@ -635,6 +691,22 @@ extern "C" int ms_col_prepare_for_random_access(ms_col *c, int *acted, void *str
     temps sc(S(stream));
     uint64_t *rest = (uint64_t *)(c->data + pt.rest_off);
     MS_TRY(launch_fill_u64(rest + pt.n_rest, ru - pt.n_rest, 0, S(stream)));
+    if (c->f.kind == MS_BLOCKWISE_INDIVIDUAL) {
+        // one block, formatted by the column's selector, its code byte at
+        // sizeCompr and its padding aligned to the image (create_aligned_ptr)
+        bwi_work w;
+        MS_TRY(bwi_plan(c->f, rest, ru, nullptr, m.size_compr, w, sc));
+        uint64_t *ti = (uint64_t *)sc.get((bwi_index_entries(w.h) + 1) * 8 + 64);
+        uint64_t *tr = (uint64_t *)sc.get((bwi_refs_entries(w.h) + 8) * 8 + 64);
+        if (!ti || !tr) return ms_fail(MS_ERR_NOMEM, "prepare scratch");
+        if (w.h.C > pt.rest_off) return ms_fail(MS_ERR_CAPACITY, "compressed rest does not fit the gap");
+        MS_TRY(bwi_write_image(w, c->data, pt.rest_off, &c->meta->status, ti, tr, sc));
+        MS_HIP(hipStreamSynchronize(S(stream)));
+        c->prepared = true;
+        c->prepared_bytes = w.h.C - m.size_compr;
+        if (acted) *acted = 1;
+        return MS_OK;
+    }
     uint64_t *d_m = (uint64_t *)sc.get(64);
     if (!d_m) return ms_fail(MS_ERR_NOMEM, "prepare scratch");
     MS_TRY(launch_fill_u64(d_m, 1, ru, S(stream)));
@@ -715,6 +787,10 @@ extern "C" int ms_col_load(ms_col **out, const ms_format_desc *f, const char *pa
         std::vector<uint64_t> idx, refs;
         if (f->kind == MS_STATIC_VBP) {
             end = nc * f->bw / 8;
+        } else if (f->kind == MS_BLOCKWISE_INDIVIDUAL) {
+            bwi_host hb;
+            MS_TRY(bwi_walk_image(*f, img.data(), nc, m.size_used, hb, view_walk));
+            end = hb.h.C;
         } else {
             const ms_col_meta walk = {nc, m.size_used, m.size_used, m.size_used};
             MS_TRY(host_index(*f, g, img.data(), walk, idx, refs, &end));
@@ -842,7 +918,7 @@ static uint64_t wit_chunk(const ms_format_desc &f)
 static bool wit_ok(const ms_format_desc *f)
 {
     if (f->kind == MS_UNCOMPR) return true;
-    if (f->kind == MS_RLE || xfmt(f->kind)) return false;
+    if (f->kind == MS_RLE || xfmt(f->kind) || f->kind == MS_BLOCKWISE_INDIVIDUAL) return false;
     const uint64_t bs = ms_make_geom(*f).bs;
     return wit_chunk(*f) % bs == 0;
 }
@@ -958,7 +1034,40 @@ static ms_format_desc uncompr_desc()
     return f;
 }
 
+static int host_count(const ms_col *c, void *stream, uint64_t *n)
+{
+    ms_col_meta m;
+    MS_TRY(ms_col_get_meta(c, &m, stream));
+    *n = m.count;
+    return MS_OK;
+}
+
 // ------------------------------------------------------------------ morph
+// A blockwise_individual_f column of the n values at d_v, the block
+// formats given by codes (host, one per block) or chosen by f's selector.
+static int bwi_new(const ms_format_desc &f, const uint64_t *d_v, uint64_t n, const uint8_t *codes, temps &sc,
+                   ms_col **out)
+{
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "blockwise_individual_f compression synchronises: not in a plan");
+    bwi_work w;
+    MS_TRY(bwi_plan(f, d_v, n, codes, 0, w, sc));
+    const ms_geom g = ms_make_geom(f);
+    const uint64_t rest = n - w.h.nb * w.h.B;
+    const uint64_t R = ms_align64(w.h.C + g.gap);
+    // room for the rest zero-padded to a block (a preparation for random access)
+    const uint64_t alloc = std::max(rest ? R + 8 * rest : w.h.C, R + 8 * g.bs) + 64;
+    const col_caps caps = {bwi_index_entries(w.h) + 1, bwi_refs_entries(w.h) + 8, bwi_tab_bytes(w.h.nb)};
+    ms_col *c;
+    MS_TRY(new_col(f, n, alloc, sc.s, &c, &caps));
+    int rc = bwi_pack(w, c, sc);
+    if (rc) {
+        ms_col_free(c);
+        return rc;
+    }
+    *out = c;
+    return MS_OK;
+}
+
 extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in, void *stream)
 {
     MS_TRY(ms_need_init());
@@ -967,13 +1076,22 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
     temps sc(S(stream));
     const uint64_t nb = in->count_bound;
     if (memcmp(dst, &in->f, sizeof *dst) == 0 || (dst->kind == MS_UNCOMPR && in->f.kind == MS_UNCOMPR)) {
-        // same format: a copy of the image, meta and index
+        // same format: a copy of the image, meta, index and view table
         ms_col *c;
-        MS_TRY(new_col(in->f, nb, in->size_alloc, S(stream), &c));
+        const col_caps caps = {in->index_cap, in->refs_cap,
+                               in->vmeta ? (uint64_t)(in->data - (uint8_t *)in->vmeta) : 0};
+        MS_TRY(new_col(in->f, nb, in->size_alloc, S(stream), &c, in->bwi ? &caps : nullptr));
         const uint64_t bytes = (uint64_t)(c->data - (uint8_t *)c->meta) + in->size_alloc;
         MS_HIP(hipMemcpyAsync(c->meta, in->meta, bytes, hipMemcpyDeviceToDevice, S(stream)));
         c->prepared = in->prepared;
         c->prepared_bytes = in->prepared_bytes;
+        if (in->bwi) {
+            c->bwi = new (std::nothrow) ms_bwi(*in->bwi);
+            if (!c->bwi) {
+                ms_col_free(c);
+                return ms_fail(MS_ERR_NOMEM, "host allocation");
+            }
+        }
         *out = c;
         return MS_OK;
     }
@@ -989,6 +1107,11 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
     }
     const uint64_t *vals, *d_n;
     MS_TRY(values_of(in, sc, &vals, &d_n));
+    if (dst->kind == MS_BLOCKWISE_INDIVIDUAL) {
+        uint64_t n;
+        MS_TRY(host_count(in, stream, &n));
+        return bwi_new(*dst, vals, n, nullptr, sc, out);
+    }
     if (dst->kind == MS_RLE) {
         MS_TRY(new_col(*dst, nb, 16 * nb + 64, S(stream), &c));
         int rc = launch_rle_compress(c, vals, d_n, nb, sc);
@@ -1000,6 +1123,35 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
     int rc = xfmt(dst->kind) ? launch_xfmt_compress(c, vals, d_n, nb, 0, sc) : launch_emit(c, vals, d_n, nb, 0, nullptr, sc);
     if (rc) { ms_col_free(c); return rc; }
     *out = c;
+    return MS_OK;
+}
+
+extern "C" int ms_morph_blockwise(ms_col **out, const ms_format_desc *dst, const ms_col *in, const uint8_t *codes,
+                                  uint64_t ncodes, void *stream)
+{
+    MS_TRY(ms_need_init());
+    if (!out || !in || (!codes && ncodes)) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (!valid_format(dst) || dst->kind != MS_BLOCKWISE_INDIVIDUAL)
+        return ms_fail(MS_ERR_INVALID, "destination is not a blockwise_individual_f");
+    uint64_t n;
+    MS_TRY(host_count(in, stream, &n));
+    if (ncodes != n / dst->block)
+        return ms_fail(MS_ERR_INVALID, "%llu codes for %llu blocks", (unsigned long long)ncodes,
+                       (unsigned long long)(n / dst->block));
+    temps sc(S(stream));
+    const uint64_t *vals, *d_n;
+    MS_TRY(values_of(in, sc, &vals, &d_n));
+    return bwi_new(*dst, vals, n, codes, sc, out);
+}
+
+extern "C" int ms_col_block_codes(const ms_col *c, uint8_t *codes, uint64_t cap, uint64_t *n)
+{
+    if (!c || !n) return ms_fail(MS_ERR_INVALID, "null argument");
+    if (!c->bwi) return ms_fail(MS_ERR_INVALID, "not a blockwise_individual_f column");
+    const ms_bwi &h = *c->bwi;
+    *n = h.nb;
+    if (codes)
+        for (uint64_t b = 0; b < h.nb && b < cap; b++) codes[b] = (uint8_t)h.v[b].code;
     return MS_OK;
 }
 
@@ -1088,7 +1240,8 @@ extern "C" int ms_project(ms_col **out, const ms_format_desc *out_f, const ms_co
     // extra codecs) is decompressed first
     ms_col tmpdata;
     const ms_col *dsrc = data;
-    if (data->f.kind == MS_DELTA || data->f.kind == MS_RLE || xfmt(data->f.kind)) {
+    if (data->f.kind == MS_DELTA || data->f.kind == MS_RLE || xfmt(data->f.kind) ||
+        data->f.kind == MS_BLOCKWISE_INDIVIDUAL) {
         const uint64_t *dv, *d_dn;
         MS_TRY(values_of(data, sc, &dv, &d_dn));
         ms_dev_meta *dm = (ms_dev_meta *)sc.get(sizeof(ms_dev_meta));
@@ -1147,14 +1300,6 @@ extern "C" int ms_agg_sum(ms_col **out, const ms_col *in, void *stream)
 }
 
 // ------------------------------------------------------------------ group, join
-static int host_count(const ms_col *c, void *stream, uint64_t *n)
-{
-    ms_col_meta m;
-    MS_TRY(ms_col_get_meta(c, &m, stream));
-    *n = m.count;
-    return MS_OK;
-}
-
 extern "C" int ms_group(ms_col **out_gr, ms_col **out_ext, const ms_format_desc *gr_f, const ms_format_desc *ext_f,
                         const ms_col *in_gr, const ms_col *in_data, void *stream)
 {

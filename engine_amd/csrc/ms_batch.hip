@@ -245,6 +245,7 @@ using namespace ms;
 // ------------------------------------------------------------------ host
 struct ms_batch {
     std::vector<ms_bentry> E;
+    std::vector<const ms_col *> cols;   // the entries' columns (blockwise_individual_f views)
     std::vector<ms_col_meta> meta;   // read at the first build
     std::vector<ms_bitem> items;
     ms_batch_config cfg;
@@ -315,6 +316,8 @@ extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uin
         return ms_fail(MS_ERR_INVALID, "no batched decoder for format kind %u: decompress it first", in->g.kind);
     if (in->g.kind == MS_DELTA && !delta_batchable(in->g))
         return ms_fail(MS_ERR_INVALID, "batched delta_f needs step 8 over an 8-lane dynamic_vbp inner format");
+    if (in->g.kind == MS_BLOCKWISE_INDIVIDUAL && (!in->bwi || in->bwi->lcount[BWK_GS]))
+        return ms_fail(MS_ERR_INVALID, "no batched decoder for group_simple_f blocks: decompress the column first");
     ms_bentry e = {};
     MS_TRY(colv_of(in, e.c));
     e.p = make_pred(cmp, val);
@@ -322,6 +325,7 @@ extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uin
     e.tab = make_bwtab(e.p);
     if (entry) *entry = b->E.size();
     b->E.push_back(e);
+    b->cols.push_back(in);
     b->built = false;
     b->have_meta = false;
     return MS_OK;
@@ -375,6 +379,44 @@ static void make_queue(ms_batch *b)
         proto.aux = g.bw;
         proto.ulim = pt.n_compr / 512;   // BI_DELTA: units of the compressed part
         switch (g.kind) {
+        case MS_BLOCKWISE_INDIVIDUAL: {
+            // every block's view as items of its own layout
+            const ms_col *c = b->cols[e];
+            const ms_bwi &h = *c->bwi;
+            const uint64_t B = h.B, T = (B + MS_DELTA_TILE - 1) / MS_DELTA_TILE;
+            for (uint64_t v = 0; v < h.nb; v++) {
+                const ms_bwi_view &w = h.v[v];
+                const ms_bitem keep = proto;
+                if (w.code == 0) {
+                    proto.data = c->data + w.off;
+                    add(e, BI_RAW, 0, B, cf.values_per_item);
+                } else if (w.code <= 64) {
+                    proto.data = c->data + w.off;
+                    proto.aux = w.code;
+                    add(e, BI_STATIC, 0, B / 512, cf.units_per_item);
+                } else if (w.code == MS_BWI_RLE) {
+                    proto.data = c->data + w.off;
+                    add(e, BI_RLE, 0, w.size / 16, cf.runs_per_item);
+                } else {
+                    const int k = w.code == MS_BWI_DYNAMIC_VBP ? BWK_DVBP
+                                : w.code == MS_BWI_DELTA_DYNAMIC_VBP ? BWK_DELTA : BWK_FOR;
+                    proto.data = c->data;   // absolute index entries
+                    proto.index = c->index + h.idx_off[k] + (uint64_t)w.slot * (B / 512);
+                    proto.lg_spb = 0;
+                    proto.lg_ipo = 1;
+                    proto.ulim = B / 512;
+                    if (k == BWK_DELTA) {
+                        proto.refs = c->refs + h.refs_off[k] + (uint64_t)w.slot * T * 8;
+                        add(e, BI_DELTA, 0, (B / 512 + 3) / 4, cf.tiles_per_item);
+                    } else {
+                        proto.refs = k == BWK_FOR ? c->refs + h.refs_off[k] + (uint64_t)w.slot * (B / 1024) : nullptr;
+                        add(e, k == BWK_FOR ? BI_FOR : BI_BLOCKS, 0, B / 512, cf.units_per_item);
+                    }
+                }
+                proto = keep;
+            }
+            break;
+        }
         case MS_RLE:
             add(e, BI_RLE, 0, m.size_used / 16, cf.runs_per_item);
             break;

@@ -697,9 +697,12 @@ static uint32_t stride_grid(const void *kernel, uint64_t items)
 int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s,
                       const ss_buffers *bufs)
 {
-    (void)accumulate;
     ms_colv v;
     MS_TRY(colv_of(in, v));
+    if (v.g.kind == MS_BLOCKWISE_INDIVIDUAL) {
+        if (bufs) return ms_fail(MS_ERR_INVALID, "blockwise_individual_f: select_sum needs temporaries of its own");
+        return launch_bwi_select_sum(in, p, d_result, accumulate, s);
+    }
     temps sc(s);
     const uint64_t nb = in->count_bound;
     uint32_t grid = 1;
@@ -858,6 +861,8 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
     case MS_GROUP_SIMPLE:
         MS_TRY(launch_xfmt_decompress(in, d_out, s));
         break;
+    case MS_BLOCKWISE_INDIVIDUAL:
+        return launch_bwi_decompress(in, d_out, s);   // copies its rest itself
     default:
         return ms_fail(MS_ERR_INVALID, "decompress: unsupported format");
     }

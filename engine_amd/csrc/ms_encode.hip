@@ -42,6 +42,11 @@ struct emit_layout {
     uint64_t gap;        // rest gap
     uint64_t cap;        // bytes of the output image
     uint64_t *status;    // the output's status word (MS_ERR_CAPACITY past cap)
+    // Chunked mode (both null otherwise): every K values are one morph_batch
+    // call over the values v[cmap[c] * K ...] (its delta predecessors start
+    // at zero), written at cout[c] of the image.
+    const uint32_t *cmap;
+    const uint64_t *cout;
 };
 
 struct emit_plan {
@@ -92,6 +97,26 @@ __device__ __forceinline__ uint64_t hdr_of(const emit_layout &l, uint64_t b)
     return h;
 }
 
+// The values of the call holding value `first`: returns the array they are
+// indexed in and sets `lo` to first's index in it (delta predecessors exist
+// from index dstep of that array on).
+__device__ __forceinline__ const uint64_t *chunk_src(const emit_layout &l, const uint64_t *v, uint64_t first,
+                                                     uint64_t &lo)
+{
+    if (!l.cmap) { lo = first; return v; }
+    const uint64_t c = first / l.K;
+    lo = first - c * l.K;
+    return v + (uint64_t)l.cmap[c] * l.K;
+}
+
+// Image offset of innermost block b's headers.
+__device__ __forceinline__ uint64_t out_pos(const emit_layout &l, const uint64_t *start, uint64_t b)
+{
+    if (!l.cout) return start[b];
+    const uint64_t bpc = l.K / l.B, c = b / bpc;
+    return l.cout[c] + start[b] - start[c * bpc];
+}
+
 __device__ __forceinline__ uint64_t xform(const emit_layout &l, const uint64_t *v, uint64_t gi, uint64_t ref)
 {
     const uint64_t x = v[gi];
@@ -106,7 +131,8 @@ __global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
     for (uint64_t fb = blockIdx.x; fb < pl.nfb; fb += gridDim.x) {
-        const uint64_t *x = v + fb * l.L;
+        uint64_t lo;
+        const uint64_t *x = chunk_src(l, v, fb * l.L, lo) + lo;
         uint64_t r;
         if (l.E == 1) {
             r = ~0ull;
@@ -148,8 +174,10 @@ __global__ __launch_bounds__(64) void k_block_sizes(emit_layout l, const uint64_
         uint32_t bw = l.bw;
         if (l.leaf == MS_DYNAMIC_VBP) {
             const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
+            uint64_t lo;
+            const uint64_t *src = chunk_src(l, v, b * l.B, lo);
             uint64_t o = 1;
-            for (uint64_t i = t; i < l.B; i += 64) o |= xform(l, v, b * l.B + i, ref);
+            for (uint64_t i = t; i < l.B; i += 64) o |= xform(l, src, lo + i, ref);
             o = wave_or_u64(o);
             bw = 64 - __clzll(o);
         }
@@ -176,7 +204,9 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
     for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
         const uint32_t bw = bws[b];
         const uint64_t h = hdr_of(l, b);
-        const uint64_t off = start[b] + h;
+        const uint64_t off = out_pos(l, start, b) + h;
+        uint64_t lo;
+        const uint64_t *src = chunk_src(l, v, b * l.B, lo);
         const uint64_t fb = b / l.bpo;
         const uint64_t ref = l.kind == MS_FOR ? refs[fb] : 0;
         const uint64_t words = l.B * bw / 64;
@@ -191,7 +221,7 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
             // Thread (j, g) owns codes g*C .. g*C+C-1 of lane j: consecutive
             // codes share words, so it ORs them in registers and issues one
             // LDS atomic per word it touches instead of one per code.
-            const uint64_t base = b * l.B + j;
+            const uint64_t base = lo + j;
             uint32_t k = g * C, bit = k * bw, cw = bit >> 6;
             uint64_t acc = 0;
             // A delta step of one lane width makes each code's predecessor
@@ -200,14 +230,14 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
             uint64_t prev = 0;
             if (own_pred) {
                 const uint64_t g0 = base + (uint64_t)k * S;
-                prev = g0 >= S ? v[g0 - S] : 0ull;
+                prev = g0 >= S ? src[g0 - S] : 0ull;
             }
             for (uint32_t c0 = 0; c0 < C; c0 += 8) {
                 uint64_t x[8];
 #pragma unroll
                 for (uint32_t q = 0; q < 8; ++q) {
                     const uint64_t gi = base + (uint64_t)(k + q) * S;
-                    x[q] = c0 + q < C ? (own_pred ? v[gi] : xform(l, v, gi, ref)) : 0ull;
+                    x[q] = c0 + q < C ? (own_pred ? src[gi] : xform(l, src, gi, ref)) : 0ull;
                 }
                 if (own_pred) {
 #pragma unroll
@@ -239,7 +269,7 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
             atomicOr(&lds[cw * S + j], (unsigned long long)acc);
         } else {
             for (uint64_t i = t; i < l.B; i += 64) {
-                const uint64_t c = xform(l, v, b * l.B + i, ref);
+                const uint64_t c = xform(l, src, lo + i, ref);
                 const uint64_t j = i % l.S, k = i / l.S;
                 const uint64_t bit = k * bw;
                 const uint64_t w = bit >> 6;
@@ -270,8 +300,8 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
                 q = b % l.bpo;
                 call_end = (b / l.bpo + 1) * l.bpo;
             }
-            const uint64_t bpf = b - q % l.P;                       // first block of the page
-            const uint64_t meta = start[bpf] + hdr_of(l, bpf) - l.P; // meta bytes end the header
+            const uint64_t bpf = b - q % l.P;                                 // first block of the page
+            const uint64_t meta = out_pos(l, start, bpf) + hdr_of(l, bpf) - l.P; // meta bytes end the header
             if (t == 0) out[meta + q % l.P] = (uint8_t)bw;
             if (b == bpf) {
                 const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
@@ -287,7 +317,7 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
                 uint64_t fcall_end = l.K ? (fb / fpc + 1) * fpc : pl.nfb;
                 if (fcall_end > pl.nfb) fcall_end = pl.nfb;
                 const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
-                uint64_t *rp = (uint64_t *)(out + start[b]);
+                uint64_t *rp = (uint64_t *)(out + out_pos(l, start, b));
                 for (uint64_t x = t; x < l.FP; x += 64)
                     rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
             }
@@ -435,6 +465,95 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
                        (const ms_dev_meta *)out->meta, out->data);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------
+// Chunked emission: one morph_batch call per chunk of K values.
+// ------------------------------------------------------------------
+static emit_layout chunk_layout(const chunk_emit &ce)
+{
+    const ms_format_desc &f = ce.f;
+    const ms_geom g = ms_make_geom(f);
+    emit_layout l = {};
+    l.kind = f.kind;
+    l.leaf = g.leaf;
+    l.bw = g.bw;
+    l.S = g.lanes;
+    l.dstep = g.dstep;
+    l.B = g.inner_bs;
+    l.P = g.leaf_page;
+    l.bs = g.bs;
+    l.K = ce.K;
+    l.cap = ~0ull;
+    if (f.kind == MS_DELTA || f.kind == MS_FOR) {
+        l.L = g.outer_block;
+        l.FP = g.outer_page;
+        l.bpo = g.outer_block / g.inner_bs;
+        l.E = f.kind == MS_FOR ? (f.step ? f.step : g.lanes) : 1;
+    } else {
+        l.L = l.B;
+        l.bpo = 1;
+    }
+    l.cmap = ce.cmap;
+    return l;
+}
+
+__global__ __launch_bounds__(WG) void k_chunk_bytes(const uint64_t *start, uint64_t bpc, uint64_t n, uint64_t *csz)
+{
+    for (uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x; c < n; c += (uint64_t)gridDim.x * WG)
+        csz[c] = start[(c + 1) * bpc] - start[c * bpc];
+}
+
+int launch_chunk_sizes(chunk_emit &ce, uint64_t *d_csz, temps &sc)
+{
+    hipStream_t s = sc.s;
+    const emit_layout l = chunk_layout(ce);
+    const ms_geom g = ms_make_geom(ce.f);
+    if (g.leaf != MS_DYNAMIC_VBP || (ce.f.kind != MS_DYNAMIC_VBP && ce.f.kind != MS_DELTA && ce.f.kind != MS_FOR))
+        return ms_fail(MS_ERR_INVALID, "chunked emission: dynamic_vbp_f or a cascade over it");
+    if (ce.K == 0 || ce.K % l.L || ce.K % l.B || l.B * 8 > 65536)
+        return ms_fail(MS_ERR_INVALID, "chunked emission: chunk of %llu values", (unsigned long long)ce.K);
+    const uint64_t m = ce.nchunks * ce.K;
+    ce.nb = m / l.B;
+    ce.d_m = (uint64_t *)sc.get(64);
+    ce.sz = (uint64_t *)sc.get((ce.nb + 1) * 8 + 64);
+    ce.start = (uint64_t *)sc.get((ce.nb + 1) * 8 + 64);
+    ce.bws = (uint32_t *)sc.get((ce.nb + 1) * 4 + 64);
+    ce.refs = nullptr;
+    if (!ce.d_m || !ce.sz || !ce.start || !ce.bws) return ms_fail(MS_ERR_NOMEM, "chunked emission scratch");
+    MS_TRY(launch_fill_u64(ce.d_m, 1, m, s));
+    if (ce.nchunks == 0) return MS_OK;
+    if (ce.f.kind == MS_FOR) {
+        ce.refs = (uint64_t *)sc.get((m / l.L + 1) * 8 + 64);
+        if (!ce.refs) return ms_fail(MS_ERR_NOMEM, "chunked emission scratch");
+        hipLaunchKernelGGL(k_for_refs, dim3(grid_for(m / l.L, 1, 65536)), dim3(64), 0, s, l, ce.v, ce.d_m, ce.refs);
+        MS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(ce.nb + 1, 1, 65536)), dim3(64), 0, s, l, ce.v, ce.d_m,
+                       (const uint64_t *)ce.refs, ce.nb + 1, ce.sz, ce.bws);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(ce.sz, ce.start, ce.nb + 1, sc));
+    if (d_csz) {
+        hipLaunchKernelGGL(k_chunk_bytes, dim3(grid_for(ce.nchunks, WG)), dim3(WG), 0, s, (const uint64_t *)ce.start,
+                           ce.K / l.B, ce.nchunks, d_csz);
+        MS_HIP(hipGetLastError());
+    }
+    return MS_OK;
+}
+
+int launch_chunk_pack(const chunk_emit &ce, const uint64_t *d_cout, uint8_t *out, uint64_t cap, uint64_t *status,
+                      uint64_t *index_out, uint64_t *refs_out, hipStream_t s)
+{
+    if (ce.nchunks == 0) return MS_OK;
+    emit_layout l = chunk_layout(ce);
+    l.cout = d_cout;
+    l.cap = cap;
+    l.status = status;
+    hipLaunchKernelGGL(k_pack_blocks, dim3(grid_for(ce.nb, 1, 65536)), dim3(64), l.B * 8, s, l, ce.v,
+                       (const uint64_t *)ce.d_m, (const uint64_t *)ce.refs, (const uint64_t *)ce.start,
+                       (const uint32_t *)ce.bws, out, index_out, refs_out);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

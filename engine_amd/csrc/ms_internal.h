@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/morphstore_hip.h"
 
 #define MS_HD __host__ __device__ __forceinline__
@@ -89,6 +91,28 @@ MS_HD uint64_t ms_gs_size_max(uint32_t E, uint64_t n)
 // Values of a k_wise_ns_f column per block-index entry (32 pairs).
 #define MS_KWNS_PER_ENTRY 64ull
 
+// blockwise_individual_helper::get_size_max_byte_single_block
+// (blockwise_individual.h:57-79) over the default formats that
+// default_formats.h defines (uncompr_f, static_vbp_f<vbp_l<64, 8>>,
+// dynamic_vbp_f<512, 64, 8>, group_simple_f<8, uint64_t, 64>, delta_f and
+// for_f over that dynamic_vbp_f), for B a multiple of 1024.
+MS_HD uint64_t ms_bwi_single_max(uint64_t B)
+{
+    uint64_t m = 8 * B;
+    const uint64_t dvbp = ms_leaf_size_max(MS_DYNAMIC_VBP, 0, 512, 64, B);
+    const uint64_t gs = ms_gs_size_max(8, B);
+    const uint64_t forf = (B + 8 * 1024 - 1) / (8 * 1024) * 64 + B / 1024 * ms_leaf_size_max(MS_DYNAMIC_VBP, 0, 512, 64, 1024);
+    m = dvbp > m ? dvbp : m;
+    m = gs > m ? gs : m;
+    m = forf > m ? forf : m;
+    return m;
+}
+
+// blockwise_individual_f::get_size_max_byte (blockwise_individual.h:141-153):
+// per block the largest single block plus the 64-byte vector of code and
+// padding.
+MS_HD uint64_t ms_bwi_size_max(uint64_t B, uint64_t n) { return n / B * (ms_bwi_single_max(B) + 64); }
+
 MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
 {
     ms_geom g = {};
@@ -135,6 +159,10 @@ MS_HD ms_geom ms_make_geom(const ms_format_desc &f)
     case MS_GROUP_SIMPLE:
         g.leaf = MS_GROUP_SIMPLE; g.lanes = f.step; g.bs = f.step; g.inner_bs = f.step;
         g.gap = 2 * ms_gs_size_max(f.step, f.step);
+        break;
+    case MS_BLOCKWISE_INDIVIDUAL:
+        g.leaf = MS_BLOCKWISE_INDIVIDUAL; g.lanes = 8; g.bs = f.block; g.inner_bs = f.block;
+        g.gap = 2 * ms_bwi_size_max(f.block, f.block);
         break;
     default:   // MS_RLE: no block size, no rest
         g.leaf = MS_RLE; g.lanes = 1; g.bs = 0; g.inner_bs = 0; g.gap = 0;
@@ -191,6 +219,26 @@ MS_HD ms_parts ms_parts_of(const ms_geom &g, uint64_t count, uint64_t size_used,
     return p;
 }
 
+// blockwise_individual_f columns: where each block's view lies.  Views
+// are listed per kind (codes 65, 67, 69, 66, 71, then uncompr_f and
+// static_vbp_f); the block index entries, for_f references and delta_f
+// carries of codes 65, 67 and 69 are stored kind after kind, in list
+// order, so that the views of one of these codes read as one virtual
+// column.  `slot` is a view's position in its kind's list.
+enum { BWK_DVBP = 0, BWK_DELTA = 1, BWK_FOR = 2, BWK_GS = 3, BWK_RLE = 4, BWK_PLAIN = 5, BWK_N = 6 };
+struct ms_bwi_view {
+    uint64_t off;     // payload offset in the image (64-byte aligned)
+    uint64_t size;    // payload bytes
+    uint32_t code;    // ms_format_code
+    uint32_t slot;
+};
+struct ms_bwi {
+    uint64_t n, nb, B, C;          // values, blocks, block size, sizeCompr
+    std::vector<ms_bwi_view> v;
+    uint64_t lcount[BWK_N], loff[BWK_N];
+    uint64_t idx_off[3], refs_off[3];
+};
+
 // Host-side handle behind ms_col*.
 struct ms_col {
     ms_format_desc f;
@@ -207,4 +255,6 @@ struct ms_col {
     struct ms_plan *arena;   // non-null: memory belongs to this plan's arena
     uint64_t prepared_bytes; // prepared for random access: compressed rest bytes at sizeCompr
     bool prepared;
+    ms_bwi *bwi;             // blockwise_individual_f: the views (host)
+    ms_dev_meta *vmeta;      // blockwise_individual_f: meta of every view, then of the 3 groups
 };

@@ -4,6 +4,8 @@
 // the grids.
 #pragma once
 
+#include <vector>
+
 #include "ms_internal.h"
 #include "ms_device.hpp"
 
@@ -15,7 +17,7 @@ struct temps;   // stream-ordered temporaries of one operator call
 // the call on the call's stream (hipFreeAsync keeps the ordering).
 struct temps {
     hipStream_t s;
-    void *ptrs[64];
+    void *ptrs[192];
     int n = 0;
     explicit temps(hipStream_t st) : s(st) {}
     void *get(size_t bytes);
@@ -74,6 +76,81 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
                 uint64_t *d_append, temps &sc);
 // rle_f compression of d_v[0 .. *d_m)
 int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc);
+
+// One morph_batch call of format f (dynamic_vbp_f, or delta_f / for_f over
+// it) per chunk of K values; chunk c is v[cmap[c] * K .. + K) and its
+// delta predecessors start at zero.  launch_chunk_sizes fills the state
+// (temporaries of sc) and each chunk's bytes into d_csz (may be null);
+// launch_chunk_pack then writes chunk c at image offset d_cout[c], its
+// block index entries (absolute offsets) to index_out and for_f
+// references to refs_out, both in chunk order.
+struct chunk_emit {
+    ms_format_desc f;
+    const uint64_t *v;
+    const uint32_t *cmap;
+    uint64_t nchunks, K;
+    uint64_t *d_m, *sz, *start, *refs;
+    uint32_t *bws;   // bit width per innermost block
+    uint64_t nb;     // innermost blocks
+};
+int launch_chunk_sizes(chunk_emit &ce, uint64_t *d_csz, temps &sc);
+int launch_chunk_pack(const chunk_emit &ce, const uint64_t *d_cout, uint8_t *out, uint64_t cap, uint64_t *status,
+                      uint64_t *index_out, uint64_t *refs_out, hipStream_t s);
+
+// blockwise_individual_f (ms_bwi.hip) ------------------------------
+// The default format of a format code; false for codes without one.
+bool bwi_inner(uint32_t code, ms_format_desc &d);
+bool bwi_code_ok(uint32_t code);
+// The device view table of a column (inside its allocation, c->vmeta on):
+// meta of every view and of the 3 groups, then payload offsets, sizes,
+// the per-kind lists of block numbers and the codes.
+struct bwi_dev_tab {
+    ms_dev_meta *meta;
+    uint64_t *q, *psz;
+    uint32_t *lists;
+    uint8_t *codes;
+};
+uint64_t bwi_tab_bytes(uint64_t nb);
+bwi_dev_tab bwi_tab(const ms_col *c);
+uint64_t bwi_index_entries(const ms_bwi &h);
+uint64_t bwi_refs_entries(const ms_bwi &h);
+// Compression in two steps around the allocation of the output: bwi_plan
+// fixes the codes (given, or chosen by the format's selector), the views'
+// sizes and offsets (host copies in w.h); bwi_pack writes the column.
+struct bwi_work {
+    const uint64_t *v;
+    ms_bwi h;
+    std::vector<uint8_t> codes;
+    std::vector<uint32_t> lists;
+    uint8_t *d_codes;
+    uint32_t *d_lists;
+    uint64_t *d_psz, *d_q;
+    chunk_emit ce[3];
+    uint8_t *gbw;
+    uint64_t *gs_cs;
+    uint64_t base0;   // image offset of the first code byte
+};
+int bwi_plan(const ms_format_desc &f, const uint64_t *d_v, uint64_t n, const uint8_t *h_codes, uint64_t base0,
+             bwi_work &w, temps &sc);
+int bwi_pack(bwi_work &w, ms_col *c, temps &sc);
+// Only the image bytes [base0, w.h.C) (a preparation for random access).
+int bwi_write_image(bwi_work &w, uint8_t *img, uint64_t cap, uint64_t *status, uint64_t *index_out,
+                    uint64_t *refs_out, temps &sc);
+// Upload of a host image: the walk (view_walk: the block index walk of one
+// view's format over `count` values in `avail` bytes, reporting its bytes),
+// then the device table and the delta carries.
+struct bwi_host {
+    ms_bwi h;
+    std::vector<uint8_t> codes;
+    std::vector<uint32_t> lists;
+    std::vector<uint64_t> idx, refs;
+};
+int bwi_walk_image(const ms_format_desc &f, const uint8_t *img, uint64_t count, uint64_t size_compr, bwi_host &out,
+                   int (*view_walk)(const ms_format_desc &, const uint8_t *, uint64_t, uint64_t,
+                                    std::vector<uint64_t> &, std::vector<uint64_t> &, uint64_t *));
+int bwi_finish_upload(ms_col *c, bwi_host &hb, temps &sc);
+int launch_bwi_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
+int launch_bwi_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s);
 
 // static_vbp<vbp_padding_l>, k_wise_ns, group_simple (ms_xfmt.hip): the
 // column-level morph of d_v[0 .. *d_m) into `out` (the call starts base0

@@ -8,10 +8,18 @@ Here the same arguments fill an ``ms_format_desc`` (include/morphstore_hip.h).
 import ctypes
 
 (MS_UNCOMPR, MS_STATIC_VBP, MS_DYNAMIC_VBP, MS_DELTA, MS_FOR, MS_RLE, MS_STATIC_VBP_PAD, MS_K_WISE_NS,
- MS_GROUP_SIMPLE) = range(9)
+ MS_GROUP_SIMPLE, MS_BLOCKWISE_INDIVIDUAL) = range(10)
 KIND_NAMES = {MS_UNCOMPR: "uncompr_f", MS_STATIC_VBP: "static_vbp_f", MS_DYNAMIC_VBP: "dynamic_vbp_f",
               MS_DELTA: "delta_f", MS_FOR: "for_f", MS_RLE: "rle_f", MS_STATIC_VBP_PAD: "static_vbp_f",
-              MS_K_WISE_NS: "k_wise_ns_f", MS_GROUP_SIMPLE: "group_simple_f"}
+              MS_K_WISE_NS: "k_wise_ns_f", MS_GROUP_SIMPLE: "group_simple_f",
+              MS_BLOCKWISE_INDIVIDUAL: "blockwise_individual_f"}
+
+# format_code (include/core/format_selection/format_selector_commons.h:25-103);
+# 1..64 are static_vbp_f at that bit width.  71 (rle_f pairs) is this
+# library's own code.
+(MS_BWI_UNCOMPR, MS_BWI_DYNAMIC_VBP, MS_BWI_GROUP_SIMPLE, MS_BWI_DELTA_DYNAMIC_VBP, MS_BWI_DELTA_GROUP_SIMPLE,
+ MS_BWI_FOR_DYNAMIC_VBP, MS_BWI_FOR_GROUP_SIMPLE, MS_BWI_RLE) = (0, 65, 66, 67, 68, 69, 70, 71)
+MS_BWI_SEL_BEST, MS_BWI_SEL_WORST, MS_BWI_SEL_STATIC = range(3)
 
 MS_CMP_EQ, MS_CMP_NE, MS_CMP_LT, MS_CMP_LE, MS_CMP_GT, MS_CMP_GE = range(6)
 (MS_OP_ADD, MS_OP_SUB, MS_OP_MUL, MS_OP_DIV, MS_OP_MOD, MS_OP_MIN, MS_OP_MAX,
@@ -42,6 +50,10 @@ class FormatDesc(ctypes.Structure):
             return f"k_wise_ns_f<{self.block}>"
         if k == MS_GROUP_SIMPLE:
             return f"group_simple_f<{self.step},uint64_t,{8 * self.step}>"
+        if k == MS_BLOCKWISE_INDIVIDUAL:
+            sel = ("exhaustive_size_format_selector_helper<true>", "exhaustive_size_format_selector_helper<false>",
+                   f"static_format_selector_helper<{self.page}>")[self.bw] if self.bw <= 2 else f"selector{self.bw}"
+            return f"blockwise_individual_f<{self.block},{sel}::selector>"
         inner = FormatDesc(self.inner_kind, self.inner_bw, self.inner_block, self.inner_page,
                            self.inner_step).name()
         if k == MS_DELTA:
@@ -97,6 +109,12 @@ def static_vbp_padding_f(bw, step):
 def k_wise_ns_f(block=2):
     """k_wise_ns_f<block> (the reference implements block 2)."""
     return FormatDesc(MS_K_WISE_NS, 0, block, 0, 0)
+
+
+def blockwise_individual_f(block, selector=MS_BWI_SEL_BEST, code=0):
+    """blockwise_individual_f<block, selector, ve> with the 8-lane processing
+    style's default formats; `code` is the fixed choice of MS_BWI_SEL_STATIC."""
+    return FormatDesc(MS_BLOCKWISE_INDIVIDUAL, selector, block, code if selector == MS_BWI_SEL_STATIC else 0, 8)
 
 
 def group_simple_f(lanes=8):

@@ -60,7 +60,33 @@ enum ms_format_kind {
     MS_RLE = 5,          /* rle_f                           rle.h:42-49           */
     MS_STATIC_VBP_PAD = 6, /* static_vbp_f<vbp_padding_l<bw,step>> vbp_padding.h:52-66 */
     MS_K_WISE_NS = 7,    /* k_wise_ns_f<block>              k_wise_ns.h:43-135    */
-    MS_GROUP_SIMPLE = 8  /* group_simple_f<step, uint64_t, 8*step> group_simple.h:64-131 */
+    MS_GROUP_SIMPLE = 8, /* group_simple_f<step, uint64_t, 8*step> group_simple.h:64-131 */
+    MS_BLOCKWISE_INDIVIDUAL = 9 /* blockwise_individual_f<block, selector, ve> blockwise_individual.h:113-158 */
+};
+
+/* format_code (format_selector_commons.h:25-103): the format of one block
+ * of a blockwise_individual_f column.  Codes 1..64 are static_vbp_f at that
+ * bit width.  Codes 68 and 70 name DEFAULT_DELTA_GROUP_SIMPLE_F and
+ * DEFAULT_FOR_GROUP_SIMPLE_F, which the reference's default_formats.h does
+ * not define, so their layout is unknown: they are refused.  Code 71 is
+ * this library's own: rle_f pairs of the block, delimited by their
+ * lengths; the reference has no batch-level rle_f and no such code. */
+enum ms_format_code {
+    MS_BWI_UNCOMPR = 0,
+    MS_BWI_DYNAMIC_VBP = 65,          /* dynamic_vbp_f<512, 64, 8>                    */
+    MS_BWI_GROUP_SIMPLE = 66,         /* group_simple_f<8, uint64_t, 64>              */
+    MS_BWI_DELTA_DYNAMIC_VBP = 67,    /* delta_f<1024, 8, dynamic_vbp_f<512, 64, 8>>  */
+    MS_BWI_DELTA_GROUP_SIMPLE = 68,   /* refused                                      */
+    MS_BWI_FOR_DYNAMIC_VBP = 69,      /* for_f<1024, 8, dynamic_vbp_f<512, 64, 8>>    */
+    MS_BWI_FOR_GROUP_SIMPLE = 70,     /* refused                                      */
+    MS_BWI_RLE = 71                   /* rle_f pairs (extension)                      */
+};
+
+/* Format selectors of blockwise_individual_f (the `bw` field). */
+enum ms_format_selector {
+    MS_BWI_SEL_BEST = 0,    /* exhaustive_size_format_selector_helper<true>  (exhaustive_size_format_selector.h:33-141) */
+    MS_BWI_SEL_WORST = 1,   /* exhaustive_size_format_selector_helper<false> */
+    MS_BWI_SEL_STATIC = 2   /* static_format_selector_helper<page>           (dummy_format_selectors.h:20-39) */
 };
 
 /* Meta byte of a non-existent block in the last page of a dynamic_vbp
@@ -70,6 +96,11 @@ enum ms_format_kind {
 /*
  * A format descriptor is the run-time form of the reference's format
  * template arguments.
+ *   blockwise_individual_f<block, selector, ve> (MS_BLOCKWISE_INDIVIDUAL):
+ *       block (a multiple of 1024, at most 2^20), bw = the selector
+ *       (ms_format_selector), page = the code of MS_BWI_SEL_STATIC, step = 8
+ *       (the 8-lane processing style whose default formats the codes name;
+ *       blocks are aligned to its 64-byte vectors)
  *   static_vbp_f<vbp_l<bw,step>>     : bw, step
  *   static_vbp_f<vbp_padding_l<bw,step>> (MS_STATIC_VBP_PAD): bw, step; a
  *       block is 64/bw codes per lane, one 64-bit word per lane
@@ -223,6 +254,19 @@ int ms_col_free(ms_col *c);
  * nothing with `in` (the reference returns `in` itself). */
 int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *in,
              void *stream);
+
+/* morph_batch_t<ve, blockwise_individual_f<...>, uncompr_f>
+ * (blockwise_individual.h:185-322) at column level with the format codes
+ * given instead of chosen by the selector: codes[i] (ms_format_code) for
+ * block i, ncodes = floor(count / block).  `dst` must be a
+ * blockwise_individual_f.  The learned format selection feeds its
+ * per-block picks through this.  Synchronises `stream`. */
+int ms_morph_blockwise(ms_col **out, const ms_format_desc *dst, const ms_col *in, const uint8_t *codes,
+                       uint64_t ncodes, void *stream);
+
+/* The format codes of a blockwise_individual_f column, one per block, into
+ * codes[0 .. cap); *n = the number of blocks. */
+int ms_col_block_codes(const ms_col *c, uint8_t *codes, uint64_t cap, uint64_t *n);
 
 /* ------------------------------------------------------------------ */
 /* Operators                                                           */

@@ -22,6 +22,7 @@
 unsigned oc_effective_bitwidth(uint64_t v) { return 64u - (unsigned)__builtin_clzll(v | 1u); }
 
 /* math.h:146-153 */
+static uint64_t bwi_single_max(uint64_t B);
 static uint64_t bw_max(unsigned bw) { return bw >= 64 ? U64MAX : ((1ull << bw) - 1ull); }
 
 /* create_aligned_ptr with 64-byte alignment, relative to an aligned base. */
@@ -82,6 +83,11 @@ static int valid(const ms_format_desc *f)
         return f->block == 2;
     case MS_GROUP_SIMPLE:     /* group_simple.h:64-131 with uint64_t, 8 E-byte alignment */
         return f->step >= 1 && f->step <= 64;
+    case MS_BLOCKWISE_INDIVIDUAL: {   /* blockwise_individual.h:125-139 */
+        ms_format_desc d;
+        return f->block > 0 && f->block % 1024 == 0 && f->block <= (1u << 20) && f->step == 8 &&
+               f->bw <= MS_BWI_SEL_STATIC && (f->bw != MS_BWI_SEL_STATIC || oc_bwi_inner(f->page, &d) == MS_OK);
+    }
     case MS_DELTA:
     case MS_FOR:
         inner_of(f, &in);
@@ -114,6 +120,7 @@ int oc_block_size(const ms_format_desc *f, uint64_t *out)
     case MS_STATIC_VBP_PAD: *out = (uint64_t)(64 / f->bw) * f->step; return MS_OK;   /* vbp_padding.h:64-66 */
     case MS_K_WISE_NS: *out = f->block; return MS_OK;
     case MS_GROUP_SIMPLE: *out = f->step; return MS_OK;                              /* t_GroupSizeLog */
+    case MS_BLOCKWISE_INDIVIDUAL: *out = f->block; return MS_OK;                     /* blockwise_individual.h:155 */
     default: return MS_ERR_INVALID;                              /* rle_f has no m_BlockSize */
     }
 }
@@ -147,6 +154,9 @@ int oc_size_max_byte(const ms_format_desc *f, uint64_t n, uint64_t *out)
         *out = calls * 9 + round_up(sels, 2) + calls * (8ull * f->step - 1) + 8 * n;
         return MS_OK;
     }
+    case MS_BLOCKWISE_INDIVIDUAL:   /* blockwise_individual.h:141-153 */
+        *out = n / f->block * (bwi_single_max(f->block) + 64);
+        return MS_OK;
     default: return MS_ERR_INVALID;
     }
 }
@@ -642,6 +652,153 @@ static int gs_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n,
 static int morph_batch_compress_at(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out,
                                    uint64_t base, uint64_t *out_bytes);
 
+/* ------------------------------------------------------------------ */
+/* blockwise_individual_f<B, selector, ve> (blockwise_individual.h:     */
+/* 113-600) with the format codes of format_selector_commons.h:25-103.  */
+/* Per block of B values: the code byte, zero padding to the next       */
+/* 64-byte boundary of the image (create_aligned_ptr with the 64-byte   */
+/* vector size of the 8-lane processing style), then one morph_batch    */
+/* call of the code's default format (default_formats.h:35-94 with 8    */
+/* lanes) over the B values.  Codes 68 and 70 name macros that          */
+/* default_formats.h does not define; they are refused.  Code 71 is     */
+/* this library's own extension: rle_f's (value, length) pairs of the   */
+/* block, which delimit themselves by their lengths.                    */
+/* ------------------------------------------------------------------ */
+
+int oc_bwi_inner(unsigned code, ms_format_desc *in)
+{
+    memset(in, 0, sizeof *in);
+    if (code == 0) { in->kind = MS_UNCOMPR; return MS_OK; }
+    if (code <= 64) { in->kind = MS_STATIC_VBP; in->bw = code; in->step = 8; return MS_OK; }
+    switch (code) {
+    case 65: in->kind = MS_DYNAMIC_VBP; in->block = 512; in->page = 64; in->step = 8; return MS_OK;
+    case 66: in->kind = MS_GROUP_SIMPLE; in->step = 8; return MS_OK;
+    case 67:
+    case 69:
+        in->kind = code == 67 ? MS_DELTA : MS_FOR;
+        in->block = 1024;
+        in->page = code == 67 ? 0 : 8;
+        in->step = 8;
+        in->inner_kind = MS_DYNAMIC_VBP;
+        in->inner_block = 512;
+        in->inner_page = 64;
+        in->inner_step = 8;
+        return MS_OK;
+    case 71: in->kind = MS_RLE; return MS_OK;
+    default: return MS_ERR_INVALID;
+    }
+}
+
+/* blockwise_individual_helper::get_size_max_byte_single_block
+ * (blockwise_individual.h:57-79) over the defined default formats. */
+static uint64_t bwi_single_max(uint64_t B)
+{
+    static const unsigned codes[] = {0, 64, 65, 66, 67, 69};
+    uint64_t m = 0, x;
+    ms_format_desc in;
+    for (unsigned i = 0; i < sizeof codes / sizeof codes[0]; i++) {
+        oc_bwi_inner(codes[i], &in);
+        if (oc_size_max_byte(&in, B, &x) == MS_OK && x > m) m = x;
+    }
+    return m;
+}
+
+/* exhaustive_size_format_selector.h:80-141 (best) and its worst-size
+ * variant; the static selector of dummy_format_selectors.h:20-39. */
+static int bwi_choose(const ms_format_desc *f, const uint64_t *in, uint64_t B, uint8_t *tmp, unsigned *code)
+{
+    static const unsigned cand[] = {65, 66, 67, 69};
+    const int best = f->bw == MS_BWI_SEL_BEST;
+    uint64_t size, x;
+    unsigned c, i;
+    ms_format_desc d;
+    if (f->bw == MS_BWI_SEL_STATIC) { *code = f->page; return MS_OK; }
+    {
+        const unsigned mbw = block_bw(in, B);
+        if (best && mbw < 64) { size = (uint64_t)mbw * B / 8; c = mbw; }
+        else { size = 8 * B; c = 0; }
+    }
+    for (i = 0; i < sizeof cand / sizeof cand[0]; i++) {
+        oc_bwi_inner(cand[i], &d);
+        oc_size_max_byte(&d, B, &x);
+        memset(tmp, 0, x + 64);
+        if (morph_batch_compress_at(&d, in, B, tmp, 0, &x) != MS_OK) return MS_ERR_INVALID;
+        if (best == (x < size)) { size = x; c = cand[i]; }
+    }
+    *code = c;
+    return MS_OK;
+}
+
+/* `codes`: one per block, or NULL to run the format's selector; `chosen`
+ * (may be NULL) receives the codes used. */
+static int bwi_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, const uint8_t *codes, uint8_t *out,
+                        uint64_t base, uint64_t *out_bytes, uint8_t *chosen)
+{
+    const uint64_t B = f->block, nb = n / B;
+    uint64_t pos = 0, x;
+    uint8_t *tmp = NULL;
+    ms_format_desc d;
+    if (!codes) {
+        tmp = (uint8_t *)malloc(bwi_single_max(B) + 128);
+        if (!tmp) return MS_ERR_NOMEM;
+    }
+    for (uint64_t b = 0; b < nb; b++) {
+        unsigned code;
+        int rc = MS_OK;
+        if (codes) code = codes[b];
+        else rc = bwi_choose(f, in + b * B, B, tmp, &code);
+        if (rc == MS_OK) rc = oc_bwi_inner(code, &d);
+        if (rc != MS_OK) { free(tmp); return rc; }
+        if (chosen) chosen[b] = (uint8_t)code;
+        out[pos] = (uint8_t)code;
+        pos = align64(base + pos + 1) - base;
+        if (d.kind == MS_RLE) x = rle_compress(in + b * B, B, out + pos);
+        else if ((rc = morph_batch_compress_at(&d, in + b * B, B, out + pos, base + pos, &x)) != MS_OK) {
+            free(tmp);
+            return rc;
+        }
+        pos += x;
+    }
+    free(tmp);
+    *out_bytes = pos;
+    return MS_OK;
+}
+
+static int bwi_decompress(const ms_format_desc *f, const uint8_t *in, uint64_t n, uint64_t *out, uint64_t base,
+                          uint64_t *used)
+{
+    const uint64_t B = f->block;
+    uint64_t pos = 0, x;
+    ms_format_desc d;
+    for (uint64_t b = 0; b < n / B; b++) {
+        const unsigned code = in[pos];
+        int rc;
+        if ((rc = oc_bwi_inner(code, &d)) != MS_OK) return rc;
+        pos = align64(base + pos + 1) - base;
+        if (d.kind == MS_RLE) {
+            /* pairs until the lengths cover the block */
+            uint64_t done = 0;
+            x = 0;
+            while (done < B) {
+                uint64_t v, len;
+                memcpy(&v, in + pos + x, 8);
+                memcpy(&len, in + pos + x + 8, 8);
+                if (len == 0 || len > B - done) return MS_ERR_INVALID;
+                for (uint64_t i = 0; i < len; i++) out[b * B + done + i] = v;
+                done += len;
+                x += 16;
+            }
+        } else if (d.kind == MS_GROUP_SIMPLE) {
+            if ((rc = gs_decompress(&d, in + pos, B, out + b * B, base + pos, &x)) != MS_OK) return rc;
+        } else if ((rc = oc_morph_batch_decompress(&d, in + pos, B, out + b * B, &x)) != MS_OK) {
+            return rc;
+        }
+        pos += x;
+    }
+    *used = pos;
+    return MS_OK;
+}
+
 int oc_morph_batch_compress(const ms_format_desc *f, const uint64_t *in, uint64_t n, uint8_t *out, uint64_t *out_bytes)
 {
     return morph_batch_compress_at(f, in, n, out, 0, out_bytes);
@@ -668,6 +825,7 @@ static int morph_batch_compress_at(const ms_format_desc *f, const uint64_t *in, 
     case MS_STATIC_VBP_PAD: *out_bytes = vbpp_compress(f, in, n, out); return MS_OK;
     case MS_K_WISE_NS: *out_bytes = kwns_compress(in, n, out); return MS_OK;
     case MS_GROUP_SIMPLE: *out_bytes = gs_compress(f, in, n, out, base); return MS_OK;
+    case MS_BLOCKWISE_INDIVIDUAL: return bwi_compress(f, in, n, NULL, out, base, out_bytes, NULL);
     default: return MS_ERR_INVALID;
     }
 }
@@ -681,6 +839,7 @@ int oc_morph_batch_decompress(const ms_format_desc *f, const uint8_t *in, uint64
     case MS_STATIC_VBP_PAD: *in_bytes = vbpp_decompress(f, in, n, out); return MS_OK;
     case MS_K_WISE_NS: *in_bytes = kwns_decompress(in, n, out); return MS_OK;
     case MS_GROUP_SIMPLE: return gs_decompress(f, in, n, out, 0, in_bytes);
+    case MS_BLOCKWISE_INDIVIDUAL: return bwi_decompress(f, in, n, out, 0, in_bytes);
     case MS_UNCOMPR:
     case MS_STATIC_VBP:
     case MS_DYNAMIC_VBP: return leaf_decompress(f, in, n, out, in_bytes);
@@ -751,6 +910,7 @@ int oc_morph_compress(oc_col *out, const ms_format_desc *dst, const uint64_t *in
     uint64_t alloc, bs, nc, c = 0, r;
     if (!valid(dst)) return MS_ERR_INVALID;
     if (dst->kind == MS_UNCOMPR) return oc_col_from_values(out, in, n);
+    if (dst->kind == MS_BLOCKWISE_INDIVIDUAL) return oc_morph_compress_codes(out, dst, in, n, NULL);
     if (dst->kind == MS_RLE) {
         if (col_alloc(out, dst, 16 * n + 16)) return MS_ERR_NOMEM;
         c = rle_compress(in, n, out->data);
@@ -773,6 +933,78 @@ int oc_morph_compress(oc_col *out, const ms_format_desc *dst, const uint64_t *in
         out->size_used = c;
     }
     return MS_OK;
+}
+
+/* morph<ve, blockwise_individual_f, uncompr_f> at column level with the
+ * block codes given (one per block) or, codes == NULL, chosen by the
+ * format's selector. */
+int oc_morph_compress_codes(oc_col *out, const ms_format_desc *dst, const uint64_t *in, uint64_t n,
+                            const uint8_t *codes)
+{
+    uint64_t alloc, nc, c = 0, r;
+    int rc;
+    if (!valid(dst) || dst->kind != MS_BLOCKWISE_INDIVIDUAL) return MS_ERR_INVALID;
+    oc_size_max_byte_any_len(dst, n, &alloc);
+    alloc += n / dst->block * 8 * dst->block;   /* rle_f views (code 71) take up to 16 bytes per value */
+    if (col_alloc(out, dst, alloc)) return MS_ERR_NOMEM;
+    nc = round_down(n, dst->block);
+    if (nc && (rc = bwi_compress(dst, in, nc, codes, out->data, 0, &c, NULL)) != MS_OK) {
+        oc_col_free(out);
+        return rc;
+    }
+    out->count = n;
+    out->size_compr = c;
+    if (n - nc) {
+        oc_rest_offset(dst, c, &r);
+        memcpy(out->data + r, in + nc, 8 * (n - nc));
+        out->size_used = r + 8 * (n - nc);
+    } else {
+        out->size_used = c;
+    }
+    return MS_OK;
+}
+
+/* The code byte of every block of a blockwise_individual_f column. */
+int oc_bwi_codes(const oc_col *c, uint8_t *codes, uint64_t cap, uint64_t *nblocks)
+{
+    uint64_t pos = 0, x;
+    const uint64_t B = c->f.block, nb = (c->count - oc_count_values_uncompr(c)) / B;
+    ms_format_desc d;
+    if (c->f.kind != MS_BLOCKWISE_INDIVIDUAL) return MS_ERR_INVALID;
+    *nblocks = nb;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t *tmp;
+        int rc;
+        if (pos >= c->size_compr) return MS_ERR_INVALID;
+        if (b < cap) codes[b] = c->data[pos];
+        if ((rc = oc_bwi_inner(c->data[pos], &d)) != MS_OK) return rc;
+        /* the view's size: decompress it */
+        tmp = (uint64_t *)malloc(8 * B);
+        if (!tmp) return MS_ERR_NOMEM;
+        {
+            ms_format_desc one = c->f;
+            one.bw = MS_BWI_SEL_STATIC;
+            rc = bwi_decompress(&one, c->data + pos, B, tmp, pos, &x);
+        }
+        free(tmp);
+        if (rc != MS_OK) return rc;
+        pos += x;
+    }
+    return MS_OK;
+}
+
+/* The exhaustive (or static) selector of a blockwise_individual_f format
+ * over one block of B values. */
+int oc_bwi_choose(const ms_format_desc *f, const uint64_t *in, unsigned *code)
+{
+    uint8_t *tmp;
+    int rc;
+    if (!valid(f) || f->kind != MS_BLOCKWISE_INDIVIDUAL) return MS_ERR_INVALID;
+    tmp = (uint8_t *)malloc(bwi_single_max(f->block) + 128);
+    if (!tmp) return MS_ERR_NOMEM;
+    rc = bwi_choose(f, in, f->block, tmp, code);
+    free(tmp);
+    return rc;
 }
 
 /* morph.h:203-241 (decompression), rle.h:264-297 */

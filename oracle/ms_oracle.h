@@ -56,6 +56,15 @@ int oc_col_from_image(oc_col *out, const ms_format_desc *f, const uint8_t *image
 int oc_morph_compress(oc_col *out, const ms_format_desc *dst,
                       const uint64_t *in, uint64_t n);
 int oc_morph_decompress(const oc_col *in, uint64_t *out);
+/* blockwise_individual_f (blockwise_individual.h:113-600): the default
+ * format of a format code (MS_ERR_INVALID for codes without one), the
+ * column-level morph with given codes, the codes of a column, and the
+ * format's selector over one block. */
+int oc_bwi_inner(unsigned code, ms_format_desc *in);
+int oc_morph_compress_codes(oc_col *out, const ms_format_desc *dst, const uint64_t *in, uint64_t n,
+                            const uint8_t *codes);
+int oc_bwi_codes(const oc_col *c, uint8_t *codes, uint64_t cap, uint64_t *nblocks);
+int oc_bwi_choose(const ms_format_desc *f, const uint64_t *in, unsigned *code);
 uint64_t oc_count_values_uncompr(const oc_col *c);
 /* column<F>::prepare_for_random_access (column.h:241-268); *acted = 1 if
  * the column changed. */

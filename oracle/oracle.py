@@ -67,6 +67,10 @@ def lib():
             "oc_cpu_has_avx512": (ctypes.c_int, []),
             "oc_prepare_for_random_access": (ctypes.c_int, [C, P(ctypes.c_int)]),
             "oc_cpu_select_sum": (ctypes.c_int, [P(U64), U32, U64, C, ctypes.c_int, ctypes.c_int]),
+            "oc_bwi_inner": (ctypes.c_int, [ctypes.c_uint, FD]),
+            "oc_morph_compress_codes": (ctypes.c_int, [C, FD, ctypes.c_void_p, U64, ctypes.c_void_p]),
+            "oc_bwi_codes": (ctypes.c_int, [C, ctypes.c_void_p, U64, P(U64)]),
+            "oc_bwi_choose": (ctypes.c_int, [FD, ctypes.c_void_p, P(ctypes.c_uint)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -165,6 +169,39 @@ def morph(f, v):
     c = Column()
     _check(lib().oc_morph_compress(ctypes.byref(c.c), ctypes.byref(f), v.ctypes.data, len(v)), "morph")
     return c
+
+
+def morph_codes(f, v, codes):
+    """morph<ve, blockwise_individual_f, uncompr_f> with the block codes given."""
+    v = _u64(v)
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    c = Column()
+    _check(lib().oc_morph_compress_codes(ctypes.byref(c.c), ctypes.byref(f), v.ctypes.data, len(v),
+                                         codes.ctypes.data), "morph_codes")
+    return c
+
+
+def block_codes(col):
+    """The format code of every block of a blockwise_individual_f column."""
+    n = ctypes.c_uint64()
+    _check(lib().oc_bwi_codes(ctypes.byref(col.c), None, 0, ctypes.byref(n)), "block_codes")
+    out = np.zeros(max(n.value, 1), np.uint8)
+    _check(lib().oc_bwi_codes(ctypes.byref(col.c), out.ctypes.data, n.value, ctypes.byref(n)), "block_codes")
+    return out[:n.value]
+
+
+def bwi_inner(code):
+    """The default format of a format code (None if it has none)."""
+    d = FormatDesc()
+    return d if lib().oc_bwi_inner(code, ctypes.byref(d)) == 0 else None
+
+
+def bwi_choose(f, block):
+    """The format selector of blockwise_individual_f f over one block."""
+    block = _u64(block)
+    code = ctypes.c_uint()
+    _check(lib().oc_bwi_choose(ctypes.byref(f), block.ctypes.data, ctypes.byref(code)), "bwi_choose")
+    return code.value
 
 
 def morph_batch_compress(f, v):

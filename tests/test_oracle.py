@@ -556,3 +556,108 @@ def test_extra_formats_column_layout(oracle, f, n):
     assert np.array_equal(col.values(), v)
     if n - nc:
         assert col.count_uncompr() == n - nc
+
+
+# ---------------------------------------------------------------- blockwise_individual_f
+def _bwi_payload(oracle, code, blk):
+    """One morph_batch call of the code's default format (rle_f: its pairs)."""
+    from engine_amd.formats import rle_f
+    if code == 71:
+        return oracle.morph(rle_f(), blk).image()
+    return oracle.morph_batch_compress(oracle.bwi_inner(code), blk)
+
+
+@pytest.mark.parametrize("B", [1024, 2048, 3072])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_bwi_layout(oracle, B, seed):
+    """blockwise_individual.h:185-249: per block the code byte, zero padding
+    to the next 64-byte boundary of the image (create_aligned_ptr with the
+    64-byte vectors of the 8-lane style), then the default format's own
+    morph_batch output; the rest raw behind the gap (column.h:182-196)."""
+    from tests import bwi_data
+    from engine_amd.formats import blockwise_individual_f
+    nb, rest = 9, 123
+    x = bwi_data.mixture(seed, nb, B, rest)
+    allowed = [0, 7, 33, 64, 65, 66, 67, 69, 71]
+    codes = bwi_data.fitting_codes(x, B, seed, allowed)
+    f = blockwise_individual_f(B)
+    c = oracle.morph_codes(f, x, codes)
+    img = c.image()
+    count, used, compr = c.meta()
+    pos = 0
+    for b in range(nb):
+        assert img[pos] == codes[b]
+        q = (pos + 1 + 63) // 64 * 64
+        assert not img[pos + 1:q].any()
+        pay = _bwi_payload(oracle, int(codes[b]), x[b * B:(b + 1) * B])
+        assert img[q:q + len(pay)].tobytes() == pay.tobytes()
+        pos = q + len(pay)
+    assert pos == compr and count == nb * B + rest
+    assert np.array_equal(oracle.block_codes(c), codes)
+    assert np.array_equal(c.values(), x)
+    r = oracle.rest_offset(f, compr)
+    assert used == r + 8 * rest
+    assert img[r:].view(np.uint64).tolist() == x[nb * B:].tolist()
+
+
+def _bwi_sizes(oracle, blk):
+    mbw = max(1, int(np.bitwise_or.reduce(blk)).bit_length())
+    sizes = {c: len(_bwi_payload(oracle, c, blk)) for c in (65, 66, 67, 69)}
+    return mbw, sizes
+
+
+@pytest.mark.parametrize("kind", range(6))
+def test_bwi_exhaustive_selectors(oracle, kind):
+    """exhaustive_size_format_selector.h:80-141: best = the smallest of
+    static_vbp at the maximum bit width (below 64 bits; uncompr_f
+    otherwise) and dynamic_vbp, group_simple, delta and for in this order,
+    a later one only when strictly smaller; worst starts from uncompr_f and
+    takes a later one when not smaller.  Sizes here come from compressing
+    the block in each format; the pick must be a size extremum."""
+    from tests import bwi_data
+    from engine_amd.formats import MS_BWI_SEL_BEST, MS_BWI_SEL_WORST, blockwise_individual_f
+    B = 2048
+    for s in range(3):
+        blk = bwi_data.block_values(100 * kind + s, kind, B)
+        mbw, sizes = _bwi_sizes(oracle, blk)
+        order = [(mbw, mbw * B // 8) if mbw < 64 else (0, 8 * B)] + [(c, sizes[c]) for c in (65, 66, 67, 69)]
+        best = oracle.bwi_choose(blockwise_individual_f(B, MS_BWI_SEL_BEST), blk)
+        lo = min(sz for _, sz in order)
+        assert best == next(c for c, sz in order if sz == lo)
+        worst = oracle.bwi_choose(blockwise_individual_f(B, MS_BWI_SEL_WORST), blk)
+        order_w = [(0, 8 * B)] + [(c, sizes[c]) for c in (65, 66, 67, 69)]
+        hi = max(sz for _, sz in order_w)
+        assert worst == [c for c, sz in order_w if sz == hi][-1]
+
+
+def test_bwi_static_selector_and_refused_codes(oracle):
+    from tests import bwi_data
+    from engine_amd.formats import MS_BWI_SEL_STATIC, blockwise_individual_f
+    x = bwi_data.mixture(5, 4, 1024, 10, kinds=2)
+    for code in (67, 71, 64, 0):   # 71: rle pairs of random values take 16 bytes per value
+        c = oracle.morph(blockwise_individual_f(1024, MS_BWI_SEL_STATIC, code), x)
+        assert oracle.block_codes(c).tolist() == [code] * 4
+        assert np.array_equal(c.values(), x)
+    for bad in (68, 70, 72, 200):
+        assert oracle.bwi_inner(bad) is None
+        with pytest.raises(RuntimeError):
+            oracle.morph_codes(blockwise_individual_f(1024), x, np.full(4, bad, np.uint8))
+
+
+def test_bwi_prepare_for_random_access(oracle):
+    """column.h:241-268 on blockwise_individual_f: the zero-padded rest is
+    one more block at sizeCompr, aligned to the image."""
+    from tests import bwi_data
+    from engine_amd.formats import blockwise_individual_f
+    B = 1024
+    x = bwi_data.mixture(8, 3, B, 300)
+    c = oracle.morph(blockwise_individual_f(B), x)
+    compr = c.meta()[2]
+    assert c.prepare_for_random_access()
+    img = c.image()
+    pad = np.concatenate([x[3 * B:], np.zeros(B - 300, np.uint64)])
+    code = oracle.bwi_choose(blockwise_individual_f(B), pad)
+    assert img[compr] == code
+    q = (compr + 1 + 63) // 64 * 64
+    pay = _bwi_payload(oracle, code, pad)
+    assert img[q:q + len(pay)].tobytes() == pay.tobytes()
