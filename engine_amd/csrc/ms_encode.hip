@@ -559,37 +559,89 @@ int launch_chunk_pack(const chunk_emit &ce, const uint64_t *d_cout, uint8_t *out
 }
 
 // ------------------------------------------------------------------
-// rle_f (rle.h:95-147): maximal runs from run-start flags and a scan.
+// rle_f (rle.h:95-147): maximal runs.  The column is cut into chunks of
+// RLE_CCH values, a wave per chunk.  A run belongs to the chunk holding
+// its first value; pass 1 counts each chunk's run starts (ballots) and
+// records its first start, a scan of the counts places every chunk's
+// runs, pass 2 writes each run's value and, when the next start lies in
+// the same chunk, its length; the chunk's last run gets its length from
+// the next chunk that starts a run (pass 3, a thread per chunk).
 // ------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_rle_flags(const uint64_t *v, const uint64_t *d_m, uint64_t bound,
-                                                  uint64_t *flag)
-{
-    const uint64_t m = *d_m;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < bound; i += (uint64_t)gridDim.x * WG)
-        flag[i] = i < m && (i == 0 || v[i] != v[i - 1]) ? 1 : 0;
-}
+constexpr uint64_t RLE_CCH = 16384;
 
-__global__ __launch_bounds__(WG) void k_rle_starts(const uint64_t *v, const uint64_t *d_m, const uint64_t *flag,
-                                                   const uint64_t *rid, uint64_t *out_w, uint64_t *starts)
+__global__ __launch_bounds__(64) void k_rle_count(const uint64_t *v, const uint64_t *d_m, uint64_t nch_bound,
+                                                  uint64_t *cnt, uint64_t *first)
 {
     const uint64_t m = *d_m;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < m; i += (uint64_t)gridDim.x * WG) {
-        if (flag[i]) {
-            const uint64_t r = rid[i];
-            out_w[2 * r] = v[i];
-            starts[r] = i;
+    const uint64_t nch = (m + RLE_CCH - 1) / RLE_CCH;
+    const uint32_t t = threadIdx.x;
+    for (uint64_t c = blockIdx.x; c < nch_bound; c += gridDim.x) {
+        uint64_t runs = 0, f = ~0ull;
+        if (c < nch) {
+            const uint64_t lo = c * RLE_CCH, hi = lo + RLE_CCH < m ? lo + RLE_CCH : m;
+            for (uint64_t i0 = lo; i0 < hi; i0 += 64) {
+                const uint64_t i = i0 + t;
+                const bool st = i < hi && (i == 0 || v[i] != v[i - 1]);
+                const uint64_t b = __ballot(st);
+                if (b && f == ~0ull) f = i0 + __ffsll((unsigned long long)b) - 1;
+                runs += __popcll(b);
+            }
+        }
+        if (t == 0) {
+            cnt[c] = runs;
+            first[c] = f;
         }
     }
 }
 
-__global__ __launch_bounds__(WG) void k_rle_lens(const uint64_t *d_m, const uint64_t *flag, const uint64_t *rid,
-                                                 const uint64_t *starts, uint64_t *out_w, ms_dev_meta *meta)
+__global__ __launch_bounds__(64) void k_rle_write(const uint64_t *v, const uint64_t *d_m, const uint64_t *roff,
+                                                  uint64_t *out_w)
 {
     const uint64_t m = *d_m;
-    const uint64_t runs = m ? rid[m - 1] + flag[m - 1] : 0;
-    for (uint64_t r = (uint64_t)blockIdx.x * WG + threadIdx.x; r < runs; r += (uint64_t)gridDim.x * WG)
-        out_w[2 * r + 1] = (r + 1 < runs ? starts[r + 1] : m) - starts[r];
+    const uint64_t nch = (m + RLE_CCH - 1) / RLE_CCH;
+    const uint32_t t = threadIdx.x;
+    for (uint64_t c = blockIdx.x; c < nch; c += gridDim.x) {
+        const uint64_t lo = c * RLE_CCH, hi = lo + RLE_CCH < m ? lo + RLE_CCH : m;
+        uint64_t r0 = roff[c], last = 0;
+        bool any = false;
+        for (uint64_t i0 = lo; i0 < hi; i0 += 64) {
+            const uint64_t i = i0 + t;
+            const bool st = i < hi && (i == 0 || v[i] != v[i - 1]);
+            const uint64_t b = __ballot(st);
+            if (st) {
+                const uint64_t below = b & ((1ull << t) - 1ull);
+                const uint64_t r = r0 + __popcll(below);
+                out_w[2 * r] = v[i];
+                // the run before this start: in this chunk, its length
+                if (below) out_w[2 * (r - 1) + 1] = i - (i0 + 63 - __clzll(below));
+                else if (any) out_w[2 * (r - 1) + 1] = i - last;
+            }
+            if (b) {
+                last = i0 + 63 - __clzll(b);
+                any = true;
+            }
+            r0 += __popcll(b);
+        }
+        // the chunk's last run: its start, until pass 3 knows the next one
+        if (t == 0 && any) out_w[2 * (r0 - 1) + 1] = last;
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_rle_close(const uint64_t *d_m, const uint64_t *cnt, const uint64_t *first,
+                                                  const uint64_t *roff, uint64_t *out_w, ms_dev_meta *meta)
+{
+    const uint64_t m = *d_m;
+    const uint64_t nch = (m + RLE_CCH - 1) / RLE_CCH;
+    for (uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x; c < nch; c += (uint64_t)gridDim.x * WG) {
+        if (!cnt[c]) continue;
+        uint64_t next = m;
+        for (uint64_t d = c + 1; d < nch; d++)
+            if (cnt[d]) { next = first[d]; break; }
+        const uint64_t r = roff[c] + cnt[c] - 1;
+        out_w[2 * r + 1] = next - out_w[2 * r + 1];
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint64_t runs = nch ? roff[nch - 1] + cnt[nch - 1] : 0;
         meta->count = m;
         meta->size_used = meta->size_compr = 16 * runs;
     }
@@ -598,18 +650,19 @@ __global__ __launch_bounds__(WG) void k_rle_lens(const uint64_t *d_m, const uint
 int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc)
 {
     hipStream_t s = sc.s;
-    const uint64_t bound = m_bound + 1;
-    uint64_t *flag = (uint64_t *)sc.get(bound * 8 + 64);
-    uint64_t *rid = (uint64_t *)sc.get(bound * 8 + 64);
-    uint64_t *starts = (uint64_t *)sc.get(bound * 8 + 64);
-    if (!flag || !rid || !starts) return ms_fail(MS_ERR_NOMEM, "rle scratch");
-    hipLaunchKernelGGL(k_rle_flags, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_v, d_m, bound, flag);
+    const uint64_t nch_bound = m_bound / RLE_CCH + 2;
+    uint64_t *cnt = (uint64_t *)sc.get(nch_bound * 8 + 64);
+    uint64_t *first = (uint64_t *)sc.get(nch_bound * 8 + 64);
+    uint64_t *roff = (uint64_t *)sc.get(nch_bound * 8 + 64);
+    if (!cnt || !first || !roff) return ms_fail(MS_ERR_NOMEM, "rle scratch");
+    const uint32_t grid = grid_for(nch_bound, 1, 65536);
+    hipLaunchKernelGGL(k_rle_count, dim3(grid), dim3(64), 0, s, d_v, d_m, nch_bound, cnt, first);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan(flag, rid, bound, sc));
-    hipLaunchKernelGGL(k_rle_starts, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_v, d_m, flag, rid,
-                       (uint64_t *)out->data, starts);
-    hipLaunchKernelGGL(k_rle_lens, dim3(grid_for(bound, WG * 4ull)), dim3(WG), 0, s, d_m, flag, rid, starts,
-                       (uint64_t *)out->data, out->meta);
+    MS_TRY(launch_exclusive_scan(cnt, roff, nch_bound, sc));
+    hipLaunchKernelGGL(k_rle_write, dim3(grid), dim3(64), 0, s, d_v, d_m, (const uint64_t *)roff,
+                       (uint64_t *)out->data);
+    hipLaunchKernelGGL(k_rle_close, dim3(grid_for(nch_bound, WG)), dim3(WG), 0, s, d_m, (const uint64_t *)cnt,
+                       (const uint64_t *)first, (const uint64_t *)roff, (uint64_t *)out->data, out->meta);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

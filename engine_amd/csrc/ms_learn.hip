@@ -254,7 +254,7 @@ namespace ms {
 // Features of a chunk from its statistics (MS_NFEAT floats) and the exact
 // size in bytes of each candidate format, for 2^16-value chunks (no rest):
 //   static_vbp<maxbw, 8>              : 2^16 * maxbw / 8
-//   dynamic_vbp<512, 2048, 8>         : one page of 2048 meta bytes + 64 bw per block
+//   dynamic_vbp<512, 64, 8>           : two pages of 64 meta bytes + 64 bw per block
 //   delta<1024, 8, dvbp<512, 64, 8>>  : per 1024 values one inner call (64 meta bytes) + 64 dbw per block
 //   for<1024, 8, dvbp<512, 64, 8>>    : + one page of 8 references per 8 blocks of 1024
 //   rle                               : 16 bytes per run
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(WG) void k_chunk_features(const uint64_t *__restric
     if (sizes) {
         uint64_t *z = sizes + c * MS_NCAND;
         z[MS_CAND_STATIC] = (uint64_t)CHUNK / 8 * s[MS_ST_MAX_BW];
-        z[MS_CAND_DVBP] = 2048 + 64 * s[MS_ST_SUM_BW];
+        z[MS_CAND_DVBP] = (CBLOCKS / 64) * 64 + 64 * s[MS_ST_SUM_BW];
         z[MS_CAND_DELTA] = (CHUNK / 1024) * 64 + 64 * s[MS_ST_SUM_DBW];
         z[MS_CAND_FOR] = (CHUNK / 1024 / 8) * 64 + (CHUNK / 1024) * 64 + 64 * s[MS_ST_SUM_FBW];
         z[MS_CAND_RLE] = 16 * s[MS_ST_RUNS];
@@ -599,7 +599,7 @@ extern "C" int ms_candidate_format(uint32_t cand, uint32_t static_bw, ms_format_
         f.kind = MS_STATIC_VBP; f.bw = static_bw; f.step = 8;
         break;
     case MS_CAND_DVBP:
-        f.kind = MS_DYNAMIC_VBP; f.block = 512; f.page = 2048; f.step = 8;
+        f.kind = MS_DYNAMIC_VBP; f.block = 512; f.page = 64; f.step = 8;
         break;
     case MS_CAND_DELTA:
     case MS_CAND_FOR:

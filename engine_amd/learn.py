@@ -342,6 +342,17 @@ def label(specs, tables, cost, batch=2048, stream=None):
 
 
 # ------------------------------------------------------------------ workload
+# blockwise_individual_f format codes of the candidates
+# (format_selector_commons.h:25-103): static_vbp_f at the chunk's maximum
+# bit width is that bit width; rle_f's pairs are this library's code 71.
+_CAND_CODE = (None, 65, 67, 69, 71)
+
+
+def format_codes(cand, max_bw):
+    """Per-chunk candidate picks as blockwise_individual_f codes."""
+    return np.array([int(b) if int(k) == 0 else _CAND_CODE[int(k)] for k, b in zip(cand, max_bw)], np.uint8)
+
+
 def _masked_sum_col(col, val, res):
     ea.select_sum(2, val, col, res)
     return int(res.get()[0])
@@ -362,14 +373,17 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
     """The learned-selection workload: ncols columns of chunks_per_col
     chunks (a mixture of the format-cascade-sweep distributions switching
     every 2^16 values).  Trains the selector on ntrain generated chunks,
-    picks a format per chunk, and times SELECT SUM(x) WHERE x < val over
-    every column: per-column calls on whole columns in their column-level
-    best format (plain, eager and as one hipGraph) against one autotuned
-    batch over the learned per-chunk columns and over the exact-size
-    per-chunk columns.  Every path's per-column sums must equal the sums
-    over the uncompressed columns."""
+    picks a format per chunk, and stores every column as one
+    blockwise_individual_f<2^16> column holding each chunk in its pick.
+    Times SELECT SUM(x) WHERE x < val over every column: per-column calls
+    on whole columns in their column-level best format (plain, eager and as
+    one hipGraph) against one autotuned batch over the learned-pick columns
+    and over the exact-best-pick columns.  Every path's per-column sums
+    must equal the sums over the uncompressed columns."""
     import time
+    from .formats import blockwise_individual_f
     say = log or (lambda *a: None)
+    bwi = blockwise_individual_f(CHUNK)
     t0 = time.perf_counter()
     tables = GeoTables()
     if cost is None:
@@ -416,21 +430,17 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
         pc = ea.morph(candidate_format(k, mbw), whole)
         plain.append(pc)
         foot["plain"] += pc.meta().size_used
+        lc = ea.morph_blockwise(bwi, whole, format_codes(choice, st[:, ST_MAX_BW]))
+        ec = lc if np.array_equal(choice, ex) else ea.morph_blockwise(bwi, whole, format_codes(ex, st[:, ST_MAX_BW]))
+        learned.append(lc)
+        exact.append(ec)
+        foot["learned"] += lc.meta().size_used
+        foot["exact"] += ec.meta().size_used
         del whole
-        for i in range(n):
-            raw = ea.from_device(d_vals.ptr(i * CHUNK), CHUNK)
-            bw = int(st[i, ST_MAX_BW])
-            lc = ea.morph(candidate_format(int(choice[i]), bw), raw)
-            ec = lc if ex[i] == choice[i] else ea.morph(candidate_format(int(ex[i]), bw), raw)
-            learned.append(lc)
-            exact.append(ec)
-            foot["learned"] += int(esz[i, choice[i]])
-            foot["exact"] += int(esz[i, ex[i]])
     t_prep = time.perf_counter() - t2
     nchunks = ncols * n
     say(f"selection workload: {ncols} columns x {n} chunks prepared in {t_prep:.1f}s; footprint learned "
         f"{foot['learned']} exact {foot['exact']} plain {foot['plain']}; fallbacks {fallbacks}")
-    res = ea.DeviceU64(nchunks)
     resc = ea.DeviceU64(ncols)
     slots = [resc.at(k) for k in range(ncols)]
 
@@ -438,10 +448,6 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
         if got != want:
             bad = [k for k in range(ncols) if got[k] != want[k]]
             raise RuntimeError(f"selection workload: {tag} sums differ at columns {bad[:5]}")
-
-    def per_col(r):
-        with np.errstate(over="ignore"):
-            return r.reshape(ncols, n).sum(axis=1, dtype=np.uint64).tolist()
 
     def timed(fn, reps):
         for _ in range(warmup):
@@ -479,17 +485,18 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
         b = ea.Batch()
         for col in cols:
             b.enqueue(col, 2, val)
-        b.run(res)
-        check_cols(per_col(res.get()), f"batched {tag}")
-        cfg = b.tune(res, reps=3)
-        b.run(res)
-        check_cols(per_col(res.get()), f"batched {tag} tuned")
-        out[tag] = (timed(lambda: b.run(res), steps), cfg.as_dict())
+        b.run(resc)
+        check_cols(resc.get().tolist(), f"batched {tag}")
+        cfg = b.tune(resc, reps=3)
+        b.run(resc)
+        check_cols(resc.get().tolist(), f"batched {tag} tuned")
+        out[tag] = (timed(lambda: b.run(resc), steps), cfg.as_dict())
     values = nchunks * CHUNK
     ue = lambda ms: round(8 * values / ms / 1e6, 1)  # noqa: E731
     rep = {
         "workload": f"{ncols} uint64 columns of {n} x 2^16 values (a mixture of the format-cascade-sweep "
                     f"distributions switching every 2^16 values), select(x < {val}) + agg_sum per column",
+        "layout": "each column one blockwise_individual_f<65536> column, a format code per chunk",
         "train_chunks": ntrain, "epochs": epochs, "loss_first_epoch": float(losses[0]),
         "loss_last_epoch": float(losses[-1]), "train_seconds": round(t_train, 2),
         "calibration_seconds": round(t_cal, 1),

@@ -227,6 +227,86 @@ struct group_simple_f : format {
     }
 };
 
+/* format_code (format_selector_commons.h:25-103): the format of one block
+ * of a blockwise_individual_f column; static_vbp_f at bit width bw is
+ * code bw.  defaultDeltaGroupSimple and defaultForGroupSimple name formats
+ * the reference never defines: the library refuses them. */
+enum class format_code : uint8_t {
+    uncompr = 0,
+    defaultDynamicVbp = 65,
+    defaultGroupSimple,
+    defaultDeltaDynamicVbp,
+    defaultDeltaGroupSimple,
+    defaultForDynamicVbp,
+    defaultForGroupSimple,
+    defaultStaticVbp_1 = 1, defaultStaticVbp_2 = 2, defaultStaticVbp_3 = 3, defaultStaticVbp_4 = 4,
+    defaultStaticVbp_5 = 5, defaultStaticVbp_6 = 6, defaultStaticVbp_7 = 7, defaultStaticVbp_8 = 8,
+    defaultStaticVbp_9 = 9, defaultStaticVbp_10 = 10, defaultStaticVbp_11 = 11, defaultStaticVbp_12 = 12,
+    defaultStaticVbp_13 = 13, defaultStaticVbp_14 = 14, defaultStaticVbp_15 = 15, defaultStaticVbp_16 = 16,
+    defaultStaticVbp_17 = 17, defaultStaticVbp_18 = 18, defaultStaticVbp_19 = 19, defaultStaticVbp_20 = 20,
+    defaultStaticVbp_21 = 21, defaultStaticVbp_22 = 22, defaultStaticVbp_23 = 23, defaultStaticVbp_24 = 24,
+    defaultStaticVbp_25 = 25, defaultStaticVbp_26 = 26, defaultStaticVbp_27 = 27, defaultStaticVbp_28 = 28,
+    defaultStaticVbp_29 = 29, defaultStaticVbp_30 = 30, defaultStaticVbp_31 = 31, defaultStaticVbp_32 = 32,
+    defaultStaticVbp_33 = 33, defaultStaticVbp_34 = 34, defaultStaticVbp_35 = 35, defaultStaticVbp_36 = 36,
+    defaultStaticVbp_37 = 37, defaultStaticVbp_38 = 38, defaultStaticVbp_39 = 39, defaultStaticVbp_40 = 40,
+    defaultStaticVbp_41 = 41, defaultStaticVbp_42 = 42, defaultStaticVbp_43 = 43, defaultStaticVbp_44 = 44,
+    defaultStaticVbp_45 = 45, defaultStaticVbp_46 = 46, defaultStaticVbp_47 = 47, defaultStaticVbp_48 = 48,
+    defaultStaticVbp_49 = 49, defaultStaticVbp_50 = 50, defaultStaticVbp_51 = 51, defaultStaticVbp_52 = 52,
+    defaultStaticVbp_53 = 53, defaultStaticVbp_54 = 54, defaultStaticVbp_55 = 55, defaultStaticVbp_56 = 56,
+    defaultStaticVbp_57 = 57, defaultStaticVbp_58 = 58, defaultStaticVbp_59 = 59, defaultStaticVbp_60 = 60,
+    defaultStaticVbp_61 = 61, defaultStaticVbp_62 = 62, defaultStaticVbp_63 = 63, defaultStaticVbp_64 = 64
+};
+
+/* format_selector<ve> (format_selector_commons.h:121-138) and the
+ * selectors of exhaustive_size_format_selector.h:33-141 and
+ * dummy_format_selectors.h:20-39.  The selection runs in the library; a
+ * selector only names it. */
+template <class t_vector_extension>
+struct format_selector {
+};
+
+template <bool t_ChooseBest>
+struct exhaustive_size_format_selector_helper {
+    template <class t_vector_extension>
+    struct selector : format_selector<t_vector_extension> {
+        static constexpr uint32_t ms_selector = t_ChooseBest ? MS_BWI_SEL_BEST : MS_BWI_SEL_WORST;
+        static constexpr uint32_t ms_code = 0;
+    };
+};
+
+template <format_code t_FormatCode>
+struct static_format_selector_helper {
+    template <class t_vector_extension>
+    struct selector : format_selector<t_vector_extension> {
+        static constexpr uint32_t ms_selector = MS_BWI_SEL_STATIC;
+        static constexpr uint32_t ms_code = (uint32_t)t_FormatCode;
+    };
+};
+
+/* blockwise_individual_f<t_BlockSizeLog, t_format_selector, ve>
+ * (blockwise_individual.h:113-158): a format chosen per block of
+ * t_BlockSizeLog values among the 8-lane style's default formats. */
+template <size_t t_BlockSizeLog, template <class> class t_format_selector, class t_vector_extension>
+struct blockwise_individual_f : format {
+    static_assert(std::is_base_of<format_selector<t_vector_extension>, t_format_selector<t_vector_extension>>::value,
+                  "blockwise_individual_f: the template parameter t_format_selector must be a subclass of "
+                  "format_selector");
+    static_assert(t_vector_extension::lanes == 8, "blockwise_individual_f: the 8-lane processing style's formats");
+    static_assert(t_BlockSizeLog % 1024 == 0 && t_BlockSizeLog <= (1u << 20),
+                  "blockwise_individual_f: the logical block size must be a multiple of each default format's");
+    static constexpr size_t m_BlockSize = t_BlockSizeLog;
+    static ms_format_desc desc()
+    {
+        ms_format_desc d = {};
+        d.kind = MS_BLOCKWISE_INDIVIDUAL;
+        d.block = (uint32_t)t_BlockSizeLog;
+        d.bw = t_format_selector<t_vector_extension>::ms_selector;
+        d.page = t_format_selector<t_vector_extension>::ms_code;
+        d.step = 8;
+        return d;
+    }
+};
+
 // ------------------------------------------------------------------ column
 /* column<F> (column.h:45-287): a device column in format F. */
 template <class F>

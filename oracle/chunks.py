@@ -142,7 +142,7 @@ def stats(x):
 def sizes(st):
     st = [int(v) for v in st]
     return np.array([CHUNK // 8 * st[ST_MAX_BW],
-                     2048 + 64 * st[ST_SUM_BW],
+                     (CHUNK // 512 // 64) * 64 + 64 * st[ST_SUM_BW],
                      (CHUNK // 1024) * 64 + 64 * st[ST_SUM_DBW],
                      (CHUNK // 1024 // 8) * 64 + (CHUNK // 1024) * 64 + 64 * st[ST_SUM_FBW],
                      16 * st[ST_RUNS]], dtype=np.uint64)

@@ -116,6 +116,22 @@ int main()
         threw = true;
     }
     CHECK(threw);
+    // blockwise_individual_f: the exhaustive best-size selector and a static
+    // one, read back by decompression and by agg_sum
+    {
+        using bf = blockwise_individual_f<2048, exhaustive_size_format_selector_helper<true>::selector, ve>;
+        using rf = blockwise_individual_f<2048, static_format_selector_helper<format_code::defaultForDynamicVbp>::selector,
+                                          ve>;
+        const column<bf> *cb = morph<ve, bf>(base);
+        const column<rf> *cr = morph<ve, rf>(base);
+        CHECK(values_of(cb) == x);
+        CHECK(values_of(cr) == x);
+        uint64_t total = 0;
+        for (uint64_t v : x) total += v;
+        CHECK(values_of(agg_sum<ve, bf>(cb)) == std::vector<uint64_t>{total});
+        CHECK(values_of(agg_sum<ve, rf>(cr)) == std::vector<uint64_t>{total});
+        CHECK(cr->get_size_compr_byte() >= cb->get_size_compr_byte());
+    }
     delete pos; delete dat; delete sum; delete pd; delete cf; delete is; delete ms; delete s;
     delete rl; delete a; delete b; delete l; delete r; delete c; delete base;
     std::printf("template api ok: %zu values, %llu selected\n", n, (unsigned long long)cnt);

@@ -31,7 +31,7 @@ def sample_chunks():
 def candidate_formats(static_bw):
     from engine_amd.formats import (default_delta_dynamic_vbp_f, default_for_dynamic_vbp_f, dynamic_vbp_f, rle_f,
                                     static_vbp_f)
-    return [static_vbp_f(static_bw, 8), dynamic_vbp_f(512, 2048, 8), default_delta_dynamic_vbp_f(8),
+    return [static_vbp_f(static_bw, 8), dynamic_vbp_f(512, 64, 8), default_delta_dynamic_vbp_f(8),
             default_for_dynamic_vbp_f(8), rle_f()]
 
 
