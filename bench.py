@@ -359,7 +359,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 2^24-value static_vbp query")
     ap.add_argument("--c5-cols", type=int, default=1024,
                     help="columns of 64 x 2^16 values of the learned-selection workload (0: skip it)")
-    ap.add_argument("--c5-train", type=int, default=1_000_000, help="labelled training chunks of the selector")
+    ap.add_argument("--c5-train", type=int, default=32768,
+                    help="training chunks of the selector (each compressed and timed in every candidate format)")
     ap.add_argument("--batch-config", default=None,
                     help="JSON ms_batch_config for the sweep's batch instead of autotuning (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")

@@ -47,6 +47,7 @@ ABI = {
     "ms_morph": (ctypes.c_int, [_COLP, _FD, VP, VP]),
     "ms_morph_blockwise": (ctypes.c_int, [_COLP, _FD, VP, VP, U64, VP]),
     "ms_col_block_codes": (ctypes.c_int, [VP, VP, U64, P(U64)]),
+    "ms_batch_view_times": (ctypes.c_int, [VP, VP, U32, VP, U64, P(U64), VP]),
     "ms_select": (ctypes.c_int, [_COLP, U32, U64, _FD, VP, U32, VP]),
     "ms_between": (ctypes.c_int, [_COLP, U32, U64, U32, U64, _FD, VP, U32, VP]),
     "ms_project": (ctypes.c_int, [_COLP, _FD, VP, VP, U32, VP]),
@@ -403,6 +404,15 @@ class Batch:
 
     def run(self, results, stream=None):
         check(lib().ms_batch_run(self.h, results.p, stream))
+
+    def view_times(self, results, reps=3, stream=None):
+        """ns per view (entry, or block of a blockwise_individual_f entry):
+        the wall-clock time of its work items in a timed run."""
+        n = U64()
+        check(lib().ms_batch_view_times(self.h, results.p, reps, None, 0, ctypes.byref(n), stream))
+        out = np.zeros(max(n.value, 1), np.float64)
+        check(lib().ms_batch_view_times(self.h, results.p, reps, out.ctypes.data, n.value, ctypes.byref(n), stream))
+        return out[:n.value]
 
     def tune(self, results, reps=5, stream=None):
         c = BatchConfig()

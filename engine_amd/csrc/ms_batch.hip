@@ -38,7 +38,7 @@ struct ms_bitem {
     const uint64_t *refs;    // for_f references, delta_f tile carries
     uint64_t first;          // first unit / tile / run / value of the item
     uint64_t ulim;           // BI_DELTA: units of the column's compressed part
-    uint64_t pad2;
+    uint64_t tag;            // timed runs: the view (entry, or block of a blockwise_individual_f entry) it belongs to
     uint64_t plo, phi;       // predicate range
     uint32_t entry;
     uint32_t kind_n;         // kind << 28 | number of units / tiles / runs / values
@@ -116,9 +116,13 @@ __device__ __attribute__((noinline, pure)) uint64_t item_generic(const ms_bentry
 }
 
 // The loop body, instantiated by kernels that differ only in their
-// register budget (waves per SIMD), a choice of the autotuner.
+// register budget (waves per SIMD), a choice of the autotuner.  TIMED adds
+// each item's wall-clock ticks (the constant-rate counter) to its view's
+// slot of `ticks`.
+template <bool TIMED = false>
 __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I,
-                                           uint32_t n_items, uint64_t *__restrict__ part)
+                                           uint32_t n_items, uint64_t *__restrict__ part,
+                                           unsigned long long *__restrict__ ticks = nullptr)
 {
     const uint32_t w = blockIdx.x * WAVES + wave_index();
     const uint32_t W = gridDim.x * WAVES;
@@ -147,6 +151,8 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
         p.neg = it.pflags & 1;
         p.empty = (it.pflags >> 1) & 1;
         const uint32_t kind = it.kind_n >> 28, n = it.kind_n & 0x0fffffffu;
+        uint64_t t_start = 0;
+        if (TIMED) t_start = wall_clock64();
         switch (kind) {
         case BI_STATIC: {
             // unit u of a static_vbp column: 64 bw bytes at u * 64 bw,
@@ -180,6 +186,11 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
             acc += item_generic(&E[it.entry], it.first, n, t);
             break;
         }
+        if (TIMED) {
+            __builtin_amdgcn_s_waitcnt(0);   // the item's loads have returned
+            const uint64_t t_end = wall_clock64();
+            if (t == 0 && it.tag != ~0ull) atomicAdd(ticks + it.tag, (unsigned long long)(t_end - t_start));
+        }
     }
     if (run_entry != 0xffffffffu) {
         const uint64_t s = wave_sum_u64(acc);
@@ -204,6 +215,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I, uint32_t n_items, uint64_t *__restrict__ part)
 {
     batch_loop(E, I, n_items, part);
+}
+
+// The timed run: per view, the wall-clock ticks its items took.
+__global__ __launch_bounds__(WG) void k_batch_select_sum_timed(const ms_bentry *__restrict__ E,
+                                                               const ms_bitem *__restrict__ I, uint32_t n_items,
+                                                               uint64_t *__restrict__ part,
+                                                               unsigned long long *__restrict__ ticks)
+{
+    batch_loop<true>(E, I, n_items, part, ticks);
 }
 
 // One workgroup (256 or 1024 threads) per entry: the sum of its partial
@@ -261,6 +281,7 @@ struct ms_batch {
     std::vector<hipEvent_t> ev;     // kernel timing: a ring of (start, end) pairs
     uint32_t ev_next = 0, ev_used = 0;
     uint32_t holds = 0;             // live plans whose graph reads dE, dI and dP
+    uint64_t ntags = 0;             // views of the queue (timed runs)
 };
 
 static void batch_release(void *p) { ((ms_batch *)p)->holds--; }
@@ -349,6 +370,7 @@ static void make_queue(ms_batch *b)
     const ms_batch_config &cf = b->cfg;
     b->items.clear();
     b->bytes = 0;
+    b->ntags = 0;
     ms_bitem proto = {};
     auto add = [&](uint32_t e, uint32_t kind, uint64_t first, uint64_t n, uint64_t per) {
         for (uint64_t k = 0; k < n; k += per) {
@@ -378,6 +400,7 @@ static void make_queue(ms_batch *b)
         proto.lg_ipo = g.lg_ipo;
         proto.aux = g.bw;
         proto.ulim = pt.n_compr / 512;   // BI_DELTA: units of the compressed part
+        proto.tag = g.kind == MS_BLOCKWISE_INDIVIDUAL ? ~0ull : b->ntags++;
         switch (g.kind) {
         case MS_BLOCKWISE_INDIVIDUAL: {
             // every block's view as items of its own layout
@@ -387,6 +410,7 @@ static void make_queue(ms_batch *b)
             for (uint64_t v = 0; v < h.nb; v++) {
                 const ms_bwi_view &w = h.v[v];
                 const ms_bitem keep = proto;
+                proto.tag = b->ntags++;
                 if (w.code == 0) {
                     proto.data = c->data + w.off;
                     add(e, BI_RAW, 0, B, cf.values_per_item);
@@ -645,6 +669,44 @@ extern "C" int ms_batch_kernel_times(ms_batch *b, float *ms, uint32_t *n)
         MS_HIP(hipEventElapsedTime(&ms[i], b->ev[2 * k], b->ev[2 * k + 1]));
     }
     *n = cnt;
+    return MS_OK;
+}
+
+extern "C" int ms_batch_view_times(ms_batch *b, uint64_t *d_results, uint32_t reps, double *ns, uint64_t cap,
+                                   uint64_t *nviews, void *stream)
+{
+    if (!b || !d_results || !nviews || reps == 0) return ms_fail(MS_ERR_INVALID, "bad argument");
+    if (plan_capturing()) return ms_fail(MS_ERR_INVALID, "timed runs synchronise: not in a plan");
+    MS_TRY(check_unheld(b));
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->built) MS_TRY(build(b, s));
+    *nviews = b->ntags;
+    if (b->E.empty() || b->items.empty()) return MS_OK;
+    int dev = 0, khz = 0;
+    MS_HIP(hipGetDevice(&dev));
+    MS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0) return ms_fail(MS_ERR_HIP, "no wall clock rate");
+    unsigned long long *ticks = nullptr;
+    MS_HIP(hipMallocAsync((void **)&ticks, std::max<uint64_t>(b->ntags, 1) * 8, s));
+    int rc = launch_fill_u64((uint64_t *)ticks, std::max<uint64_t>(b->ntags, 1), 0, s);
+    for (uint32_t r = 0; r < reps && !rc; r++) {
+        hipLaunchKernelGGL(k_batch_select_sum_timed, dim3(b->grid), dim3(WG), 0, s, (const ms_bentry *)b->dE,
+                           (const ms_bitem *)b->dI, (uint32_t)b->items.size(), b->dP, ticks);
+        if (hipGetLastError() != hipSuccess) rc = ms_fail(MS_ERR_HIP, "timed launch");
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_batch_finish, dim3((uint32_t)b->E.size()), dim3(b->finish_threads), 0, s,
+                           (const ms_bentry *)b->dE, (const uint64_t *)b->dP, b->grid * (uint32_t)WAVES, d_results);
+        if (hipGetLastError() != hipSuccess) rc = ms_fail(MS_ERR_HIP, "finish launch");
+    }
+    std::vector<unsigned long long> h(b->ntags);
+    if (!rc && b->ntags &&
+        hipMemcpyAsync(h.data(), ticks, b->ntags * 8, hipMemcpyDeviceToHost, s) != hipSuccess)
+        rc = ms_fail(MS_ERR_HIP, "copy");
+    (void)hipFreeAsync(ticks, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = ms_fail(MS_ERR_HIP, "synchronise");
+    if (rc) return rc;
+    for (uint64_t v = 0; v < b->ntags && v < cap && ns; v++) ns[v] = (double)h[v] / reps * 1e6 / khz;
     return MS_OK;
 }
 

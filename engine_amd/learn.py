@@ -341,6 +341,58 @@ def label(specs, tables, cost, batch=2048, stream=None):
     return d_feat, d_y, sizes, times
 
 
+def label_measured(specs, tables, batch=1024, reps=3, val=1 << 24, stream=None):
+    """Labels the engine generates by compressing and timing: the chunks of
+    `specs` are generated on the device, compressed in every candidate
+    format (one blockwise_individual_f column per candidate, one chunk per
+    block) and run through the persistent kernel's fused select + sum
+    (x < val), one batch per candidate, which records the wall-clock time
+    of every chunk's work items (ms_batch_view_times).  The times are scaled
+    so that each batch's chunk times add up to the batch's run time measured
+    by events: ns of device time per chunk.  Returns device features and targets, host exact sizes and
+    the measured times."""
+    from .formats import blockwise_individual_f
+    bwi = blockwise_individual_f(CHUNK)
+    specs = spec_array(specs)
+    n = len(specs)
+    d_feat = hip.DeviceArray(n * NFEAT, np.float32)
+    sizes = np.zeros((n, NCAND), np.uint64)
+    times = np.zeros((n, NCAND), np.float64)
+    d_vals = hip.DeviceArray(batch * CHUNK, np.uint64)
+    d_st = hip.DeviceArray(batch * NSTAT, np.uint64)
+    d_sz = hip.DeviceArray(batch * NCAND, np.uint64)
+    res = ea.DeviceU64(NCAND)
+    for c0 in range(0, n, batch):
+        m = min(batch, n - c0)
+        gen_chunks(specs[c0:c0 + m], tables, d_vals, 0, stream)
+        chunk_stats(d_vals, m, d_st, None, None, stream)
+        chunk_features(d_st, m, d_feat.ptr(c0 * NFEAT), d_sz, stream)
+        st = d_st.numpy(m * NSTAT).reshape(m, NSTAT)
+        sizes[c0:c0 + m] = d_sz.numpy(m * NCAND).reshape(m, NCAND)
+        whole = ea.from_device(d_vals.ptr(), m * CHUNK)
+        for k in range(NCAND):
+            # one candidate per batch: the chunks share the device with their
+            # own kind only, and the batch's run time, measured by events,
+            # is split over its chunks in proportion to their item times
+            col = ea.morph_blockwise(bwi, whole, format_codes(np.full(m, k), st[:, ST_MAX_BW]))
+            b = ea.Batch()
+            b.enqueue(col, 2, val)
+            b.run(res)
+            hip.synchronize()
+            e0, e1 = hip.Event(), hip.Event()
+            e0.record()
+            for _ in range(reps):
+                b.run(res)
+            e1.record()
+            t_run = e0.elapsed_ms(e1) / reps * 1e6
+            t = b.view_times(res, reps)
+            times[c0:c0 + m, k] = t * (t_run / max(t.sum(), 1e-9))
+            del b, col
+        del whole
+    d_y = hip.DeviceArray.from_numpy(targets(sizes, times).reshape(-1))
+    return d_feat, d_y, sizes, times
+
+
 # ------------------------------------------------------------------ workload
 # blockwise_individual_f format codes of the candidates
 # (format_selector_commons.h:25-103): static_vbp_f at the chunk's maximum
@@ -368,12 +420,13 @@ def column_level_choice(esz, max_bw):
     return k, int(max_bw)
 
 
-def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epochs=30, seed=0x4D6F7270 + 4,
-                           val=1 << 24, steps=10, warmup=2, log=None, cost=None, graph_leg=True):
+def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=30, seed=0x4D6F7270 + 4,
+                           val=1 << 24, steps=10, warmup=2, log=None, cost=None, graph_leg=True, labels="measured"):
     """The learned-selection workload: ncols columns of chunks_per_col
     chunks (a mixture of the format-cascade-sweep distributions switching
-    every 2^16 values).  Trains the selector on ntrain generated chunks,
-    picks a format per chunk, and stores every column as one
+    every 2^16 values).  Trains the selector on ntrain generated chunks
+    (labels measured by the engine, or with labels="table" from the
+    per-(format, bit width) timing calibration), picks a format per chunk, and stores every column as one
     blockwise_individual_f<2^16> column holding each chunk in its pick.
     Times SELECT SUM(x) WHERE x < val over every column: per-column calls
     on whole columns in their column-level best format (plain, eager and as
@@ -386,11 +439,15 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
     bwi = blockwise_individual_f(CHUNK)
     t0 = time.perf_counter()
     tables = GeoTables()
-    if cost is None:
-        cost = calibrate(log=say)
-    t_cal = time.perf_counter() - t0
     specs = training_specs(seed, ntrain, tables)
-    d_x, d_y, _, _ = label(specs, tables, cost)
+    if labels == "measured":
+        d_x, d_y, _, _ = label_measured(specs, tables, val=val)
+    else:
+        if cost is None:
+            cost = calibrate(log=say)
+        d_x, d_y, _, _ = label(specs, tables, cost)
+    t_cal = time.perf_counter() - t0
+    say(f"selector labels ({labels}): {ntrain} chunks in {t_cal:.1f}s")
     net = MLP(seed=seed)
     t1 = time.perf_counter()
     # Adam with the step size halved after each third of the epochs
@@ -497,9 +554,9 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=1_000_000, epoc
         "workload": f"{ncols} uint64 columns of {n} x 2^16 values (a mixture of the format-cascade-sweep "
                     f"distributions switching every 2^16 values), select(x < {val}) + agg_sum per column",
         "layout": "each column one blockwise_individual_f<65536> column, a format code per chunk",
-        "train_chunks": ntrain, "epochs": epochs, "loss_first_epoch": float(losses[0]),
+        "labels": labels, "train_chunks": ntrain, "epochs": epochs, "loss_first_epoch": float(losses[0]),
         "loss_last_epoch": float(losses[-1]), "train_seconds": round(t_train, 2),
-        "calibration_seconds": round(t_cal, 1),
+        "label_seconds": round(t_cal, 1),
         "footprint_bytes": foot, "learned_over_exact": round(foot["learned"] / foot["exact"], 5),
         "exact_fallbacks": fallbacks, "chunks": nchunks,
         "learned_picks": {CAND_NAMES[k]: int(picks[k]) for k in range(NCAND)},

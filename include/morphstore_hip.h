@@ -452,6 +452,17 @@ int ms_batch_kernel_times(ms_batch *b, float *ms, uint32_t *n);
 
 int ms_batch_free(ms_batch *b);
 
+/* No counterpart in the reference: the engine's own timing of its fused
+ * decode, for the learned format selection's labels.  Runs the batch
+ * `reps` times with every work item's wall-clock time added to its view
+ * (an entry, or one block of a blockwise_individual_f entry, in enqueue
+ * order; the raw rest of such an entry is not timed), writes the results
+ * as ms_batch_run does and ns[v] = the mean time of view v's items, summed
+ * over the waves that ran them, for v < cap; *nviews = the number of
+ * views.  Synchronises `stream`. */
+int ms_batch_view_times(ms_batch *b, uint64_t *d_results, uint32_t reps, double *ns, uint64_t cap,
+                        uint64_t *nviews, void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Learned per-chunk format selection (replaces the calibration-driven  */
 /* choice of src/calibration and the exhaustive size search of          */

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import engine_amd as ea
-from engine_amd import hip, learn
+from engine_amd import datagen, hip, learn
 from oracle import chunks as och
 from oracle import mlp as omlp
 
@@ -193,3 +193,43 @@ def test_selection_workload_small():
     assert rep["learned_over_exact"] <= 1.03
     assert rep["footprint_bytes"]["learned"] <= rep["footprint_bytes"]["plain"] * 1.03
     assert rep["chunks"] == 48
+
+
+def test_view_times_follow_bytes():
+    """The timed run: one time per view (a block of a blockwise_individual_f
+    entry, or a whole entry), wider blocks take longer, and the run's sums
+    are the ordinary ones."""
+    from engine_amd.formats import blockwise_individual_f
+    B = 1 << 16
+    narrow = datagen.uniform_bits(SEED + 21, 16 * B, 2)
+    wide = datagen.uniform_bits(SEED + 22, 16 * B, 48)
+    x = np.concatenate([narrow, wide])
+    codes = np.array([2] * 16 + [48] * 16, np.uint8)
+    col = ea.morph_blockwise(blockwise_individual_f(B), ea.upload_values(x), codes)
+    plain = ea.upload_values(narrow[:5000])
+    b = ea.Batch()
+    b.enqueue(col, 2, 1 << 40)
+    b.enqueue(plain, 2, 3)
+    res = ea.DeviceU64(2)
+    t = b.view_times(res, reps=5)
+    assert len(t) == 33 and (t > 0).all()
+    assert t[16:32].mean() > t[:16].mean()
+    got = res.get()
+    assert int(got[0]) == int(np.sum(x[x < np.uint64(1 << 40)], dtype=np.uint64))
+    assert int(got[1]) == int(np.sum(narrow[:5000][narrow[:5000] < 3], dtype=np.uint64))
+
+
+def test_measured_labels():
+    """Labels from compressing and timing: the same exact sizes as the
+    closed forms, positive times, and wider chunks slower in the formats
+    whose size follows the values' width (static_vbp, dynamic_vbp, for_f;
+    the deltas of uniform values wrap to 64 bits at any width)."""
+    tables = learn.GeoTables()
+    specs = [(learn.GEN_UNIFORM, bw, 0, 0, SEED + bw + 7 * k, 0) for k in range(4) for bw in (2, 2, 40, 40)]
+    d_x, d_y, sizes, times = learn.label_measured(specs, tables, batch=8, reps=5)
+    _, _, sizes_t, _ = learn.label(specs, tables, synthetic_cost())
+    assert np.array_equal(sizes, sizes_t)
+    assert times.shape == (16, learn.NCAND) and (times > 0).all()
+    narrow, wide = times[0::4] + times[1::4], times[2::4] + times[3::4]
+    for k in (0, 1, 3):
+        assert wide[:, k].mean() > narrow[:, k].mean(), k
