@@ -40,7 +40,8 @@ CMP_LT = 2
 # autotuner's own pick and its gain over the library default are reported
 # beside it (autotune).
 SWEEP_CFG = {"wg_per_cu": 4, "units_per_item": 32, "tiles_per_item": 8, "runs_per_item": 4096,
-             "values_per_item": 2048, "waves_per_simd": 0}
+             "values_per_item": 2048, "waves_per_simd": 0, "units_per_item_for": 0,
+             "units_per_item_narrow": 0, "unroll": 0}
 
 
 def log(*a):

@@ -135,11 +135,16 @@ class Column:
     def __init__(self, handle, stream=None):
         self.h = handle
         self._stream = stream if isinstance(stream, Stream) else None
+        if self._stream is not None:
+            self._stream._columns += 1
 
     def __del__(self):
         if self.h and _lib is not None:
             _lib.ms_col_free(self.h)
             self.h = None
+        st, self._stream = self._stream, None
+        if st is not None:
+            st._column_released()
 
     def meta(self, stream=None):
         m = ColMeta()
@@ -355,18 +360,7 @@ class Plan:
         check(lib().ms_plan_create(ctypes.byref(self.h), arena_bytes))
 
     def record(self, stream):
-        plan = self
-
-        class _Rec:
-            def __enter__(self):
-                check(lib().ms_plan_begin(plan.h, stream))
-
-            def __exit__(self, exc_type, *exc):
-                rc = lib().ms_plan_end(plan.h, stream)
-                if exc_type is None:        # an operator's own error takes precedence
-                    check(rc)
-                return False
-        return _Rec()
+        return _PlanRecording(self, stream)
 
     def launch(self, stream):
         check(lib().ms_plan_launch(self.h, stream))
@@ -384,6 +378,23 @@ class Plan:
         if self.h and _lib is not None:
             _lib.ms_plan_free(self.h)
             self.h = None
+
+
+class _PlanRecording:
+    """The context of Plan.record (a plain object: no reference cycle keeps
+    the plan or the stream alive after the block)."""
+
+    def __init__(self, plan, stream):
+        self.plan, self.stream = plan, stream
+
+    def __enter__(self):
+        check(lib().ms_plan_begin(self.plan.h, self.stream))
+
+    def __exit__(self, exc_type, *exc):
+        rc = lib().ms_plan_end(self.plan.h, self.stream)
+        if exc_type is None:        # an operator's own error takes precedence
+            check(rc)
+        return False
 
 
 class Batch:

@@ -1,6 +1,8 @@
 // ms_abi.hip -- the C-ABI of include/morphstore_hip.h: column management
 // and operator orchestration.  Entry points never throw; errors are status
 // codes with a per-thread message (ms_last_error).
+#include <mutex>
+#include <unordered_set>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -278,6 +280,11 @@ struct col_caps {
     uint64_t index_cap, refs_cap, tab_bytes;
 };
 
+// Every live column handle: ms_col_free refuses a handle it did not hand
+// out (or already freed) instead of freeing it twice.
+static std::mutex g_cols_mu;
+static std::unordered_set<const ms_col *> g_cols;
+
 // One allocation: [meta 256 B][index][refs][view table][image]
 static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_alloc, hipStream_t s, ms_col **out,
                    const col_caps *caps = nullptr)
@@ -314,6 +321,10 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
         dev_free(base, s);
         delete c;
         return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_cols_mu);
+        g_cols.insert(c);
     }
     *out = c;
     return MS_OK;
@@ -653,6 +664,10 @@ extern "C" uint64_t ms_col_count_bound(const ms_col *c) { return c ? c->count_bo
 extern "C" int ms_col_free(ms_col *c)
 {
     if (!c) return MS_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_cols_mu);
+        if (!g_cols.erase(c)) return ms_fail(MS_ERR_INVALID, "ms_col_free: not a live column handle");
+    }
     if (c->arena) {
         // arena columns die with their plan
     } else if (g_rec && g_rec->recording && g_rec->mode == 1) {
@@ -1429,25 +1444,57 @@ extern "C" int ms_merge_sorted(ms_col **out, const ms_format_desc *out_f, const 
     return emit_new(*out_f, o, d_cnt, bound, wit_chunk(*out_f), 0, sc, out, &app);
 }
 
+// The count calc_binary works on: the operands' common count, or 0 and
+// MS_ERR_SIZE when they differ (scalar/calc_uncompr.h:84-88).
+__global__ void k_calc_count(const ms_dev_meta *l, const ms_dev_meta *r, uint64_t *d_n, uint64_t *d_status)
+{
+    const uint64_t a = l->count, b = r->count;
+    *d_n = a == b ? a : 0;
+    *d_status = a == b ? 0 : (uint64_t)MS_ERR_SIZE;
+}
+
+__global__ void k_keep_status(uint64_t *dst, const uint64_t *src)
+{
+    if (*src) *dst = *src;
+}
+
 extern "C" int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f, const ms_col *l,
                               const ms_col *r, void *stream)
 {
     MS_TRY(ms_need_init());
     if (!out || !l || !r || op > MS_OP_XOR) return ms_fail(MS_ERR_INVALID, "bad argument");
     if (!valid_format(out_f) || !wit_ok(out_f)) return ms_fail(MS_ERR_INVALID, "invalid output format");
-    ms_col_meta ml, mr;
-    MS_TRY(ms_col_get_meta(l, &ml, stream));
-    MS_TRY(ms_col_get_meta(r, &mr, stream));
-    if (ml.count != mr.count)
-        return ms_fail(MS_ERR_SIZE, "calc: inDataLCol and inDataRCol must contain the same number of data elements");
+    if (!plan_capturing()) {
+        // eagerly the mismatch is an error of this call, as in the reference
+        ms_col_meta ml, mr;
+        MS_TRY(ms_col_get_meta(l, &ml, stream));
+        MS_TRY(ms_col_get_meta(r, &mr, stream));
+        if (ml.count != mr.count)
+            return ms_fail(MS_ERR_SIZE, "calc: inDataLCol and inDataRCol must contain the same number of data elements");
+    }
+    // inside a plan the counts are compared on the device: a mismatch
+    // leaves an empty output whose status reads MS_ERR_SIZE (ms_col_check)
     temps sc(S(stream));
     const uint64_t *a, *b, *d_na, *d_nb;
     MS_TRY(values_of(l, sc, &a, &d_na));
     MS_TRY(values_of(r, sc, &b, &d_nb));
-    const uint64_t bound = l->count_bound;
+    uint64_t *d = (uint64_t *)sc.get(64);
+    if (!d) return ms_fail(MS_ERR_NOMEM, "calc scratch");
+    hipLaunchKernelGGL(k_calc_count, dim3(1), dim3(1), 0, S(stream), (const ms_dev_meta *)l->meta,
+                       (const ms_dev_meta *)r->meta, d, d + 1);
+    MS_HIP(hipGetLastError());
+    const uint64_t bound = std::min(l->count_bound, r->count_bound);
     uint64_t *o = (uint64_t *)sc.get(bound * 8 + 64);
     if (!o) return ms_fail(MS_ERR_NOMEM, "calc scratch");
-    MS_TRY(launch_calc(op, a, b, d_na, bound, o, S(stream)));
+    MS_TRY(launch_calc(op, a, b, d, bound, o, S(stream)));
     uint64_t *app;
-    return emit_new(*out_f, o, d_na, bound, wit_chunk(*out_f), 0, sc, out, &app);
+    ms_col *c;
+    MS_TRY(emit_new(*out_f, o, d, bound, wit_chunk(*out_f), 0, sc, &c, &app));
+    hipLaunchKernelGGL(k_keep_status, dim3(1), dim3(1), 0, S(stream), &c->meta->status, (const uint64_t *)(d + 1));
+    if (hipGetLastError() != hipSuccess) {
+        ms_col_free(c);
+        return ms_fail(MS_ERR_HIP, "calc status");
+    }
+    *out = c;
+    return MS_OK;
 }

@@ -52,7 +52,7 @@ static_assert(sizeof(ms_bitem) == 96, "work item layout");
 
 // The item's n <= 64 units: lane k holds unit u0 + k's index entry (and
 // for_f reference), loaded together.
-template <bool IS_FOR>
+template <bool IS_FOR, int NU>
 __device__ __forceinline__ void item_blocks(const ms_bentry &en, const ms_bitem &it, const ms_pred &p, uint32_t n,
                                             uint32_t t, uint32_t j, uint32_t gl, uint64_t &acc)
 {
@@ -65,16 +65,16 @@ __device__ __forceinline__ void item_blocks(const ms_bentry &en, const ms_bitem 
         if (IS_FOR) my_r = gload(it.refs + (((u0 + t) >> lg_spb) >> lg_ipo));
     }
     const bool le = IS_FOR && pred_le_form(p);
-    for (uint32_t k = 0; k < n; k += 4) {
-        const uint32_t cnt = n - k < 4 ? n - k : 4u;
-        uint64_t e[4], r[4];
+    for (uint32_t k = 0; k < n; k += NU) {
+        const uint32_t cnt = n - k < (uint32_t)NU ? n - k : (uint32_t)NU;
+        uint64_t e[4] = {0, 0, 0, 0}, r[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < NU; u++) {
             e[u] = readlane64(my_e, k + u);
             r[u] = IS_FOR ? readlane64(my_r, k + u) : 0ull;
         }
-        if (le) vbp_units_ss<IS_FOR, true>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
-        else vbp_units_ss<IS_FOR, false>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
+        if (le) vbp_units_ss<IS_FOR, true, NU>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
+        else vbp_units_ss<IS_FOR, false, NU>(it.data, e, r, cnt, u0 + k, mask, p, en.tab, j, gl, acc);
     }
 }
 
@@ -119,7 +119,7 @@ __device__ __attribute__((noinline, pure)) uint64_t item_generic(const ms_bentry
 // register budget (waves per SIMD), a choice of the autotuner.  TIMED adds
 // each item's wall-clock ticks (the constant-rate counter) to its view's
 // slot of `ticks`.
-template <bool TIMED = false>
+template <bool TIMED = false, int NU = 4>
 __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I,
                                            uint32_t n_items, uint64_t *__restrict__ part,
                                            unsigned long long *__restrict__ ticks = nullptr)
@@ -158,20 +158,17 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
             // unit u of a static_vbp column: 64 bw bytes at u * 64 bw,
             // given to the indexed-layout routine as index entries
             const uint32_t bw = it.aux;
-            for (uint32_t k = 0; k < n; k += 4) {
-                const uint32_t cnt = n - k < 4 ? n - k : 4u;
-                uint64_t e[4], r[4];
+            for (uint32_t k = 0; k < n; k += NU) {
+                const uint32_t cnt = n - k < (uint32_t)NU ? n - k : (uint32_t)NU;
+                uint64_t e[4] = {0, 0, 0, 0}, r[4] = {0, 0, 0, 0};
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    e[u] = (((it.first + k + u) * 64ull * bw) << 8) | bw;
-                    r[u] = 0;
-                }
-                vbp_units_ss<false, false>(it.data, e, r, cnt, 0, 0, p, en.tab, j, gl, acc);
+                for (int u = 0; u < NU; u++) e[u] = (((it.first + k + u) * 64ull * bw) << 8) | bw;
+                vbp_units_ss<false, false, NU>(it.data, e, r, cnt, 0, 0, p, en.tab, j, gl, acc);
             }
             break;
         }
-        case BI_BLOCKS: item_blocks<false>(en, it, p, n, t, j, gl, acc); break;
-        case BI_FOR: item_blocks<true>(en, it, p, n, t, j, gl, acc); break;
+        case BI_BLOCKS: item_blocks<false, NU>(en, it, p, n, t, j, gl, acc); break;
+        case BI_FOR: item_blocks<true, NU>(en, it, p, n, t, j, gl, acc); break;
         case BI_DELTA: item_delta(it, p, n, t, t >> 3, t & 7, acc); break;   // lane-major threads
         case BI_RLE: acc += rle_runs_ss((const u64x2 *)it.data, it.first, it.first + n, p, t); break;
         case BI_RAW: {
@@ -215,6 +212,21 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void
     const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I, uint32_t n_items, uint64_t *__restrict__ part)
 {
     batch_loop(E, I, n_items, part);
+}
+
+// The same with two units' words in flight per wave instead of four (the
+// tuner's unroll choice: fewer registers per wave, more waves).
+__global__ __launch_bounds__(WG) void k_batch_select_sum_u2(const ms_bentry *__restrict__ E,
+                                                            const ms_bitem *__restrict__ I, uint32_t n_items,
+                                                            uint64_t *__restrict__ part)
+{
+    batch_loop<false, 2>(E, I, n_items, part);
+}
+
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_batch_select_sum_w8_u2(
+    const ms_bentry *__restrict__ E, const ms_bitem *__restrict__ I, uint32_t n_items, uint64_t *__restrict__ part)
+{
+    batch_loop<false, 2>(E, I, n_items, part);
 }
 
 // The timed run: per view, the wall-clock ticks its items took.
@@ -295,14 +307,16 @@ static int check_unheld(const ms_batch *b)
     return MS_OK;
 }
 
-static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0};
+static const ms_batch_config k_default_cfg = {4, 16, 2, 4096, 8192, 0, 0, 0, 0};
 
 static bool cfg_ok(const ms_batch_config &c)
 {
     return c.wg_per_cu >= 1 && c.wg_per_cu <= 16 && c.units_per_item >= 1 && c.units_per_item <= 64 &&
            c.tiles_per_item >= 1 && c.tiles_per_item <= 16 && c.runs_per_item >= 64 &&
            c.runs_per_item <= (1u << 27) && c.values_per_item >= 64 && c.values_per_item <= (1u << 27) &&
-           (c.waves_per_simd == 0 || c.waves_per_simd == 6 || c.waves_per_simd == 8);
+           (c.waves_per_simd == 0 || c.waves_per_simd == 6 || c.waves_per_simd == 8) &&
+           c.units_per_item_for <= 64 && c.units_per_item_narrow <= 64 &&
+           (c.unroll == 0 || c.unroll == 4 || (c.unroll == 2 && c.waves_per_simd != 6));
 }
 
 extern "C" int ms_batch_create(ms_batch **out)
@@ -368,6 +382,9 @@ static int ensure(T **p, size_t &cap, size_t n)
 static void make_queue(ms_batch *b)
 {
     const ms_batch_config &cf = b->cfg;
+    // item sizes per format and bit-width class (0: the bit-packed default)
+    const uint32_t upi_for = cf.units_per_item_for ? cf.units_per_item_for : cf.units_per_item;
+    const uint32_t upi_narrow = cf.units_per_item_narrow ? cf.units_per_item_narrow : cf.units_per_item;
     b->items.clear();
     b->bytes = 0;
     b->ntags = 0;
@@ -417,7 +434,7 @@ static void make_queue(ms_batch *b)
                 } else if (w.code <= 64) {
                     proto.data = c->data + w.off;
                     proto.aux = w.code;
-                    add(e, BI_STATIC, 0, B / 512, cf.units_per_item);
+                    add(e, BI_STATIC, 0, B / 512, w.code <= 8 ? upi_narrow : cf.units_per_item);
                 } else if (w.code == MS_BWI_RLE) {
                     proto.data = c->data + w.off;
                     add(e, BI_RLE, 0, w.size / 16, cf.runs_per_item);
@@ -434,7 +451,8 @@ static void make_queue(ms_batch *b)
                         add(e, BI_DELTA, 0, (B / 512 + 3) / 4, cf.tiles_per_item);
                     } else {
                         proto.refs = k == BWK_FOR ? c->refs + h.refs_off[k] + (uint64_t)w.slot * (B / 1024) : nullptr;
-                        add(e, k == BWK_FOR ? BI_FOR : BI_BLOCKS, 0, B / 512, cf.units_per_item);
+                        add(e, k == BWK_FOR ? BI_FOR : BI_BLOCKS, 0, B / 512,
+                            k == BWK_FOR ? upi_for : cf.units_per_item);
                     }
                 }
                 proto = keep;
@@ -447,7 +465,7 @@ static void make_queue(ms_batch *b)
         case MS_UNCOMPR:
             break;
         case MS_STATIC_VBP:
-            if (g.lanes == 8) add(e, BI_STATIC, 0, pt.n_compr / 512, cf.units_per_item);
+            if (g.lanes == 8) add(e, BI_STATIC, 0, pt.n_compr / 512, g.bw <= 8 ? upi_narrow : cf.units_per_item);
             else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item);
             break;
         case MS_DYNAMIC_VBP:
@@ -457,7 +475,8 @@ static void make_queue(ms_batch *b)
             const bool blocks = g.leaf == MS_DYNAMIC_VBP && g.lanes == 8 && g.lg_spb != 0xff &&
                                 (g.kind == MS_DYNAMIC_VBP || g.lg_ipo != 0xff);
             if (g.kind == MS_DELTA) add(e, BI_DELTA, 0, (units + 3) / 4, cf.tiles_per_item);
-            else if (blocks) add(e, g.kind == MS_FOR ? BI_FOR : BI_BLOCKS, 0, units, cf.units_per_item);
+            else if (blocks) add(e, g.kind == MS_FOR ? BI_FOR : BI_BLOCKS, 0, units,
+                                 g.kind == MS_FOR ? upi_for : cf.units_per_item);
             else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item);
             break;
         }
@@ -508,8 +527,10 @@ static int launch(ms_batch *b, uint64_t *d_results, hipStream_t s)
 {
     const uint64_t nE = b->E.size();
     if (nE == 0) return MS_OK;
-    auto kern = b->cfg.waves_per_simd == 8 ? k_batch_select_sum_w8
-              : b->cfg.waves_per_simd == 6 ? k_batch_select_sum_w6 : k_batch_select_sum;
+    const bool u2 = b->cfg.unroll == 2;
+    auto kern = b->cfg.waves_per_simd == 8 ? (u2 ? k_batch_select_sum_w8_u2 : k_batch_select_sum_w8)
+              : b->cfg.waves_per_simd == 6 ? k_batch_select_sum_w6
+              : (u2 ? k_batch_select_sum_u2 : k_batch_select_sum);
     const uint32_t slots = plan_capturing() ? 0 : (uint32_t)b->ev.size() / 2;
     const uint32_t k = slots ? b->ev_next % slots : 0;
     if (slots) MS_HIP(hipEventRecord(b->ev[2 * k], s));
@@ -579,17 +600,33 @@ extern "C" int ms_batch_tune(ms_batch *b, uint64_t *d_results, uint32_t reps, ms
     static const uint32_t rpi[] = {1024, 4096, 16384};
     static const uint32_t vpi[] = {2048, 8192, 32768};
     static const uint32_t wps[] = {0, 6, 8};
+    static const uint32_t unr[] = {4, 2};
+    static const uint32_t upo[] = {0, 8, 16, 32, 64};   // for_f items; 0: as the other bit-packed layouts
+    static const uint32_t upn[] = {0, 16, 32, 64};      // static_vbp of at most 8 bits
+    struct axis {
+        const uint32_t *vals;
+        int nv;
+        uint32_t ms_batch_config::*field;
+    };
+    static const axis axes[] = {{wps, 3, &ms_batch_config::waves_per_simd},
+                                {unr, 2, &ms_batch_config::unroll},
+                                {wg, 5, &ms_batch_config::wg_per_cu},
+                                {upi, 6, &ms_batch_config::units_per_item},
+                                {upo, 5, &ms_batch_config::units_per_item_for},
+                                {upn, 4, &ms_batch_config::units_per_item_narrow},
+                                {tpi, 4, &ms_batch_config::tiles_per_item},
+                                {rpi, 3, &ms_batch_config::runs_per_item},
+                                {vpi, 3, &ms_batch_config::values_per_item}};
     for (int sweep = 0; sweep < 2; sweep++) {
         bool changed = false;
-        for (int f = 0; f < 6; f++) {
-            const uint32_t *vals = f == 0 ? wps : f == 1 ? wg : f == 2 ? upi : f == 3 ? tpi : f == 4 ? rpi : vpi;
-            const int nv = f == 0 ? 3 : f == 1 ? 5 : f == 2 ? 6 : f == 3 ? 4 : 3;
-            for (int k = 0; k < nv; k++) {
+        for (const axis &a : axes) {
+            for (int k = 0; k < a.nv; k++) {
                 ms_batch_config c = cur;
-                uint32_t *field = f == 0 ? &c.waves_per_simd : f == 1 ? &c.wg_per_cu : f == 2 ? &c.units_per_item
-                                : f == 3 ? &c.tiles_per_item : f == 4 ? &c.runs_per_item : &c.values_per_item;
-                if (*field == vals[k]) continue;
-                *field = vals[k];
+                uint32_t *field = &(c.*a.field);
+                if (*field == a.vals[k] || (a.field == &ms_batch_config::unroll && *field == 0 && a.vals[k] == 4))
+                    continue;
+                *field = a.vals[k];
+                if (!cfg_ok(c)) continue;
                 float t;
                 MS_TRY(time_cfg(b, c, d_results, reps, s, &t));
                 if (t < cur_ms * 0.99f) {

@@ -400,22 +400,28 @@ __device__ __forceinline__ void vbp_unit_finish(const uint8_t *data, uint64_t e,
     else ss_cut_sum_dispatch<IS_FOR>(bw, w, g, mode, lo, span, r, acc);
 }
 
-template <bool IS_FOR, bool LEFORM>
+// NU (2 or 4) units: all their words requested before the first is cut.
+template <bool IS_FOR, bool LEFORM, int NU = 4>
 __device__ __forceinline__ void vbp_units_ss(const uint8_t *data, const uint64_t (&e)[4], const uint64_t (&r)[4],
                                              uint32_t cnt, uint64_t u0, uint64_t mask, const ms_pred &p,
                                              const ms_bwtab &tab, uint32_t j, uint32_t gl, uint64_t &acc)
 {
-    // four named word arrays (not one indexed array): they stay in registers
-    uint32_t m0, m1, m2, m3, l0, l1, l2, l3, s0, s1, s2, s3;
+    static_assert(NU == 2 || NU == 4, "units per group");
+    // named word arrays (not one indexed array): they stay in registers
+    uint32_t m0, m1, m2 = 0, m3 = 0, l0, l1, l2 = 0, l3 = 0, s0, s1, s2 = 0, s3 = 0;
     uint64_t w0[5], w1[5], w2[5], w3[5];
     vbp_unit_prep<IS_FOR, LEFORM>(data, e[0], r[0], cnt > 0, u0, mask, p, tab, j, gl, m0, l0, s0, w0);
     vbp_unit_prep<IS_FOR, LEFORM>(data, e[1], r[1], cnt > 1, u0 + 1, mask, p, tab, j, gl, m1, l1, s1, w1);
-    vbp_unit_prep<IS_FOR, LEFORM>(data, e[2], r[2], cnt > 2, u0 + 2, mask, p, tab, j, gl, m2, l2, s2, w2);
-    vbp_unit_prep<IS_FOR, LEFORM>(data, e[3], r[3], cnt > 3, u0 + 3, mask, p, tab, j, gl, m3, l3, s3, w3);
+    if (NU == 4) {
+        vbp_unit_prep<IS_FOR, LEFORM>(data, e[2], r[2], cnt > 2, u0 + 2, mask, p, tab, j, gl, m2, l2, s2, w2);
+        vbp_unit_prep<IS_FOR, LEFORM>(data, e[3], r[3], cnt > 3, u0 + 3, mask, p, tab, j, gl, m3, l3, s3, w3);
+    }
     vbp_unit_finish<IS_FOR>(data, e[0], r[0], u0, mask, p, j, gl, m0, l0, s0, w0, acc);
     vbp_unit_finish<IS_FOR>(data, e[1], r[1], u0 + 1, mask, p, j, gl, m1, l1, s1, w1, acc);
-    vbp_unit_finish<IS_FOR>(data, e[2], r[2], u0 + 2, mask, p, j, gl, m2, l2, s2, w2, acc);
-    vbp_unit_finish<IS_FOR>(data, e[3], r[3], u0 + 3, mask, p, j, gl, m3, l3, s3, w3, acc);
+    if (NU == 4) {
+        vbp_unit_finish<IS_FOR>(data, e[2], r[2], u0 + 2, mask, p, j, gl, m2, l2, s2, w2, acc);
+        vbp_unit_finish<IS_FOR>(data, e[3], r[3], u0 + 3, mask, p, j, gl, m3, l3, s3, w3, acc);
+    }
 }
 
 // ------------------------------------------------------------------

@@ -86,7 +86,8 @@ class ColMeta(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     """ms_batch_config."""
     _fields_ = [(n, ctypes.c_uint32) for n in ("wg_per_cu", "units_per_item", "tiles_per_item", "runs_per_item",
-                                                "values_per_item", "waves_per_simd")]
+                                                "values_per_item", "waves_per_simd", "units_per_item_for",
+                                                "units_per_item_narrow", "unroll")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -50,6 +50,8 @@ class Stream:
 
     def __init__(self):
         self.s = ctypes.c_void_p()
+        self._columns = 0          # live columns created on this stream
+        self._orphaned = False     # finalised while columns still used it
         _ok(hip().hipStreamCreate(ctypes.byref(self.s)), "hipStreamCreate")
 
     def synchronize(self):
@@ -63,9 +65,24 @@ class Stream:
     def _as_parameter_(self):
         return self.s
 
-    def __del__(self):
+    def _destroy(self):
         if self.s and _hip is not None:
             _hip.hipStreamDestroy(self.s)
+        self.s = ctypes.c_void_p()
+
+    def _column_released(self):
+        self._columns -= 1
+        if self._orphaned and self._columns <= 0:
+            self._destroy()
+
+    def __del__(self):
+        # In a reference cycle the finalisers run in no particular order: a
+        # column released later still frees its memory in order on this
+        # stream, so the stream outlives the last such column.
+        if self._columns > 0:
+            self._orphaned = True
+        else:
+            self._destroy()
 
 
 class DeviceArray:

@@ -358,7 +358,10 @@ int ms_semi_join(ms_col **out, const ms_format_desc *out_f, const ms_col *l,
                  const ms_col *r, void *stream);
 
 /* calc_binary<op, ve, out_f, l_f, r_f>(l, r): out_i = op(l_i, r_i) mod 2^64.
- * Counts must match (MS_ERR_SIZE otherwise, scalar/calc_uncompr.h:84-88). */
+ * Counts must match (MS_ERR_SIZE otherwise, scalar/calc_uncompr.h:84-88):
+ * eagerly the call fails; inside a plan the counts are compared on the
+ * device and a mismatch yields an empty column whose ms_col_check reports
+ * MS_ERR_SIZE. */
 int ms_calc_binary(ms_col **out, uint32_t op, const ms_format_desc *out_f,
                    const ms_col *l, const ms_col *r, void *stream);
 
@@ -379,7 +382,8 @@ int ms_plan_create(ms_plan **out, uint64_t arena_bytes);
  * the default one) by the calling thread.  While capturing, their output
  * columns and temporaries live in the plan's arena (ms_col_free of such a
  * column releases only the handle) and no call may synchronise (ms_col_*
- * meta/download calls and ms_calc_binary are not allowed). */
+ * meta/download calls, group / join and blockwise_individual_f compression
+ * are not allowed). */
 int ms_plan_begin(ms_plan *p, void *stream);
 
 /* Ends recording; after a capturing pass, instantiates the graph. */
@@ -411,6 +415,10 @@ typedef struct ms_batch_config {
     uint32_t values_per_item;  /* values per work item (uncompressed parts)      */
     uint32_t waves_per_simd;   /* register budget of the kernel: 0 (the compiler's),
                                   6 or 8 resident waves per SIMD                 */
+    uint32_t units_per_item_for;    /* units per item of for_f layouts (0: units_per_item) */
+    uint32_t units_per_item_narrow; /* units per item of static_vbp of <= 8 bits (0: units_per_item) */
+    uint32_t unroll;           /* units whose words one wave requests together
+                                  before decoding: 4 (0) or 2                   */
 } ms_batch_config;
 
 int ms_batch_create(ms_batch **out);

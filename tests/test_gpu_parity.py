@@ -9,7 +9,7 @@ import pytest
 
 import engine_amd as ea
 from engine_amd import datagen
-from engine_amd.formats import (MS_CMP_EQ, MS_CMP_GE, MS_CMP_GT, MS_CMP_LE, MS_CMP_LT, MS_CMP_NE, MS_OP_ADD,
+from engine_amd.formats import (MS_CMP_EQ, MS_CMP_GE, MS_CMP_GT, MS_CMP_LE, MS_CMP_LT, MS_CMP_NE, MS_ERR_SIZE, MS_OP_ADD,
                                 default_delta_dynamic_vbp_f, default_dynamic_vbp_f, default_for_dynamic_vbp_f,
                                 delta_f, dynamic_vbp_f, for_f, rle_f, static_vbp_f, uncompr_f)
 from tests import refmodel as rm
@@ -330,6 +330,43 @@ def test_plan_graph_replays_match_eager(oracle):
         assert other == rm.masked_sum(ms, MS_CMP_LT, 100)
     del keep, res
     plan.free()
+
+
+def test_calc_binary_in_a_plan():
+    """calc_binary recorded into a plan: the count check runs on the device, so
+    replays compute the same column as the eager call, and a count mismatch
+    inside a plan shows as MS_ERR_SIZE in the output's status."""
+    from engine_amd import hip
+    n = 40_000 + 3
+    x = datagen.uniform_bits(11, n, 30)
+    y = datagen.uniform_bits(12, n, 30)
+    st = hip.Stream()
+    lx = ea.morph(default_dynamic_vbp_f(8), ea.upload_values(x), st.handle)
+    ly = ea.upload_values(y)
+    want = (x + y).tolist()
+    assert ea.calc_binary(MS_OP_ADD, uncompr_f(), lx, ly, st.handle).values(st.handle).tolist() == want
+    plan = ea.Plan()
+    with plan.record(st.handle):
+        ea.calc_binary(MS_OP_ADD, uncompr_f(), lx, ly, st.handle)
+    with plan.record(st.handle):
+        out = ea.calc_binary(MS_OP_ADD, uncompr_f(), lx, ly, st.handle)
+    for rep in range(3):
+        plan.launch(st.handle)
+        st.synchronize()
+        assert out.values(st.handle).tolist() == want, rep
+    plan.free()
+    short = ea.upload_values(y[:-1])
+    with pytest.raises(ea.MsError):
+        ea.calc_binary(MS_OP_ADD, uncompr_f(), lx, short, st.handle)
+    p2 = ea.Plan(64 << 20)   # an arena given: the first recording captures (no eager measuring pass)
+    with p2.record(st.handle):
+        bad = ea.calc_binary(MS_OP_ADD, uncompr_f(), lx, short, st.handle)
+    p2.launch(st.handle)
+    st.synchronize()
+    with pytest.raises(ea.MsError) as e:
+        bad.check(st.handle)
+    assert e.value.code == MS_ERR_SIZE
+    p2.free()
 
 
 PRA_FORMATS = [static_vbp_f(12, 1), static_vbp_f(7, 8), static_vbp_f(40, 8), dynamic_vbp_f(512, 2048, 8),
