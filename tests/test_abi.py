@@ -55,3 +55,13 @@ def test_size_functions_match_oracle(oracle):
         for n in [0, 1, 511, 512, 100000]:
             assert lib.ms_size_max_byte_any_len(ctypes.byref(f), n, ctypes.byref(out)) == 0
             assert out.value == oracle.size_max_byte_any_len(f, n), (f, n)
+
+
+def test_col_free_refuses_unknown_handles():
+    """ms_col_free frees only handles the library handed out (a second free
+    or a foreign pointer is refused, not dereferenced); no GPU needed."""
+    lib = ctypes.CDLL(ea.LIB_PATH)
+    lib.ms_col_free.argtypes = [ctypes.c_void_p]
+    assert lib.ms_col_free(None) == 0
+    buf = ctypes.create_string_buffer(512)
+    assert lib.ms_col_free(ctypes.cast(buf, ctypes.c_void_p)) == 1   # MS_ERR_INVALID
