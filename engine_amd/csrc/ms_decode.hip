@@ -935,6 +935,7 @@ __global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, c
     const uint32_t t = threadIdx.x & 63;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nunits] = 0;   // the scan's total slot
     uint64_t e_n, r_n;   // the next unit's entry, requested one unit ahead
     unit_entry(c, ncu, u, e_n, r_n);
     for (; u < nunits; u += W) {
@@ -990,6 +991,41 @@ __global__ __launch_bounds__(WG) void k_sel_compact_blocks(const uint64_t *flags
         if (u == nunits - 1 && t == 0) *d_cnt = o;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && nunits == 0) *d_cnt = 0;
+}
+
+bool select_flags_ok(const ms_col *in)
+{
+    ms_colv v;
+    if (colv_of(in, v) != MS_OK) return false;
+    return (v.g.kind == MS_DYNAMIC_VBP || v.g.kind == MS_FOR) && fast_blocks(v);
+}
+
+// The count pass of the block-layout select and the scan of its unit
+// counts (with the total at uoff[units]); no compaction.
+int launch_select_flags(const ms_col *in, const ms_pred &p, sel_flags &sf, temps &sc)
+{
+    hipStream_t s = sc.s;
+    const uint64_t nb = in->count_bound;
+    ms_colv v;
+    MS_TRY(colv_of(in, v));
+    if (!((v.g.kind == MS_DYNAMIC_VBP || v.g.kind == MS_FOR) && fast_blocks(v)))
+        return ms_fail(MS_ERR_INVALID, "select flags: layout without a block decoder");
+    const uint64_t units = nb / 512 + 1;
+    uint64_t *d_u = (uint64_t *)sc.get(64);
+    uint64_t *cnt = (uint64_t *)sc.get(units * 8 + 64);
+    uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
+    uint64_t *flags = (uint64_t *)sc.get(units * 64 + 64);
+    if (!d_u || !cnt || !off || !flags) return ms_fail(MS_ERR_NOMEM, "select scratch");
+    hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
+    hipLaunchKernelGGL(k_sel_count_blocks, dim3(units_grid(nb)), dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt,
+                       flags);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 1, units + 1, sc));
+    sf.flags = flags;
+    sf.uoff = off;
+    sf.d_units = d_u;
+    sf.units_bound = units;
+    return MS_OK;
 }
 
 int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, temps &sc)

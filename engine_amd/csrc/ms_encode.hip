@@ -117,26 +117,137 @@ __device__ __forceinline__ uint64_t out_pos(const emit_layout &l, const uint64_t
     return l.cout[c] + start[b] - start[c * bpc];
 }
 
-__device__ __forceinline__ uint64_t xform(const emit_layout &l, const uint64_t *v, uint64_t gi, uint64_t ref)
+// xform over a value getter: get(gi) = value gi of the call's array.
+template <class G>
+__device__ __forceinline__ uint64_t xform_g(const emit_layout &l, const G &get, uint64_t gi, uint64_t ref)
 {
-    const uint64_t x = v[gi];
-    if (l.kind == MS_DELTA) return x - (gi >= l.dstep ? v[gi - l.dstep] : 0ull);
+    const uint64_t x = get(gi);
+    if (l.kind == MS_DELTA) return x - (gi >= l.dstep ? get(gi - l.dstep) : 0ull);
     if (l.kind == MS_FOR) return x - ref;
     return x;
 }
 
-// for.h:121-161 reference value of one outer block, one wave per block.
-__global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *v, const uint64_t *d_m, uint64_t *refs)
+// ------------------------------------------------------------------
+// Values straight from a select's flags (no materialised positions): the
+// value of rank r is the row of the r-th selected row.
+// ------------------------------------------------------------------
+struct flag_src {
+    sel_flags sf;
+    const uint32_t *ustart;   // per innermost block b: the unit holding rank r0(b)
+    uint32_t pred;            // delta predecessors a block reads (0 unless delta_f)
+};
+
+// First rank a block's kernels read: its predecessors included.
+__device__ __forceinline__ uint64_t flag_r0(const emit_layout &l, const flag_src &fs, uint64_t b)
 {
+    const uint64_t first = b * l.B;
+    return first >= fs.pred ? first - fs.pred : 0ull;
+}
+
+// One wave writes the rows of ranks [r0, r0 + cnt) to st(rank - r0, row),
+// walking the units from u (a unit at or before the one holding rank r0)
+// eight at a time.  Lane t takes rows 64 gl .. 64 gl + 63 of unit
+// u + t / 8 (gl = t & 7): row 64 gl + 8 m + j of a unit is bit 8 gl + j of
+// its ballot m (the count pass's thread mapping), so the lane's row word
+// is byte gl of each of the unit's 8 ballots.  Its first rank is the
+// unit's offset plus the set bits of the unit's lower row words (a scan
+// over the lane's group of 8); then it writes its set bits in order.
+template <class St>
+__device__ __forceinline__ void gather_ranks(const sel_flags &sf, uint64_t nunits, uint64_t u, uint64_t r0,
+                                             uint64_t cnt, const St &st)
+{
+    const uint32_t t = threadIdx.x & 63, gl = t & 7;
+    const uint64_t r1 = r0 + cnt;
+    if (cnt == 0 || cnt >= (1ull << 31)) return;   // callers ask for at most a few thousand ranks
+    for (; u < nunits; u += 8) {
+        const uint64_t uu = u + (t >> 3);
+        const uint64_t ue = u + 8 < nunits ? u + 8 : nunits;
+        const uint64_t end_rank = gload(sf.uoff + ue);   // requested with the group's loads
+        uint64_t rw = 0, rank = 0;
+        if (uu < nunits) {
+            uint64_t b[8];
+#pragma unroll
+            for (int m = 0; m < 8; m++) b[m] = gload(sf.flags + 8 * uu + m);
+            rank = gload(sf.uoff + uu);
+            const uint32_t sh = 8 * gl;
+#pragma unroll
+            for (int m = 0; m < 8; m++) rw |= ((b[m] >> sh) & 0xffull) << (8 * m);
+        }
+        // exclusive scan of the row words' counts within the group of 8
+        const uint32_t c = (uint32_t)__popcll(rw);
+        uint32_t incl = c;
+#pragma unroll
+        for (uint32_t o = 1; o < 8; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 8);
+            if (gl >= o) incl += y;
+        }
+        rank += incl - c;
+        if (rank >= r1 || rank + c <= r0) rw = 0;
+        // a lane with bits left has rank in (r0 - 64, r1): 32-bit from here
+        int32_t rel = (int32_t)(int64_t)(rank - r0);
+        const uint32_t n32 = (uint32_t)cnt;
+        const uint64_t row0 = uu * 512 + 64 * gl;
+        uint32_t x = (uint32_t)rw;
+        while (x) {
+            const uint32_t p = (uint32_t)__builtin_ctz(x);
+            if ((uint32_t)rel < n32) st((uint64_t)rel, row0 + p);
+            rel++;
+            x &= x - 1;
+        }
+        x = (uint32_t)(rw >> 32);
+        while (x) {
+            const uint32_t p = (uint32_t)__builtin_ctz(x) + 32;
+            if ((uint32_t)rel < n32) st((uint64_t)rel, row0 + p);
+            rel++;
+            x &= x - 1;
+        }
+        if (end_rank >= r1) break;
+    }
+}
+
+// ustart[b] for every block whose first read rank r0(b) lies in unit u's
+// ranks [uoff[u], uoff[u+1]); one thread per unit.
+__global__ __launch_bounds__(WG) void k_flag_starts(emit_layout l, flag_src fs, uint64_t cap, uint32_t *ustart)
+{
+    const uint64_t nunits = *fs.sf.d_units;
+    for (uint64_t u = (uint64_t)blockIdx.x * WG + threadIdx.x; u < nunits; u += (uint64_t)gridDim.x * WG) {
+        const uint64_t lo = fs.sf.uoff[u], hi = fs.sf.uoff[u + 1];
+        if (lo == hi) continue;
+        // r0(b) = max(b B - pred, 0): b = 0 starts at rank 0
+        uint64_t b = (lo + fs.pred + l.B - 1) / l.B;
+        if (lo == 0) {
+            if (cap > 0) ustart[0] = (uint32_t)u;
+            if (b == 0) b = 1;
+        }
+        for (; b < cap && b * l.B - fs.pred < hi; b++) ustart[b] = (uint32_t)u;
+    }
+}
+
+// for.h:121-161 reference value of one outer block, one wave per block.
+// FL: the values come from a select's flags (dynamic LDS: L words).
+template <bool FL>
+__global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *v, flag_src fs, const uint64_t *d_m,
+                                                 uint64_t *refs)
+{
+    extern __shared__ unsigned long long fvals[];
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
+    const uint64_t nunits = FL ? *fs.sf.d_units : 0;
     for (uint64_t fb = blockIdx.x; fb < pl.nfb; fb += gridDim.x) {
         uint64_t lo;
-        const uint64_t *x = chunk_src(l, v, fb * l.L, lo) + lo;
+        const uint64_t *xg = nullptr;
+        if (FL) {
+            gather_ranks(fs.sf, nunits, fs.ustart[fb * l.L / l.B], fb * l.L, l.L,
+                         [&](uint64_t i, uint64_t row) { fvals[i] = row; });
+            __syncthreads();
+        } else {
+            xg = chunk_src(l, v, fb * l.L, lo) + lo;
+        }
+        const auto x = [&](uint64_t i) -> uint64_t { return FL ? (uint64_t)fvals[i] : xg[i]; };
         uint64_t r;
         if (l.E == 1) {
             r = ~0ull;
-            for (uint64_t i = t; i < l.L; i += 64) r = x[i] < r ? x[i] : r;
+            for (uint64_t i = t; i < l.L; i += 64) r = x(i) < r ? x(i) : r;
             for (int o = 32; o >= 1; o >>= 1) {
                 const uint64_t y = __shfl_xor(r, o, 64);
                 r = y < r ? y : r;
@@ -144,7 +255,7 @@ __global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *
         } else {
             // lane j = i % E: signed minimum per lane, then unsigned over lanes
             int64_t sm = INT64_MAX;
-            for (uint64_t i = t; i < l.L; i += 64) sm = (int64_t)x[i] < sm ? (int64_t)x[i] : sm;
+            for (uint64_t i = t; i < l.L; i += 64) sm = (int64_t)x(i) < sm ? (int64_t)x(i) : sm;
             for (uint32_t o = 32; o >= l.E; o >>= 1) {
                 const int64_t y = (int64_t)__shfl_xor((uint64_t)sm, o, 64);
                 sm = y < sm ? y : sm;
@@ -156,6 +267,145 @@ __global__ __launch_bounds__(64) void k_for_refs(emit_layout l, const uint64_t *
             }
         }
         if (t == 0) refs[fb] = r == ~0ull ? r - 1 : r;
+        if (FL) __syncthreads();
+    }
+}
+
+// Bit width of innermost block b over a value getter (one wave; lane t).
+template <class G>
+__device__ __forceinline__ uint32_t block_bw(const emit_layout &l, const G &get, uint64_t lo, uint64_t ref,
+                                             uint32_t t)
+{
+    uint64_t o = 1;
+    for (uint64_t i = t; i < l.B; i += 64) o |= xform_g(l, get, lo + i, ref);
+    o = wave_or_u64(o);
+    return 64 - __clzll(o);
+}
+
+// Packs innermost block b (one wave, lane t) at image offset off: its
+// interleaved words are assembled in lds (B words) and written out, then
+// the block index entry, dvbp page meta and for_f reference page.  sync()
+// orders the wave's LDS accesses (the workgroup barrier when the
+// workgroup is this wave).
+template <class G, class Sync>
+__device__ __forceinline__ void pack_block(const emit_layout &l, const emit_plan &pl, uint64_t b, uint32_t bw,
+                                           uint64_t off, const G &get, uint64_t lo, uint64_t ref, const uint64_t *refs,
+                                           const uint64_t *start, unsigned long long *lds, uint8_t *out,
+                                           uint64_t *index, uint64_t *refs_out, uint32_t t, const Sync &sync)
+{
+    // Lane counts that divide the wave and blocks of whole waves of codes
+    // let each thread own a run of consecutive codes of one lane.
+    const uint32_t S = (uint32_t)l.S;
+    const bool runs = S != 0 && S <= 64 && 64 % S == 0 && l.B % 64 == 0;
+    const uint32_t j = runs ? t % S : 0, g = runs ? t / S : 0, C = (uint32_t)(l.B >> 6);
+    const uint64_t fb = b / l.bpo;
+    const uint64_t words = l.B * bw / 64;
+    for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
+    sync();
+    // OR every code in, unmasked (vbp_routine_gen.py:101, 111)
+    if (runs) {
+        // Thread (j, g) owns codes g*C .. g*C+C-1 of lane j: consecutive
+        // codes share words, so it ORs them in registers and issues one
+        // LDS atomic per word it touches instead of one per code.
+        const uint64_t base = lo + j;
+        uint32_t k = g * C, bit = k * bw, cw = bit >> 6;
+        uint64_t acc = 0;
+        // A delta step of one lane width makes each code's predecessor
+        // the thread's previous code: one extra load per run.
+        const bool own_pred = l.kind == MS_DELTA && l.dstep == S;
+        uint64_t prev = 0;
+        if (own_pred) {
+            const uint64_t g0 = base + (uint64_t)k * S;
+            prev = g0 >= S ? get(g0 - S) : 0ull;
+        }
+        for (uint32_t c0 = 0; c0 < C; c0 += 8) {
+            uint64_t x[8];
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                const uint64_t gi = base + (uint64_t)(k + q) * S;
+                x[q] = c0 + q < C ? (own_pred ? get(gi) : xform_g(l, get, gi, ref)) : 0ull;
+            }
+            if (own_pred) {
+#pragma unroll
+                for (uint32_t q = 0; q < 8; ++q) {
+                    const uint64_t r = x[q];
+                    x[q] = r - prev;
+                    prev = r;
+                }
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+                if (c0 + q >= C) break;
+                const uint32_t w = bit >> 6, sh = bit & 63;
+                if (w != cw) {
+                    atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+                    acc = 0;
+                    cw = w;
+                }
+                acc |= x[q] << sh;
+                if (sh + bw > 64) {
+                    atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+                    cw = w + 1;
+                    acc = x[q] >> (64 - sh);
+                }
+                bit += bw;
+            }
+            k += 8;
+        }
+        atomicOr(&lds[cw * S + j], (unsigned long long)acc);
+    } else {
+        for (uint64_t i = t; i < l.B; i += 64) {
+            const uint64_t c = xform_g(l, get, lo + i, ref);
+            const uint64_t j = i % l.S, k = i / l.S;
+            const uint64_t bit = k * bw;
+            const uint64_t w = bit >> 6;
+            const uint32_t sh = (uint32_t)(bit & 63);
+            atomicOr(&lds[w * l.S + j], (unsigned long long)(c << sh));
+            if (sh + bw > 64) atomicOr(&lds[(w + 1) * l.S + j], (unsigned long long)(c >> (64 - sh)));
+        }
+    }
+    sync();
+    uint64_t *dst = (uint64_t *)(out + off);
+    for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
+    sync();
+    if (t == 0) index[b] = (off << 8) | bw;
+    // dynamic_vbp page meta (dynamic_vbp.h:216-255)
+    if (l.leaf == MS_DYNAMIC_VBP) {
+        uint64_t q, call_end;
+        if (l.kind == MS_DYNAMIC_VBP) {
+            if (l.K) {
+                const uint64_t bpc = l.K / l.B;
+                q = b % bpc;
+                call_end = (b / bpc + 1) * bpc;
+                if (call_end > pl.nb) call_end = pl.nb;
+            } else {
+                q = b;
+                call_end = pl.nb;
+            }
+        } else {
+            q = b % l.bpo;
+            call_end = (b / l.bpo + 1) * l.bpo;
+        }
+        const uint64_t bpf = b - q % l.P;                                 // first block of the page
+        const uint64_t meta = out_pos(l, start, bpf) + hdr_of(l, bpf) - l.P; // meta bytes end the header
+        if (t == 0) out[meta + q % l.P] = (uint8_t)bw;
+        if (b == bpf) {
+            const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
+            for (uint64_t x = npage + t; x < l.P; x += 64) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
+        }
+    }
+    // for_f reference page (for.h:207-233)
+    if (l.kind == MS_FOR && b % l.bpo == 0) {
+        if (t == 0) refs_out[fb] = ref;
+        const uint64_t fpc = l.K ? l.K / l.L : 0;
+        const uint64_t fq = l.K ? fb % fpc : fb;
+        if (fq % l.FP == 0) {
+            uint64_t fcall_end = l.K ? (fb / fpc + 1) * fpc : pl.nfb;
+            if (fcall_end > pl.nfb) fcall_end = pl.nfb;
+            const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
+            uint64_t *rp = (uint64_t *)(out + out_pos(l, start, b));
+            for (uint64_t x = t; x < l.FP; x += 64) rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
+        }
     }
 }
 
@@ -176,10 +426,7 @@ __global__ __launch_bounds__(64) void k_block_sizes(emit_layout l, const uint64_
             const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
             uint64_t lo;
             const uint64_t *src = chunk_src(l, v, b * l.B, lo);
-            uint64_t o = 1;
-            for (uint64_t i = t; i < l.B; i += 64) o |= xform(l, src, lo + i, ref);
-            o = wave_or_u64(o);
-            bw = 64 - __clzll(o);
+            bw = block_bw(l, [&](uint64_t gi) { return src[gi]; }, lo, ref, t);
         }
         if (t == 0) {
             sz[b] = hdr_of(l, b) + l.B * bw / 8;
@@ -196,132 +443,74 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
     extern __shared__ unsigned long long lds[];
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
-    // Lane counts that divide the wave and blocks of whole waves of codes
-    // let each thread own a run of consecutive codes of one lane.
-    const uint32_t S = (uint32_t)l.S;
-    const bool runs = S != 0 && S <= 64 && 64 % S == 0 && l.B % 64 == 0;
-    const uint32_t j = runs ? t % S : 0, g = runs ? t / S : 0, C = (uint32_t)(l.B >> 6);
     for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
         const uint32_t bw = bws[b];
-        const uint64_t h = hdr_of(l, b);
-        const uint64_t off = out_pos(l, start, b) + h;
-        uint64_t lo;
-        const uint64_t *src = chunk_src(l, v, b * l.B, lo);
-        const uint64_t fb = b / l.bpo;
-        const uint64_t ref = l.kind == MS_FOR ? refs[fb] : 0;
-        const uint64_t words = l.B * bw / 64;
-        if (off + 8 * words > l.cap) {            // uniform per wave
+        const uint64_t off = out_pos(l, start, b) + hdr_of(l, b);
+        if (off + 8 * (l.B * bw / 64) > l.cap) {   // uniform per wave
             if (t == 0) emit_overflow(l);
             continue;
         }
-        for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
+        uint64_t lo;
+        const uint64_t *src = chunk_src(l, v, b * l.B, lo);
+        const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
+        pack_block(l, pl, b, bw, off, [&](uint64_t gi) { return src[gi]; }, lo, ref, refs, start, lds, out, index,
+                   refs_out, t, [] { __syncthreads(); });
+    }
+}
+
+// Sizes from a select's flags, one innermost block per wave (dynamic LDS:
+// B + pred gathered values).
+__global__ __launch_bounds__(64) void k_flag_sizes(emit_layout l, flag_src fs, const uint64_t *d_m,
+                                                   const uint64_t *refs, uint64_t nb_bound, uint64_t *sz,
+                                                   uint32_t *bws)
+{
+    extern __shared__ unsigned long long fvals[];
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint32_t t = threadIdx.x;
+    const uint64_t nunits = *fs.sf.d_units;
+    for (uint64_t b = blockIdx.x; b < nb_bound; b += gridDim.x) {
+        if (b >= pl.nb) {
+            if (t == 0) { sz[b] = 0; bws[b] = 0; }
+            continue;
+        }
+        const uint64_t r0 = flag_r0(l, fs, b), lo = b * l.B;
+        gather_ranks(fs.sf, nunits, fs.ustart[b], r0, lo + l.B - r0,
+                     [&](uint64_t i, uint64_t row) { fvals[i] = row; });
         __syncthreads();
-        // OR every code in, unmasked (vbp_routine_gen.py:101, 111)
-        if (runs) {
-            // Thread (j, g) owns codes g*C .. g*C+C-1 of lane j: consecutive
-            // codes share words, so it ORs them in registers and issues one
-            // LDS atomic per word it touches instead of one per code.
-            const uint64_t base = lo + j;
-            uint32_t k = g * C, bit = k * bw, cw = bit >> 6;
-            uint64_t acc = 0;
-            // A delta step of one lane width makes each code's predecessor
-            // the thread's previous code: one extra load per run.
-            const bool own_pred = l.kind == MS_DELTA && l.dstep == S;
-            uint64_t prev = 0;
-            if (own_pred) {
-                const uint64_t g0 = base + (uint64_t)k * S;
-                prev = g0 >= S ? src[g0 - S] : 0ull;
-            }
-            for (uint32_t c0 = 0; c0 < C; c0 += 8) {
-                uint64_t x[8];
-#pragma unroll
-                for (uint32_t q = 0; q < 8; ++q) {
-                    const uint64_t gi = base + (uint64_t)(k + q) * S;
-                    x[q] = c0 + q < C ? (own_pred ? src[gi] : xform(l, src, gi, ref)) : 0ull;
-                }
-                if (own_pred) {
-#pragma unroll
-                    for (uint32_t q = 0; q < 8; ++q) {
-                        const uint64_t r = x[q];
-                        x[q] = r - prev;
-                        prev = r;
-                    }
-                }
-#pragma unroll
-                for (uint32_t q = 0; q < 8; ++q) {
-                    if (c0 + q >= C) break;
-                    const uint32_t w = bit >> 6, sh = bit & 63;
-                    if (w != cw) {
-                        atomicOr(&lds[cw * S + j], (unsigned long long)acc);
-                        acc = 0;
-                        cw = w;
-                    }
-                    acc |= x[q] << sh;
-                    if (sh + bw > 64) {
-                        atomicOr(&lds[cw * S + j], (unsigned long long)acc);
-                        cw = w + 1;
-                        acc = x[q] >> (64 - sh);
-                    }
-                    bit += bw;
-                }
-                k += 8;
-            }
-            atomicOr(&lds[cw * S + j], (unsigned long long)acc);
-        } else {
-            for (uint64_t i = t; i < l.B; i += 64) {
-                const uint64_t c = xform(l, src, lo + i, ref);
-                const uint64_t j = i % l.S, k = i / l.S;
-                const uint64_t bit = k * bw;
-                const uint64_t w = bit >> 6;
-                const uint32_t sh = (uint32_t)(bit & 63);
-                atomicOr(&lds[w * l.S + j], (unsigned long long)(c << sh));
-                if (sh + bw > 64) atomicOr(&lds[(w + 1) * l.S + j], (unsigned long long)(c >> (64 - sh)));
-            }
+        const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
+        const uint32_t bw = block_bw(l, [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; }, lo, ref, t);
+        if (t == 0) {
+            sz[b] = hdr_of(l, b) + l.B * bw / 8;
+            bws[b] = bw;
         }
         __syncthreads();
-        uint64_t *dst = (uint64_t *)(out + off);
-        for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
-        __syncthreads();
-        if (t == 0) index[b] = (off << 8) | bw;
-        // dynamic_vbp page meta (dynamic_vbp.h:216-255)
-        if (l.leaf == MS_DYNAMIC_VBP) {
-            uint64_t q, call_end;
-            if (l.kind == MS_DYNAMIC_VBP) {
-                if (l.K) {
-                    const uint64_t bpc = l.K / l.B;
-                    q = b % bpc;
-                    call_end = (b / bpc + 1) * bpc;
-                    if (call_end > pl.nb) call_end = pl.nb;
-                } else {
-                    q = b;
-                    call_end = pl.nb;
-                }
-            } else {
-                q = b % l.bpo;
-                call_end = (b / l.bpo + 1) * l.bpo;
-            }
-            const uint64_t bpf = b - q % l.P;                                 // first block of the page
-            const uint64_t meta = out_pos(l, start, bpf) + hdr_of(l, bpf) - l.P; // meta bytes end the header
-            if (t == 0) out[meta + q % l.P] = (uint8_t)bw;
-            if (b == bpf) {
-                const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
-                for (uint64_t x = npage + t; x < l.P; x += 64) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
-            }
+    }
+}
+
+// Pack from a select's flags, one innermost block per wave (dynamic LDS:
+// B packed words, then the B + pred gathered values).
+__global__ __launch_bounds__(64) void k_flag_pack(emit_layout l, flag_src fs, const uint64_t *d_m,
+                                                  const uint64_t *refs, const uint64_t *start, const uint32_t *bws,
+                                                  uint8_t *out, uint64_t *index, uint64_t *refs_out)
+{
+    extern __shared__ unsigned long long lds[];
+    unsigned long long *fvals = lds + l.B;
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint32_t t = threadIdx.x;
+    const uint64_t nunits = *fs.sf.d_units;
+    for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
+        const uint32_t bw = bws[b];
+        const uint64_t off = out_pos(l, start, b) + hdr_of(l, b);
+        if (off + 8 * (l.B * bw / 64) > l.cap) {   // uniform per wave
+            if (t == 0) emit_overflow(l);
+            continue;
         }
-        // for_f reference page (for.h:207-233)
-        if (l.kind == MS_FOR && b % l.bpo == 0) {
-            if (t == 0) refs_out[fb] = ref;
-            const uint64_t fpc = l.K ? l.K / l.L : 0;
-            const uint64_t fq = l.K ? fb % fpc : fb;
-            if (fq % l.FP == 0) {
-                uint64_t fcall_end = l.K ? (fb / fpc + 1) * fpc : pl.nfb;
-                if (fcall_end > pl.nfb) fcall_end = pl.nfb;
-                const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
-                uint64_t *rp = (uint64_t *)(out + out_pos(l, start, b));
-                for (uint64_t x = t; x < l.FP; x += 64)
-                    rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
-            }
-        }
+        const uint64_t r0 = flag_r0(l, fs, b), lo = b * l.B;
+        gather_ranks(fs.sf, nunits, fs.ustart[b], r0, lo + l.B - r0,
+                     [&](uint64_t i, uint64_t row) { fvals[i] = row; });
+        const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
+        pack_block(l, pl, b, bw, off, [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; }, lo, ref, refs, start,
+                   lds, out, index, refs_out, t, [] { __syncthreads(); });
     }
 }
 
@@ -393,10 +582,9 @@ __global__ __launch_bounds__(WG) void k_emit_rest(emit_layout l, const uint64_t 
         dst[i] = v[pl.m_c + i];
 }
 
-int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
-                uint64_t *d_append, temps &sc)
+// The emit layout of an output column written as `chunk`-value calls.
+static emit_layout out_layout(ms_col *out, uint64_t chunk)
 {
-    hipStream_t s = sc.s;
     const ms_format_desc &f = out->f;
     const ms_geom &g = out->g;
     emit_layout l = {};
@@ -421,6 +609,15 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         l.L = l.B;
         l.bpo = 1;
     }
+    return l;
+}
+
+int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
+                uint64_t *d_append, temps &sc)
+{
+    hipStream_t s = sc.s;
+    const ms_format_desc &f = out->f;
+    const emit_layout l = out_layout(out, chunk);
     if (f.kind != MS_STATIC_VBP && l.B * 8 > 65536)
         return ms_fail(MS_ERR_INVALID, "emit: innermost blocks above 8192 values are not supported");
     const uint64_t nb_bound = m_bound / l.B + 1;
@@ -439,8 +636,8 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         if (f.kind == MS_FOR) {
             refs = (uint64_t *)sc.get((m_bound / l.L + 1) * 8 + 64);
             if (!refs) return ms_fail(MS_ERR_NOMEM, "emit scratch");
-            hipLaunchKernelGGL(k_for_refs, dim3(grid_for(m_bound / l.L + 1, 1, 65536)), dim3(64), 0, s, l, d_v, d_m,
-                               refs);
+            hipLaunchKernelGGL(k_for_refs<false>, dim3(grid_for(m_bound / l.L + 1, 1, 65536)), dim3(64), 0, s, l, d_v,
+                               flag_src{}, d_m, refs);
             MS_HIP(hipGetLastError());
         }
         hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(nb_bound, 1, 65536)), dim3(64), 0, s, l, d_v, d_m,
@@ -465,6 +662,106 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
                        (const ms_dev_meta *)out->meta, out->data);
+    MS_HIP(hipGetLastError());
+    return MS_OK;
+}
+
+// ------------------------------------------------------------------
+// Emission straight from a select's flags.
+// ------------------------------------------------------------------
+// Writes into pos (indexed by rank) the positions other kernels read by
+// rank: the delta_f tile predecessors [2048 t - pred, 2048 t) of every tile
+// t >= 1 of the compressed part, and the tail [values in the compressed
+// part, *d_total).  One wave per job.
+__global__ __launch_bounds__(64) void k_flag_ranks(emit_layout l, flag_src fs, const uint64_t *d_m,
+                                                   const uint64_t *d_total, uint64_t *pos)
+{
+    const emit_plan pl = plan_of(l, *d_m);
+    const uint64_t total = *d_total, nunits = *fs.sf.d_units;
+    const uint64_t ntiles = fs.pred ? (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE : 0;
+    const uint64_t jobs = ntiles;   // tiles 1 .. ntiles-1, then the tail as job 0
+    for (uint64_t jb = blockIdx.x; jb < (jobs ? jobs : 1); jb += gridDim.x) {
+        uint64_t r0, cnt, u;
+        if (jb == 0) {
+            r0 = pl.m_c;
+            cnt = total > r0 ? total - r0 : 0;
+            u = cnt ? fs.ustart[pl.nb] : 0;
+        } else {
+            r0 = jb * MS_DELTA_TILE - fs.pred;
+            cnt = fs.pred;
+            u = fs.ustart[jb * MS_DELTA_TILE / l.B];
+        }
+        uint64_t *dst = pos + r0;
+        gather_ranks(fs.sf, nunits, u, r0, cnt, [&](uint64_t i, uint64_t row) { dst[i] = row; });
+    }
+}
+
+bool emit_flags_ok(const ms_format_desc &f)
+{
+    if (f.kind != MS_DYNAMIC_VBP && f.kind != MS_DELTA && f.kind != MS_FOR) return false;
+    const ms_geom g = ms_make_geom(f);
+    if (g.leaf != MS_DYNAMIC_VBP || g.inner_bs == 0 || g.inner_bs > 2048 || 2048 % g.inner_bs) return false;
+    if (f.kind == MS_DELTA && (g.dstep == 0 || g.dstep > 64 || g.dstep >= g.inner_bs)) return false;
+    if (f.kind == MS_FOR && g.outer_block > 4096) return false;
+    return true;
+}
+
+int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uint64_t *d_m, const uint64_t *d_total,
+                      uint64_t m_bound, uint64_t chunk, temps &sc)
+{
+    hipStream_t s = sc.s;
+    const ms_format_desc &f = out->f;
+    if (!emit_flags_ok(f)) return ms_fail(MS_ERR_INVALID, "emit from flags: format kind %u", f.kind);
+    const emit_layout l = out_layout(out, chunk);
+    const uint64_t nb_bound = m_bound / l.B + 1;
+    flag_src fs = {};
+    fs.sf = sf;
+    fs.pred = f.kind == MS_DELTA ? l.dstep : 0;
+    // blocks whose first read rank exists: b B - pred < m_bound
+    const uint64_t starts = (m_bound + fs.pred) / l.B + 2;
+    uint32_t *ustart = (uint32_t *)sc.get(starts * 4 + 64);
+    uint64_t *sz = (uint64_t *)sc.get(nb_bound * 8 + 64);
+    uint64_t *start = (uint64_t *)sc.get(nb_bound * 8 + 64);
+    uint32_t *bws = (uint32_t *)sc.get(nb_bound * 4 + 64);
+    if (!ustart || !sz || !start || !bws) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+    if (sf.units_bound >= (1ull << 32)) return ms_fail(MS_ERR_CAPACITY, "emit from flags: %llu units",
+                                                      (unsigned long long)sf.units_bound);
+    fs.ustart = ustart;
+    hipLaunchKernelGGL(k_flag_starts, dim3(grid_for(sf.units_bound, WG)), dim3(WG), 0, s, l, fs, starts, ustart);
+    MS_HIP(hipGetLastError());
+    uint64_t *refs = nullptr;
+    if (f.kind == MS_FOR) {
+        refs = (uint64_t *)sc.get((m_bound / l.L + 1) * 8 + 64);
+        if (!refs) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+        hipLaunchKernelGGL(k_for_refs<true>, dim3(grid_for(m_bound / l.L + 1, 1, 65536)), dim3(64), l.L * 8, s, l,
+                           (const uint64_t *)nullptr, fs, d_m, refs);
+        MS_HIP(hipGetLastError());
+    }
+    const uint32_t vals_lds = (uint32_t)((l.B + fs.pred) * 8);
+    const uint32_t fl_grid = grid_for(nb_bound, 1, 65536);
+    hipLaunchKernelGGL(k_flag_sizes, dim3(fl_grid), dim3(64), vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
+                       nb_bound, sz, bws);
+    MS_HIP(hipGetLastError());
+    MS_TRY(launch_exclusive_scan(sz, start, nb_bound, sc));
+    if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
+        return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
+    hipLaunchKernelGGL(k_flag_pack, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
+                       (const uint64_t *)start, (const uint32_t *)bws, out->data, out->index, out->refs);
+    MS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)start, (const uint32_t *)bws,
+                       out->meta, (uint64_t *)nullptr);
+    hipLaunchKernelGGL(k_flag_ranks, dim3(grid_for(m_bound / MS_DELTA_TILE + 1, 1, 65536)), dim3(64), 0, s, l, fs,
+                       d_m, d_total, pos);
+    MS_HIP(hipGetLastError());
+    if (f.kind == MS_DELTA) {
+        if (out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
+            return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
+        hipLaunchKernelGGL(k_delta_carries_values, dim3(grid_for(m_bound / MS_DELTA_TILE * 8 + 8, WG)), dim3(WG), 0,
+                           s, l, (const uint64_t *)pos, d_m, out->refs);
+        MS_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, (const uint64_t *)pos,
+                       d_m, (const ms_dev_meta *)out->meta, out->data);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
@@ -528,7 +825,8 @@ int launch_chunk_sizes(chunk_emit &ce, uint64_t *d_csz, temps &sc)
     if (ce.f.kind == MS_FOR) {
         ce.refs = (uint64_t *)sc.get((m / l.L + 1) * 8 + 64);
         if (!ce.refs) return ms_fail(MS_ERR_NOMEM, "chunked emission scratch");
-        hipLaunchKernelGGL(k_for_refs, dim3(grid_for(m / l.L, 1, 65536)), dim3(64), 0, s, l, ce.v, ce.d_m, ce.refs);
+        hipLaunchKernelGGL(k_for_refs<false>, dim3(grid_for(m / l.L, 1, 65536)), dim3(64), 0, s, l, ce.v, flag_src{},
+                           ce.d_m, ce.refs);
         MS_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(ce.nb + 1, 1, 65536)), dim3(64), 0, s, l, ce.v, ce.d_m,

@@ -58,6 +58,20 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
 // ascending positions with pred into d_pos (capacity >= count bound);
 // *d_cnt receives their number
 int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, temps &sc);
+// The selected rows of a select as its count pass leaves them, without
+// materialising positions: 8 ballots per 512-row unit (bit 8 gl + j of
+// ballot m = row 64 gl + 8 m + j of the unit) and uoff[u] = the number of
+// selected rows before unit u (uoff[*d_units] = all of them).  Ranks are
+// positions in the select's ascending output.
+struct sel_flags {
+    const uint64_t *flags;
+    const uint64_t *uoff;
+    const uint64_t *d_units;
+    uint64_t units_bound;    // host bound of *d_units
+};
+// true when launch_select_flags takes `in` (8-lane dynamic_vbp / for_f)
+bool select_flags_ok(const ms_col *in);
+int launch_select_flags(const ms_col *in, const ms_pred &p, sel_flags &sf, temps &sc);
 // rebuilds a delta_f column's tile carries from its image and index
 int launch_delta_carries(const ms_col *in, temps &sc);
 // exclusive prefix sums of n u64 values (in-place allowed)
@@ -74,6 +88,14 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
 // append offset (done()'s second element).
 int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
                 uint64_t *d_append, temps &sc);
+// launch_emit of the ranks [0, *d_m) of a select's flags (the positions
+// of its selected rows), for the formats emit_flags_ok takes (dynamic_vbp_f
+// and delta_f / for_f over it).  pos (capacity >= m_bound) receives only
+// the positions other kernels read by rank: the delta tile predecessors
+// and ranks [values in the compressed part, *d_total).
+bool emit_flags_ok(const ms_format_desc &f);
+int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uint64_t *d_m, const uint64_t *d_total,
+                      uint64_t m_bound, uint64_t chunk, temps &sc);
 // rle_f compression of d_v[0 .. *d_m)
 int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc);
 

@@ -123,6 +123,27 @@ def test_select_byte_identical(oracle, lanes, out_f, in_f):
         assert np.array_equal(g.values(), rm.select_positions(v, cmp, val))
 
 
+# select from dynamic_vbp / for_f inputs into the block formats packs its
+# output straight from the count pass's ballots: densities from a handful
+# of rows to all of them (units walked past empty, lanes full), delta steps
+# equal to and below the lane count, and inner blocks of 512 to 2048
+@pytest.mark.parametrize("out_f", [default_delta_dynamic_vbp_f(8), delta_f(1024, 4, dynamic_vbp_f(512, 64, 4)),
+                                   dynamic_vbp_f(2048, 64, 8), delta_f(2048, 8, dynamic_vbp_f(2048, 64, 8)),
+                                   for_f(1024, 8, dynamic_vbp_f(512, 64, 8))], ids=repr)
+@pytest.mark.parametrize("in_f", [dynamic_vbp_f(512, 2048, 8), default_for_dynamic_vbp_f(8)], ids=repr)
+def test_select_positions_from_ballots(oracle, out_f, in_f):
+    n = 3 * 65536 + 777
+    v = datagen.uniform_bits(21, n, 20)
+    col = ea.morph(in_f, ea.upload_values(v))
+    ocol = oracle.morph(in_f, v)
+    for thr in (0, 40, 3000, 300000, 950000, 1 << 20):
+        for lanes in (1, 8):
+            g = ea.select(MS_CMP_LT, thr, out_f, col, lanes)
+            o = oracle.select(MS_CMP_LT, thr, out_f, ocol, lanes)
+            assert_same_column(g, o)
+            assert np.array_equal(g.values(), np.flatnonzero(v < thr).astype(np.uint64)), (thr, lanes)
+
+
 @pytest.mark.parametrize("n", [1, 5, 13, 100])
 @pytest.mark.parametrize("out_f", OUT_FORMATS, ids=OUT_IDS)
 def test_select_tiny_inputs(oracle, n, out_f):
