@@ -21,7 +21,7 @@
 
 namespace ms {
 
-enum { BI_STATIC = 0, BI_BLOCKS = 1, BI_FOR = 2, BI_DELTA = 3, BI_RLE = 4, BI_RAW = 5, BI_GENERIC = 6 };
+enum { BI_STATIC = 0, BI_BLOCKS = 1, BI_FOR = 2, BI_DELTA = 3, BI_RLE = 4, BI_RAW = 5, BI_GENERIC = 6, BI_DELTA_GEN = 7 };
 
 struct ms_bentry {
     ms_colv c;
@@ -115,6 +115,43 @@ __device__ __attribute__((noinline, pure)) uint64_t item_generic(const ms_bentry
     return acc;
 }
 
+// delta_f layouts without a unit decoder (a step of 1, 2 or 4, or inner
+// layouts other than 8-lane dynamic_vbp): n tiles of 2048 values from
+// tile0 on, the differences read one by one (read_compr, as the
+// column-level generic decode does).  Lane t takes values base + 64 k + t,
+// k = 0 .. 31: all of one residue class r = t mod step, so each row's
+// running sums are a scan over the lanes of the class (shuffles by
+// multiples of the step) on top of the class's value before the row (the
+// tile carry, then each row's last).
+__device__ __attribute__((noinline, pure)) uint64_t item_delta_generic(const ms_bentry *en, uint64_t tile0, uint32_t n,
+                                                                       uint32_t t)
+{
+    const ms_colv c = en->c;
+    const ms_pred p = en->p;
+    const uint64_t n_compr = parts_of(c).n_compr;
+    const uint32_t ds = c.g.dstep, r = t & (ds - 1);
+    uint64_t acc = 0;
+    for (uint32_t q = 0; q < n; q++) {
+        const uint64_t base = (tile0 + q) * MS_DELTA_TILE;
+        if (base >= n_compr) break;
+        uint64_t run = gload(c.refs + (tile0 + q) * 8 + r);
+        for (uint32_t k = 0; k < MS_DELTA_TILE / 64; k++) {
+            const uint64_t i = base + 64ull * k + t;
+            if (base + 64ull * k >= n_compr) break;
+            const bool in = i < n_compr;
+            uint64_t d = in ? read_compr(c, i) : 0ull;
+            for (uint32_t o = ds; o < 64; o <<= 1) {
+                const uint64_t y = __shfl_up(d, o, MS_WAVE);
+                if (t >= o) d += y;
+            }
+            const uint64_t x = run + d;
+            acc += in && pred_eval(p, x) ? x : 0ull;
+            run += __shfl(d, (int)(64 - ds + r), MS_WAVE);
+        }
+    }
+    return acc;
+}
+
 // The loop body, instantiated by kernels that differ only in their
 // register budget (waves per SIMD), a choice of the autotuner.  TIMED adds
 // each item's wall-clock ticks (the constant-rate counter) to its view's
@@ -179,6 +216,7 @@ __device__ __forceinline__ void batch_loop(const ms_bentry *__restrict__ E, cons
             }
             break;
         }
+        case BI_DELTA_GEN: acc += item_delta_generic(&E[it.entry], it.first, n, t); break;
         default:   // BI_GENERIC
             acc += item_generic(&E[it.entry], it.first, n, t);
             break;
@@ -349,8 +387,6 @@ extern "C" int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uin
     if (cmp > MS_CMP_GE) return ms_fail(MS_ERR_INVALID, "comparison %u", cmp);
     if (in->g.kind == MS_STATIC_VBP_PAD || in->g.kind == MS_K_WISE_NS || in->g.kind == MS_GROUP_SIMPLE)
         return ms_fail(MS_ERR_INVALID, "no batched decoder for format kind %u: decompress it first", in->g.kind);
-    if (in->g.kind == MS_DELTA && !delta_batchable(in->g))
-        return ms_fail(MS_ERR_INVALID, "batched delta_f needs step 8 over an 8-lane dynamic_vbp inner format");
     if (in->g.kind == MS_BLOCKWISE_INDIVIDUAL && (!in->bwi || in->bwi->lcount[BWK_GS]))
         return ms_fail(MS_ERR_INVALID, "no batched decoder for group_simple_f blocks: decompress the column first");
     ms_bentry e = {};
@@ -474,7 +510,9 @@ static void make_queue(ms_batch *b)
             const uint64_t units = pt.n_compr / 512;
             const bool blocks = g.leaf == MS_DYNAMIC_VBP && g.lanes == 8 && g.lg_spb != 0xff &&
                                 (g.kind == MS_DYNAMIC_VBP || g.lg_ipo != 0xff);
-            if (g.kind == MS_DELTA) add(e, BI_DELTA, 0, (units + 3) / 4, cf.tiles_per_item);
+            if (g.kind == MS_DELTA && delta_batchable(g)) add(e, BI_DELTA, 0, (units + 3) / 4, cf.tiles_per_item);
+            else if (g.kind == MS_DELTA)
+                add(e, BI_DELTA_GEN, 0, (pt.n_compr + MS_DELTA_TILE - 1) / MS_DELTA_TILE, cf.tiles_per_item);
             else if (blocks) add(e, g.kind == MS_FOR ? BI_FOR : BI_BLOCKS, 0, units,
                                  g.kind == MS_FOR ? upi_for : cf.units_per_item);
             else add(e, BI_GENERIC, 0, pt.n_compr, cf.values_per_item);

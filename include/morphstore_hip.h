@@ -423,9 +423,13 @@ typedef struct ms_batch_config {
 
 int ms_batch_create(ms_batch **out);
 
-/* Adds SELECT SUM(x) WHERE cmp(x, val) over `in` (any format; delta_f needs
- * step 8 over an 8-lane dynamic_vbp inner format) as entry *entry, the
- * index of its result.  `in` must stay alive while the batch is used. */
+/* Adds SELECT SUM(x) WHERE cmp(x, val) over `in` as entry *entry, the index
+ * of its result.  Every format is taken except static_vbp_f over
+ * vbp_padding_l, k_wise_ns_f, group_simple_f and blockwise_individual_f
+ * columns holding group_simple_f blocks (MS_ERR_INVALID: decompress them
+ * first); delta_f layouts other than step 8 over 8-lane dynamic_vbp are
+ * read difference by difference.  `in` must stay alive while the batch is
+ * used. */
 int ms_batch_enqueue(ms_batch *b, const ms_col *in, uint32_t cmp, uint64_t val, uint64_t *entry);
 
 /* d_results[e] = the sum of entry e, for every entry: one persistent kernel

@@ -8,7 +8,7 @@ import engine_amd as ea
 from engine_amd import datagen
 from engine_amd.formats import (MS_CMP_EQ, MS_CMP_GE, MS_CMP_GT, MS_CMP_LE, MS_CMP_LT, MS_CMP_NE, BatchConfig,
                                 default_delta_dynamic_vbp_f, default_dynamic_vbp_f, default_for_dynamic_vbp_f,
-                                dynamic_vbp_f, for_f, rle_f, static_vbp_f, uncompr_f)
+                                delta_f, dynamic_vbp_f, for_f, group_simple_f, rle_f, static_vbp_f, uncompr_f)
 from tests import refmodel as rm
 
 pytestmark = pytest.mark.gpu
@@ -46,6 +46,12 @@ def edge_columns():
     out.append((default_dynamic_vbp_f(1), datagen.uniform_bits(seed + 12, 3000, 17)))
     out.append((for_f(1024, 8, uncompr_f()), datagen.for_base_offset(seed + 13, 5000)))
     out.append((dynamic_vbp_f(1024, 64, 8), datagen.uniform_bits(seed + 14, 7000, 13)))
+    # delta_f layouts read difference by difference: steps 1, 2, 4, 8 over
+    # other inner layouts, partial tiles and rests
+    for k, n in enumerate((1, 2047, 2048, 2049, 9000)):
+        for f in (default_delta_dynamic_vbp_f(4), default_delta_dynamic_vbp_f(2), default_delta_dynamic_vbp_f(1),
+                  delta_f(1024, 8, static_vbp_f(24, 8)), delta_f(1024, 4, uncompr_f())):
+            out.append((f, datagen.sorted_geometric(seed + 20 + k, n)))
     return out
 
 
@@ -94,7 +100,7 @@ def test_batch_configs(cfg):
     check_results(b, expect)
 
 
-def test_batch_rejects_bad_config_and_delta_layout():
+def test_batch_rejects_bad_config_and_layouts_without_decoder():
     b = ea.Batch()
     with pytest.raises(ea.MsError):
         b.config = BatchConfig(0, 1, 1, 64, 64, 0)
@@ -104,7 +110,7 @@ def test_batch_rejects_bad_config_and_delta_layout():
         b.config = BatchConfig(1, 65, 1, 64, 64, 0)
     with pytest.raises(ea.MsError):
         b.config = BatchConfig(1, 1, 1, 64, 64, 7)
-    col = ea.morph(default_delta_dynamic_vbp_f(4), ea.upload_values(datagen.sorted_geometric(1, 5000)))
+    col = ea.morph(group_simple_f(8), ea.upload_values(datagen.uniform_bits(1, 5000, 12)))
     with pytest.raises(ea.MsError):
         b.enqueue(col, MS_CMP_LT, 100)
 
