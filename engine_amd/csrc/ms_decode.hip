@@ -93,9 +93,27 @@ __global__ __launch_bounds__(WG) void k_select_sum_static(ms_colv c, ms_pred p, 
     const uint64_t nwaves = (uint64_t)gridDim.x * WAVES;
     uint64_t acc = 0;
     if (cp.mode != PM_NONE) {
-        for (uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index(); u < nunits; u += nwaves) {
+        const uint64_t *base = (const uint64_t *)c.data;
+        uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
+        // four units a step, every unit's words requested before the first is cut
+        constexpr int NW = vbp_traits<BW>::NW;
+        for (; u + 3 * nwaves < nunits; u += 4 * nwaves) {
+            uint64_t w0[NW], w1[NW], w2[NW], w3[NW];   // named arrays: they stay in registers
+            uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+            const uint64_t g0 = u * gpu + (t >> lgS), g1 = g0 + nwaves * gpu, g2 = g1 + nwaves * gpu,
+                           g3 = g2 + nwaves * gpu;
+            if (g0 < ngroups) vbp_load<BW>(base, S, j, g0, w0, s0);
+            if (g1 < ngroups) vbp_load<BW>(base, S, j, g1, w1, s1);
+            if (g2 < ngroups) vbp_load<BW>(base, S, j, g2, w2, s2);
+            if (g3 < ngroups) vbp_load<BW>(base, S, j, g3, w3, s3);
+            if (g0 < ngroups) ss_loaded<BW>(w0, s0, cp, 0, acc);
+            if (g1 < ngroups) ss_loaded<BW>(w1, s1, cp, 0, acc);
+            if (g2 < ngroups) ss_loaded<BW>(w2, s2, cp, 0, acc);
+            if (g3 < ngroups) ss_loaded<BW>(w3, s3, cp, 0, acc);
+        }
+        for (; u < nunits; u += nwaves) {
             const uint64_t g = u * gpu + (t >> lgS);
-            if (g < ngroups) ss_unit<BW>((const uint64_t *)c.data, S, j, g, cp, 0, acc);
+            if (g < ngroups) ss_unit<BW>(base, S, j, g, cp, 0, acc);
         }
     }
     wg_partial(acc, res);

@@ -104,13 +104,11 @@ __device__ __forceinline__ void ss_words(const uint64_t *w, uint32_t sh, const m
     acc += (uint64_t)s + (uint64_t)cnt * ref;
 }
 
+// Select + sum of the 8 codes in a thread's loaded words (any BW).
 template <int BW>
-__device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
-                                        uint64_t ref, uint64_t &acc)
+__device__ __forceinline__ void ss_loaded(const uint64_t (&w)[vbp_traits<BW>::NW], uint32_t sh, const ms_cpred &cp,
+                                          uint64_t ref, uint64_t &acc)
 {
-    uint64_t w[vbp_traits<BW>::NW];
-    uint32_t sh;
-    vbp_load<BW>(base, S, j, g, w, sh);
     if constexpr (BW <= 32) {
         ss_words<BW>(w, sh, cp, ref, acc);
     } else {
@@ -132,6 +130,16 @@ __device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32
         }
         acc += s + cnt * ref;
     }
+}
+
+template <int BW>
+__device__ __forceinline__ void ss_unit(const uint64_t *base, uint32_t S, uint32_t j, uint64_t g, const ms_cpred &cp,
+                                        uint64_t ref, uint64_t &acc)
+{
+    uint64_t w[vbp_traits<BW>::NW];
+    uint32_t sh;
+    vbp_load<BW>(base, S, j, g, w, sh);
+    ss_loaded<BW>(w, sh, cp, ref, acc);
 }
 
 #define MS_BW_CASES(X)                                                                                              \
