@@ -286,8 +286,10 @@ static std::mutex g_cols_mu;
 static std::unordered_set<const ms_col *> g_cols;
 
 // One allocation: [meta 256 B][index][refs][view table][image]
+// A new column; its meta is cleared on the device unless the caller's
+// first kernel writes all of it (clear_meta false).
 static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_alloc, hipStream_t s, ms_col **out,
-                   const col_caps *caps = nullptr)
+                   const col_caps *caps = nullptr, bool clear_meta = true)
 {
     ms_col *c = new (std::nothrow) ms_col();
     if (!c) return ms_fail(MS_ERR_NOMEM, "host allocation");
@@ -316,7 +318,7 @@ static int new_col(const ms_format_desc &f, uint64_t count_bound, uint64_t size_
     c->refs = c->refs_cap ? (uint64_t *)(base + 256 + idx_bytes) : nullptr;
     c->vmeta = tab_bytes ? (ms_dev_meta *)(base + 256 + idx_bytes + refs_bytes) : nullptr;
     c->data = base + 256 + idx_bytes + refs_bytes + tab_bytes;
-    int rc = launch_fill_u64((uint64_t *)base, 256 / 8, 0, s);
+    int rc = clear_meta ? launch_fill_u64((uint64_t *)base, 256 / 8, 0, s) : MS_OK;
     if (rc) {
         dev_free(base, s);
         delete c;
@@ -1143,9 +1145,8 @@ extern "C" int ms_morph(ms_col **out, const ms_format_desc *dst, const ms_col *i
     ms_col *c;
     if (dst->kind == MS_UNCOMPR) {
         // decompression straight into the new column's buffer
-        MS_TRY(new_col(*dst, nb, 8 * nb, S(stream), &c));
-        int rc = launch_decompress(in, (uint64_t *)c->data, S(stream));
-        if (!rc) rc = launch_set_meta(c->meta, &in->meta->count, 8, 0, S(stream));
+        MS_TRY(new_col(*dst, nb, 8 * nb, S(stream), &c, nullptr, false));
+        int rc = launch_decompress(in, (uint64_t *)c->data, S(stream), c->meta);   // writes the whole meta
         if (rc) { ms_col_free(c); return rc; }
         *out = c;
         return MS_OK;

@@ -348,8 +348,35 @@ __global__ __launch_bounds__(WG) void k_delta_carries(const uint64_t *scan, uint
 // requested before the first is decoded.
 enum { DL_BLOCKS = 0, DL_FOR = 1, DL_STATIC8 = 2 };
 
+// The end of a decompression kernel: the raw rest behind the compressed
+// part (grid-stride), and, when om is given, the output column's whole
+// meta (count, 8 bytes a value, status and append slots cleared).
+__device__ __forceinline__ void decomp_tail(const ms_colv &c, const ms_parts &pt, uint64_t *out, ms_dev_meta *om)
+{
+    const uint64_t *rx = (const uint64_t *)(c.data + pt.rest_off);
+    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < pt.n_rest; i += (uint64_t)gridDim.x * WG)
+        out[pt.n_compr + i] = gload(rx + i);
+    if (om && blockIdx.x == 0 && threadIdx.x == 0) {
+        om->count = pt.n;
+        om->size_used = 8 * pt.n;
+        om->size_compr = 0;
+        om->status = 0;
+        for (int k = 0; k < 4; k++) om->aux[k] = 0;
+    }
+}
+
+__global__ void k_set_meta_uncompr(const ms_dev_meta *in, ms_dev_meta *om)
+{
+    const uint64_t n = in->count;
+    om->count = n;
+    om->size_used = 8 * n;
+    om->size_compr = 0;
+    om->status = 0;
+    for (int k = 0; k < 4; k++) om->aux[k] = 0;
+}
+
 template <int LAYOUT, bool NT>
-__global__ __launch_bounds__(WG) void k_decomp_units(ms_colv c, uint64_t *out)
+__global__ __launch_bounds__(WG) void k_decomp_units(ms_colv c, uint64_t *out, ms_dev_meta *om)
 {
     const ms_parts pt = parts_of(c);
     const uint32_t lg_spb = LAYOUT == DL_STATIC8 ? 0 : c.g.lg_spb;
@@ -389,13 +416,14 @@ __global__ __launch_bounds__(WG) void k_decomp_units(ms_colv c, uint64_t *out)
         if (q + W < nq && t < 4 && un < n) entry(un, my_e, my_r);
         dec_units<NT>(c.data, e, r, cnt, u0, mask, j, gl, out);
     }
+    decomp_tail(c, pt, out, om);
 }
 
 // delta_f with 8 lanes over indexed 8-lane inner blocks: one wave per
 // 2048-value tile (lane-major threads, 8 per delta lane), the tile's index
 // entries requested one tile ahead.
 template <bool NT>
-__global__ __launch_bounds__(WG) void k_delta_decomp(ms_colv c, uint64_t *out)
+__global__ __launch_bounds__(WG) void k_delta_decomp(ms_colv c, uint64_t *out, ms_dev_meta *om)
 {
     const ms_parts pt = parts_of(c);
     const uint64_t nunits = pt.n_compr >> 9;
@@ -411,6 +439,7 @@ __global__ __launch_bounds__(WG) void k_delta_decomp(ms_colv c, uint64_t *out)
         delta_tile_store<NT>(c.data, my_e, c.refs, c.g.lg_spb, tile, nunits, t, j, gl, out);
         my_e = e_next;
     }
+    decomp_tail(c, pt, out, om);
 }
 
 template <int BW>
@@ -816,12 +845,15 @@ int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bo
 #endif
 static constexpr bool DECOMP_NT = MS_DECOMP_NT != 0;
 
-int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
+int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s, ms_dev_meta *om)
 {
     ms_colv v;
     MS_TRY(colv_of(in, v));
     const uint64_t nb = in->count_bound;
-    if (nb == 0) return MS_OK;
+    if (nb == 0) {
+        if (om) hipLaunchKernelGGL(k_set_meta_uncompr, dim3(1), dim3(1), 0, s, (const ms_dev_meta *)in->meta, om);
+        return om ? (hipGetLastError() == hipSuccess ? MS_OK : ms_fail(MS_ERR_HIP, "decompress meta")) : MS_OK;
+    }
     const uint32_t grid_units = grid_for(nb / 512 + 1, WAVES * 4ull, 8192);
     const uint64_t quads = nb / 2048 + 1;   // groups of 4 units
     switch (v.g.kind) {
@@ -838,13 +870,16 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         MS_TRY(launch_exclusive_scan(clen, cst, nch_bound, sc));
         auto k = k_rle_expand_chunks<false>;
         hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nch_bound)), dim3(WG), 0, s, v, cst, d_out);
+        if (om) hipLaunchKernelGGL(k_set_meta_uncompr, dim3(1), dim3(1), 0, s, (const ms_dev_meta *)in->meta, om);
         MS_HIP(hipGetLastError());
         return MS_OK;
     }
     case MS_STATIC_VBP:
         if (fast_static(v) && v.g.lanes == 8) {
             auto k = k_decomp_units<DL_STATIC8, DECOMP_NT>;
-            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out);
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out, om);
+            MS_HIP(hipGetLastError());
+            return MS_OK;   // rest and meta written by the kernel
         } else if (fast_static(v)) {
             hipLaunchKernelGGL(static_dec_kernel(v.g.bw), dim3(grid_units), dim3(WG), 0, s, v, d_out);
         } else {
@@ -855,7 +890,9 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
     case MS_FOR:
         if (fast_blocks(v)) {
             auto k = v.g.kind == MS_FOR ? k_decomp_units<DL_FOR, DECOMP_NT> : k_decomp_units<DL_BLOCKS, DECOMP_NT>;
-            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out);
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, quads)), dim3(WG), 0, s, v, d_out, om);
+            MS_HIP(hipGetLastError());
+            return MS_OK;   // rest and meta written by the kernel
         } else {
             hipLaunchKernelGGL(k_decomp_generic, dim3(grid_for(nb, WG * 8ull)), dim3(WG), 0, s, v, d_out);
         }
@@ -866,8 +903,10 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         MS_TRY(delta_src(in, v, sc, &arr));
         if (!arr && v.g.lg_spb != 0xff) {
             auto k = k_delta_decomp<DECOMP_NT>;
-            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nb / DELTA_TILE + 1)), dim3(WG), 0, s, v, d_out);
-            break;
+            hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nb / DELTA_TILE + 1)), dim3(WG), 0, s, v, d_out,
+                               om);
+            MS_HIP(hipGetLastError());
+            return MS_OK;   // rest and meta written by the kernel
         }
         const dim3 grid(grid_for(nb / DELTA_TILE + 1, 1, 65536));
         if (arr) hipLaunchKernelGGL((k_delta_tiles<0, 1>), grid, dim3(WG), 0, s, v, arr, d_out, ms_pred{}, nullptr);
@@ -880,12 +919,16 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s)
         MS_TRY(launch_xfmt_decompress(in, d_out, s));
         break;
     case MS_BLOCKWISE_INDIVIDUAL:
-        return launch_bwi_decompress(in, d_out, s);   // copies its rest itself
+        MS_TRY(launch_bwi_decompress(in, d_out, s));   // copies its rest itself
+        if (om) hipLaunchKernelGGL(k_set_meta_uncompr, dim3(1), dim3(1), 0, s, (const ms_dev_meta *)in->meta, om);
+        MS_HIP(hipGetLastError());
+        return MS_OK;
     default:
         return ms_fail(MS_ERR_INVALID, "decompress: unsupported format");
     }
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_copy_rest, dim3(grid_for(nb, WG * 4ull, 4096)), dim3(WG), 0, s, v, d_out);
+    if (om) hipLaunchKernelGGL(k_set_meta_uncompr, dim3(1), dim3(1), 0, s, (const ms_dev_meta *)in->meta, om);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

@@ -53,8 +53,9 @@ struct ss_buffers {
 // unless `accumulate`.  With `bufs` no device memory is allocated.
 int launch_select_sum(const ms_col *in, const ms_pred &p, uint64_t *d_result, bool accumulate, hipStream_t s,
                       const ss_buffers *bufs = nullptr);
-// writes the column's `count` values to d_out (uncompressed)
-int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s);
+// writes the column's `count` values to d_out (uncompressed); with `om`
+// also the whole meta of the uncompr_f column d_out belongs to
+int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s, ms_dev_meta *om = nullptr);
 // ascending positions with pred into d_pos (capacity >= count bound);
 // *d_cnt receives their number
 int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos, uint64_t *d_cnt, temps &sc);
