@@ -1143,7 +1143,7 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
 // ------------------------------------------------------------------
 // Exclusive scan of the first min(*d_n + extra, bound) values, sized from
 // the device count: G workgroups each take one contiguous chunk (sum,
-// scan of the G sums, chunk scan with its offset).  Arrays allocated for
+// then the chunk scan on the sum of the sums before it).  Arrays allocated for
 // a host bound are scanned only as far as they are used; without a device
 // count (d_n null) all `bound` values are scanned.  Counts are 64-bit
 // throughout.
@@ -1208,25 +1208,15 @@ __global__ __launch_bounds__(WG) void k_scan_part(const uint64_t *in, const uint
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-__global__ __launch_bounds__(WG) void k_scan_top(uint64_t *part, uint32_t G)
-{
-    uint64_t carry = 0;
-    for (uint32_t base = 0; base < G; base += WG) {
-        const uint32_t i = base + threadIdx.x;
-        const uint64_t v = i < G ? part[i] : 0;
-        uint64_t tot;
-        const uint64_t ex = block_exscan(v, &tot);
-        if (i < G) part[i] = carry + ex;
-        carry += tot;
-    }
-}
-
 __global__ __launch_bounds__(WG) void k_scan_apply(const uint64_t *in, uint64_t *out, const uint64_t *d_n,
                                                    uint64_t extra, uint64_t bound, const uint64_t *part)
 {
     uint64_t lo, hi;
     chunk_of(d_n, extra, bound, lo, hi);
-    uint64_t carry = part[blockIdx.x];
+    // the chunk's offset: the sums of the chunks before it (at most 1024)
+    uint64_t before = 0;
+    for (uint32_t k = threadIdx.x; k < blockIdx.x; k += WG) before += part[k];
+    uint64_t carry = block_sum(before);
     for (uint64_t base = lo; base < hi; base += WG) {
         const uint64_t i = base + threadIdx.x;
         const uint64_t v = i < hi ? in[i] : 0;
@@ -1245,7 +1235,6 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
     uint64_t *part = (uint64_t *)sc.get(G * 8 + 64);
     if (!part) return ms_fail(MS_ERR_NOMEM, "scan scratch");
     hipLaunchKernelGGL(k_scan_part, dim3(G), dim3(WG), 0, sc.s, d_in, d_n, extra, bound, part);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(WG), 0, sc.s, part, G);
     hipLaunchKernelGGL(k_scan_apply, dim3(G), dim3(WG), 0, sc.s, d_in, d_out, d_n, extra, bound,
                        (const uint64_t *)part);
     MS_HIP(hipGetLastError());

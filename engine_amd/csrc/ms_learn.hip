@@ -494,7 +494,7 @@ __global__ __launch_bounds__(WG) void k_mlp_predict(const float *__restrict__ pr
 __global__ __launch_bounds__(WG) void k_select_formats(const float *__restrict__ pred,
                                                        const uint64_t *__restrict__ sizes, uint64_t n,
                                                        float size_tol, float exact_tol, float time_margin,
-                                                       float4 norm, uint8_t *__restrict__ choice)
+                                                       float4 norm, uint32_t rule, uint8_t *__restrict__ choice)
 {
     const uint64_t c = (uint64_t)blockIdx.x * WG + threadIdx.x;
     if (c >= n) return;
@@ -514,6 +514,16 @@ __global__ __launch_bounds__(WG) void k_select_formats(const float *__restrict__
     const uint64_t *z = sizes + c * MS_NCAND;
     int ex = 0;
     for (int f = 1; f < MS_NCAND; f++) ex = z[f] < z[ex] ? f : ex;
+    if (rule == 1) {
+        // exact sizes admit, predicted times choose
+        int fast = ex;
+        for (int f = 0; f < MS_NCAND; f++) {
+            if ((double)z[f] > (1.0 + exact_tol) * (double)z[ex]) continue;
+            if (pt[f] * (1.f + time_margin) < pt[ex] && pt[f] < pt[fast]) fast = f;
+        }
+        choice[c] = (uint8_t)(fast == ex ? (ex | 0x80) : fast);
+        return;
+    }
     uint8_t out = (uint8_t)pick;
     if ((double)z[pick] > (1.0 + exact_tol) * (double)z[ex]) out = (uint8_t)(ex | 0x80);
     choice[c] = out;
@@ -791,8 +801,9 @@ extern "C" int ms_select_formats(const float *d_pred, const uint64_t *d_sizes, u
     if (n == 0) return MS_OK;
     hipStream_t s = (hipStream_t)stream;
     const float4 norm = make_float4(sp->size_off, sp->size_scale, sp->time_off, sp->time_scale);
+    if (sp->rule > 1) return ms_fail(MS_ERR_INVALID, "selection rule %u", sp->rule);
     hipLaunchKernelGGL(k_select_formats, dim3((n + WG - 1) / WG), dim3(WG), 0, s, d_pred, d_sizes, n, sp->size_tol,
-                       sp->exact_tol, sp->time_margin, norm, d_choice);
+                       sp->exact_tol, sp->time_margin, norm, sp->rule, d_choice);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

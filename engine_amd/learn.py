@@ -6,18 +6,22 @@ The reference chooses formats from calibration runs (src/calibration/
 bw_prof.cpp:71-292: time and size per format and bit width) and an
 exhaustive size search (exhaustive_size_format_selector.h:35-141).  Here the
 engine
-  1. calibrates: times the batched fused select+sum kernel on columns of
-     each candidate format at every bit width (ns per 512-value unit), and
-     on rle columns of several run lengths (ns per run);
-  2. labels chunks of 2^16 values generated on the device: exact size of
+  1. labels chunks of 2^16 values generated on the device: exact size of
      every candidate (closed forms from per-block bit widths) and decode
-     time (the calibration table applied to the chunk's bit widths);
-  3. trains the FP32 MLP on the device (24 features -> log2 size and log2
+     time, measured by compressing the chunks in every candidate and
+     timing them in the batched select+sum kernel's timed variant
+     (labels="measured"), or from a per-(format, bit width) timing table
+     (labels="table": ns per 512-value unit, ns per rle run);
+  2. trains the FP32 MLP on the device (24 features -> log2 size and log2
      time of the 5 candidates);
-  4. selects per chunk: the smallest predicted candidate, or one predicted
-     within `size_tol` of it and faster by more than `time_margin`; exact
-     sizes check the pick and replace it by the exact smallest when it is
-     more than `exact_tol` larger.
+  3. selects per chunk, rule 0: the smallest predicted candidate, or one
+     predicted within `size_tol` of it and faster by more than
+     `time_margin`; exact sizes check the pick and replace it by the exact
+     smallest when it is more than `exact_tol` larger.  Rule 1 (the
+     selection workload's): the exact smallest, replaced by the candidate
+     predicted fastest among those whose exact size is within `exact_tol`
+     of it, when that one is predicted faster by more than `time_margin`
+     (the exact sizes come from the same statistics pass as the features).
 Whatever is picked, the data stays bit-exact: only the format changes.
 """
 import ctypes
@@ -50,7 +54,7 @@ class ChunkSpec(ctypes.Structure):
 
 class SelectParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_float) for n in ("size_off", "size_scale", "time_off", "time_scale", "size_tol",
-                                               "time_margin", "exact_tol")]
+                                               "time_margin", "exact_tol")] + [("rule", ctypes.c_uint32)]
 
 
 def geo_table(a):
@@ -164,9 +168,11 @@ class MLP:
             self.h = None
 
 
-def select_formats(d_pred, d_sizes, n, size_tol=0.02, time_margin=0.15, exact_tol=0.03, stream=None):
-    """Per chunk: candidate index, and whether the exact-size fallback fired."""
-    sp = SelectParams(SIZE_OFF, SIZE_SCALE, TIME_OFF, TIME_SCALE, size_tol, time_margin, exact_tol)
+def select_formats(d_pred, d_sizes, n, size_tol=0.02, time_margin=0.15, exact_tol=0.03, rule=0, stream=None):
+    """Per chunk: candidate index, and whether the exact smallest was taken
+    (rule 0: the exact-size fallback fired; rule 1: no admitted candidate
+    was predicted enough faster)."""
+    sp = SelectParams(SIZE_OFF, SIZE_SCALE, TIME_OFF, TIME_SCALE, size_tol, time_margin, exact_tol, rule)
     d_c = hip.DeviceArray(n, np.uint8)
     ea.check(ea.lib().ms_select_formats(d_pred.ptr(), d_sizes.ptr(), n, ctypes.byref(sp), d_c.ptr(), stream))
     c = d_c.numpy()
@@ -420,8 +426,9 @@ def column_level_choice(esz, max_bw):
     return k, int(max_bw)
 
 
-def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=30, seed=0x4D6F7270 + 4,
-                           val=1 << 24, steps=10, warmup=2, log=None, cost=None, graph_leg=True, labels="measured"):
+def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=60, seed=0x4D6F7270 + 4,
+                           val=1 << 24, steps=10, warmup=2, log=None, cost=None, graph_leg=True, labels="measured",
+                           size_weight=4.0, rule=1):
     """The learned-selection workload: ncols columns of chunks_per_col
     chunks (a mixture of the format-cascade-sweep distributions switching
     every 2^16 values).  Trains the selector on ntrain generated chunks
@@ -451,7 +458,10 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=3
     net = MLP(seed=seed)
     t1 = time.perf_counter()
     # Adam with the step size halved after each third of the epochs
-    losses = np.concatenate([net.train(d_x, d_y, ntrain, np.ones(NOUT, np.float32), epochs=e, batch=4096, lr=lr,
+    # the size head is weighted up: the exact-size judge rejects picks more
+    # than 3% above the smallest, while a time error only costs speed
+    wout = np.concatenate([np.full(NCAND, size_weight), np.ones(NOUT - NCAND)]).astype(np.float32)
+    losses = np.concatenate([net.train(d_x, d_y, ntrain, wout, epochs=e, batch=4096, lr=lr,
                                        seed=seed + k)
                              for k, (e, lr) in enumerate(((epochs // 3, 2e-3), (epochs // 3, 1e-3),
                                                           (epochs - 2 * (epochs // 3), 5e-4)))])
@@ -477,7 +487,7 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=3
         st = d_st.numpy(n * NSTAT).reshape(n, NSTAT)
         esz = d_sz.numpy(n * NCAND).reshape(n, NCAND)
         pred = net.predict(d_f, n)
-        choice, fb = select_formats(pred, d_sz, n)
+        choice, fb = select_formats(pred, d_sz, n, rule=rule)
         ex = esz.argmin(axis=1)
         fallbacks += int(fb.sum())
         picks += np.bincount(choice, minlength=NCAND)
@@ -497,7 +507,8 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=3
     t_prep = time.perf_counter() - t2
     nchunks = ncols * n
     say(f"selection workload: {ncols} columns x {n} chunks prepared in {t_prep:.1f}s; footprint learned "
-        f"{foot['learned']} exact {foot['exact']} plain {foot['plain']}; fallbacks {fallbacks}")
+        f"{foot['learned']} exact {foot['exact']} plain {foot['plain']}; "
+        f"{'fallbacks' if rule == 0 else 'exact smallest kept'} {fallbacks}")
     resc = ea.DeviceU64(ncols)
     slots = [resc.at(k) for k in range(ncols)]
 
@@ -566,7 +577,7 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=3
         "loss_last_epoch": float(losses[-1]), "train_seconds": round(t_train, 2),
         "label_seconds": round(t_cal, 1),
         "footprint_bytes": foot, "learned_over_exact": round(foot["learned"] / foot["exact"], 5),
-        "exact_fallbacks": fallbacks, "chunks": nchunks,
+        "rule": rule, ("exact_fallbacks" if rule == 0 else "exact_smallest_kept"): fallbacks, "chunks": nchunks,
         "learned_picks": {CAND_NAMES[k]: int(picks[k]) for k in range(NCAND)},
         "plain_ms": round(ms_plain, 3), "plain_graph_ms": round(ms_graph, 3) if ms_graph else None,
         "batched_learned_ms": round(out["learned"][0], 3), "batched_exact_ms": round(out["exact"][0], 3),

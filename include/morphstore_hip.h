@@ -562,12 +562,17 @@ int ms_mlp_free(ms_mlp *m);
 typedef struct ms_select_params {
     float size_off, size_scale;   /* predicted log2 bytes = y * size_scale + size_off */
     float time_off, time_scale;   /* predicted log2 ns    = y * time_scale + time_off */
-    float size_tol;               /* candidates within (1 + size_tol) x the smallest size prediction ... */
+    float size_tol;               /* rule 0: candidates within (1 + size_tol) x the smallest size prediction ... */
     float time_margin;            /* ... replace it when predicted faster by more than this fraction */
-    float exact_tol;              /* fallback to the exact smallest beyond (1 + exact_tol) x it */
+    float exact_tol;              /* rule 0: fallback to the exact smallest beyond (1 + exact_tol) x it;
+                                     rule 1: the candidates admitted, within (1 + exact_tol) x the exact smallest */
+    uint32_t rule;                /* 0: pick by predicted size (exact sizes judge the pick);
+                                     1: the exact smallest, replaced by the admitted candidate predicted
+                                        fastest when that is faster by more than time_margin */
 } ms_select_params;
 
-/* Per chunk a candidate (bit 7 set: the exact-size fallback was taken). */
+/* Per chunk a candidate (bit 7 set: rule 0's exact-size fallback was taken,
+ * or rule 1 kept the exact smallest). */
 int ms_select_formats(const float *d_pred, const uint64_t *d_sizes, uint64_t n,
                       const ms_select_params *sp, uint8_t *d_choice, void *stream);
 

@@ -126,6 +126,16 @@ def test_training_and_selection():
     t_learned = etm[np.arange(n), choice].sum()
     t_exact = etm[np.arange(n), exact_choice].sum()
     assert t_learned <= t_exact * 1.02, (t_learned, t_exact)
+    # rule 1: exact sizes admit the candidates within 3% of the smallest,
+    # the predicted times choose among them (a 15% predicted gain to switch)
+    choice1, kept = learn.select_formats(pred, d_sz, n, rule=1)
+    assert np.all(esz[np.arange(n), choice1] <= 1.03 * best + 1e-9)
+    assert np.array_equal(kept, choice1 == exact_choice)
+    pt = np.exp2(pred.numpy(n * 10).reshape(n, 10)[:, 5:].astype(np.float64) * learn.TIME_SCALE + learn.TIME_OFF)
+    sw = np.flatnonzero(~kept)
+    assert np.all(pt[sw, choice1[sw]] * 1.15 <= pt[sw, exact_choice[sw]] * (1 + 1e-4))
+    t_rule1 = etm[np.arange(n), choice1].sum()
+    assert t_rule1 <= t_exact * 1.01, (t_rule1, t_exact)
 
 
 def test_network_gain_over_exact_sizing():
