@@ -144,6 +144,23 @@ def test_select_positions_from_ballots(oracle, out_f, in_f):
             assert np.array_equal(g.values(), np.flatnonzero(v < thr).astype(np.uint64)), (thr, lanes)
 
 
+@pytest.mark.parametrize("in_f", [dynamic_vbp_f(512, 2048, 8), default_for_dynamic_vbp_f(8)], ids=repr)
+def test_select_positions_from_ballots_small_and_ragged(oracle, in_f):
+    """The ballot-fed emit at the edges: empty and one-value inputs, counts
+    around a unit, a delta tile and a write-iterator flush, with and
+    without a scalar remainder."""
+    for n in (0, 1, 5, 511, 512, 513, 1023, 2049, 4095, 4097):
+        v = datagen.uniform_bits(30 + n, n, 9)
+        col = ea.morph(in_f, ea.upload_values(v))
+        ocol = oracle.morph(in_f, v)
+        for out_f in (default_delta_dynamic_vbp_f(8), dynamic_vbp_f(512, 2048, 8)):
+            for thr, lanes in ((0, 8), (300, 1), (300, 8), (1 << 9, 8)):
+                g = ea.select(MS_CMP_LT, thr, out_f, col, lanes)
+                o = oracle.select(MS_CMP_LT, thr, out_f, ocol, lanes)
+                assert_same_column(g, o)
+                assert np.array_equal(g.values(), np.flatnonzero(v < thr).astype(np.uint64)), (n, thr, lanes)
+
+
 @pytest.mark.parametrize("n", [1, 5, 13, 100])
 @pytest.mark.parametrize("out_f", OUT_FORMATS, ids=OUT_IDS)
 def test_select_tiny_inputs(oracle, n, out_f):
