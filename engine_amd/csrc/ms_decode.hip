@@ -487,25 +487,31 @@ __global__ __launch_bounds__(WG) void k_copy_rest(ms_colv c, uint64_t *out)
 // run's copies with contiguous stores.
 static constexpr uint64_t RLE_XCH = 1024;
 
-__global__ __launch_bounds__(WG) void k_rle_chunk_lens(ms_colv c, uint64_t *clen, uint64_t nch_bound)
+__device__ __forceinline__ uint64_t block_sum(uint64_t v);
+
+// A workgroup per chunk: each wave sums a quarter of its run lengths.
+// *d_nch receives the number of chunks (the scan's length).
+__global__ __launch_bounds__(WG) void k_rle_chunk_lens(ms_colv c, uint64_t *clen, uint64_t *d_nch)
 {
+    static_assert(RLE_XCH % (WAVES * 64) == 0, "a chunk splits into whole wave rows");
+    constexpr uint32_t PER = RLE_XCH / (WAVES * 64);   // loads per lane
     const uint64_t runs = c.meta->size_used / 16;
+    const uint64_t nch = (runs + RLE_XCH - 1) / RLE_XCH;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *d_nch = nch;
     const u64x2 *w = (const u64x2 *)c.data;
-    const uint32_t t = threadIdx.x & 63;
-    const uint64_t W = (uint64_t)gridDim.x * WAVES;
-    for (uint64_t ch = (uint64_t)blockIdx.x * WAVES + wave_index(); ch < nch_bound; ch += W) {
-        uint64_t acc = 0;
-        const uint64_t r0 = ch * RLE_XCH;
-        uint64_t len[RLE_XCH / 64];
+    for (uint64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        const uint64_t r0 = ch * RLE_XCH + (uint64_t)(threadIdx.x >> 6) * (RLE_XCH / WAVES) + (threadIdx.x & 63);
+        uint64_t len[PER];
 #pragma unroll
-        for (uint32_t i = 0; i < RLE_XCH / 64; i++) {
-            const uint64_t r = r0 + 64 * i + t;
+        for (uint32_t i = 0; i < PER; i++) {
+            const uint64_t r = r0 + 64 * i;
             len[i] = r < runs ? gload(w + r).y : 0ull;
         }
+        uint64_t acc = 0;
 #pragma unroll
-        for (uint32_t i = 0; i < RLE_XCH / 64; i++) acc += len[i];
-        acc = wave_sum_u64(acc);
-        if (t == 0) clen[ch] = acc;
+        for (uint32_t i = 0; i < PER; i++) acc += len[i];
+        acc = block_sum(acc);
+        if (threadIdx.x == 0) clen[ch] = acc;
     }
 }
 
@@ -864,10 +870,10 @@ int launch_decompress(const ms_col *in, uint64_t *d_out, hipStream_t s, ms_dev_m
         const uint64_t nch_bound = (in->size_alloc / 16 + 1) / RLE_XCH + 1;
         uint64_t *clen = (uint64_t *)sc.get(nch_bound * 8);
         uint64_t *cst = (uint64_t *)sc.get(nch_bound * 8);
-        if (!clen || !cst) return ms_fail(MS_ERR_NOMEM, "rle scratch");
-        hipLaunchKernelGGL(k_rle_chunk_lens, dim3(stride_grid((const void *)k_rle_chunk_lens, nch_bound)), dim3(WG), 0,
-                           s, v, clen, nch_bound);
-        MS_TRY(launch_exclusive_scan(clen, cst, nch_bound, sc));
+        uint64_t *d_nch = (uint64_t *)sc.get(64);
+        if (!clen || !cst || !d_nch) return ms_fail(MS_ERR_NOMEM, "rle scratch");
+        hipLaunchKernelGGL(k_rle_chunk_lens, dim3(grid_for(nch_bound, 1, 8192)), dim3(WG), 0, s, v, clen, d_nch);
+        MS_TRY(launch_exclusive_scan_n(clen, cst, d_nch, 0, nch_bound, sc));
         auto k = k_rle_expand_chunks<false>;
         hipLaunchKernelGGL(k, dim3(stride_grid((const void *)k, nch_bound)), dim3(WG), 0, s, v, cst, d_out);
         if (om) hipLaunchKernelGGL(k_set_meta_uncompr, dim3(1), dim3(1), 0, s, (const ms_dev_meta *)in->meta, om);
