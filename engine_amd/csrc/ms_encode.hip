@@ -410,12 +410,18 @@ __device__ __forceinline__ void pack_block(const emit_layout &l, const emit_plan
 }
 
 // Bit width and header+payload bytes of every innermost block.
+// With d_nb the blocks past the column's own (*d_nb, written here) are left
+// alone, for a scan sized by that count; without it they get size 0.
 __global__ __launch_bounds__(64) void k_block_sizes(emit_layout l, const uint64_t *v, const uint64_t *d_m,
                                                     const uint64_t *refs, uint64_t nb_bound, uint64_t *sz,
-                                                    uint32_t *bws)
+                                                    uint32_t *bws, uint64_t *d_nb)
 {
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
+    if (d_nb) {
+        if (blockIdx.x == 0 && t == 0) *d_nb = pl.nb;
+        if (nb_bound > pl.nb) nb_bound = pl.nb;
+    }
     for (uint64_t b = blockIdx.x; b < nb_bound; b += gridDim.x) {
         if (b >= pl.nb) {
             if (t == 0) { sz[b] = 0; bws[b] = 0; }
@@ -460,19 +466,17 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
 
 // Sizes from a select's flags, one innermost block per wave (dynamic LDS:
 // B + pred gathered values).
+// *d_nb receives the number of blocks (the scan's length).
 __global__ __launch_bounds__(64) void k_flag_sizes(emit_layout l, flag_src fs, const uint64_t *d_m,
-                                                   const uint64_t *refs, uint64_t nb_bound, uint64_t *sz,
+                                                   const uint64_t *refs, uint64_t *d_nb, uint64_t *sz,
                                                    uint32_t *bws)
 {
     extern __shared__ unsigned long long fvals[];
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
     const uint64_t nunits = *fs.sf.d_units;
-    for (uint64_t b = blockIdx.x; b < nb_bound; b += gridDim.x) {
-        if (b >= pl.nb) {
-            if (t == 0) { sz[b] = 0; bws[b] = 0; }
-            continue;
-        }
+    if (blockIdx.x == 0 && t == 0) *d_nb = pl.nb;
+    for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
         const uint64_t r0 = flag_r0(l, fs, b), lo = b * l.B;
         gather_ranks(fs.sf, nunits, fs.ustart[b], r0, lo + l.B - r0,
                      [&](uint64_t i, uint64_t row) { fvals[i] = row; });
@@ -640,10 +644,12 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
                                flag_src{}, d_m, refs);
             MS_HIP(hipGetLastError());
         }
+        uint64_t *d_nb = (uint64_t *)sc.get(64);
+        if (!d_nb) return ms_fail(MS_ERR_NOMEM, "emit scratch");
         hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(nb_bound, 1, 65536)), dim3(64), 0, s, l, d_v, d_m,
-                           (const uint64_t *)refs, nb_bound, sz, bws);
+                           (const uint64_t *)refs, nb_bound, sz, bws, d_nb);
         MS_HIP(hipGetLastError());
-        MS_TRY(launch_exclusive_scan(sz, start, nb_bound, sc));
+        MS_TRY(launch_exclusive_scan_n(sz, start, d_nb, 0, nb_bound, sc));
         if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
             return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
         hipLaunchKernelGGL(k_pack_blocks, dim3(grid_for(nb_bound, 1, 65536)), dim3(64), l.B * 8, s, l, d_v, d_m,
@@ -739,10 +745,12 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uin
     }
     const uint32_t vals_lds = (uint32_t)((l.B + fs.pred) * 8);
     const uint32_t fl_grid = grid_for(nb_bound, 1, 65536);
-    hipLaunchKernelGGL(k_flag_sizes, dim3(fl_grid), dim3(64), vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
-                       nb_bound, sz, bws);
+    uint64_t *d_nb = (uint64_t *)sc.get(64);
+    if (!d_nb) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+    hipLaunchKernelGGL(k_flag_sizes, dim3(fl_grid), dim3(64), vals_lds, s, l, fs, d_m, (const uint64_t *)refs, d_nb,
+                       sz, bws);
     MS_HIP(hipGetLastError());
-    MS_TRY(launch_exclusive_scan(sz, start, nb_bound, sc));
+    MS_TRY(launch_exclusive_scan_n(sz, start, d_nb, 0, nb_bound, sc));
     if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
         return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
     hipLaunchKernelGGL(k_flag_pack, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
@@ -830,7 +838,7 @@ int launch_chunk_sizes(chunk_emit &ce, uint64_t *d_csz, temps &sc)
         MS_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_block_sizes, dim3(grid_for(ce.nb + 1, 1, 65536)), dim3(64), 0, s, l, ce.v, ce.d_m,
-                       (const uint64_t *)ce.refs, ce.nb + 1, ce.sz, ce.bws);
+                       (const uint64_t *)ce.refs, ce.nb + 1, ce.sz, ce.bws, (uint64_t *)nullptr);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan(ce.sz, ce.start, ce.nb + 1, sc));
     if (d_csz) {
