@@ -458,7 +458,7 @@ def run_selection_workload(ncols=1024, chunks_per_col=64, ntrain=32768, epochs=6
     net = MLP(seed=seed)
     t1 = time.perf_counter()
     # Adam with the step size halved after each third of the epochs
-    # the size head is weighted up: the exact-size judge rejects picks more
+    # the size head is weighted up: the exact-size check rejects picks more
     # than 3% above the smallest, while a time error only costs speed
     wout = np.concatenate([np.full(NCAND, size_weight), np.ones(NOUT - NCAND)]).astype(np.float32)
     losses = np.concatenate([net.train(d_x, d_y, ntrain, wout, epochs=e, batch=4096, lr=lr,

@@ -566,7 +566,7 @@ typedef struct ms_select_params {
     float time_margin;            /* ... replace it when predicted faster by more than this fraction */
     float exact_tol;              /* rule 0: fallback to the exact smallest beyond (1 + exact_tol) x it;
                                      rule 1: the candidates admitted, within (1 + exact_tol) x the exact smallest */
-    uint32_t rule;                /* 0: pick by predicted size (exact sizes judge the pick);
+    uint32_t rule;                /* 0: pick by predicted size (exact sizes check the pick);
                                      1: the exact smallest, replaced by the admitted candidate predicted
                                         fastest when that is faster by more than time_margin */
 } ms_select_params;
