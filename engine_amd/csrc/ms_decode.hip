@@ -993,16 +993,19 @@ __device__ __forceinline__ void unit_flags(const ms_colv &c, const ms_parts &pt,
 
 // Decodes and tests every unit once: its 8 ballots go to flags[8 u ..]
 // (64 bytes per 512 values) and its selected count to cnt[u].
-__global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, const uint64_t *d_units, uint64_t *cnt,
+__global__ __launch_bounds__(WG) void k_sel_count_blocks(ms_colv c, ms_pred p, uint64_t *d_units, uint64_t *cnt,
                                                          uint64_t *flags)
 {
     const ms_parts pt = parts_of(c);
-    const uint64_t nunits = *d_units;
+    const uint64_t nunits = (pt.n + 511) / 512;
     const uint64_t ncu = pt.n_compr >> 9;
     const uint32_t t = threadIdx.x & 63;
     const uint64_t W = (uint64_t)gridDim.x * WAVES;
     uint64_t u = (uint64_t)blockIdx.x * WAVES + wave_index();
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[nunits] = 0;   // the scan's total slot
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *d_units = nunits;   // for the scan and the compaction after this kernel
+        cnt[nunits] = 0;     // the scan's total slot
+    }
     uint64_t e_n, r_n;   // the next unit's entry, requested one unit ahead
     unit_entry(c, ncu, u, e_n, r_n);
     for (; u < nunits; u += W) {
@@ -1083,9 +1086,7 @@ int launch_select_flags(const ms_col *in, const ms_pred &p, sel_flags &sf, temps
     uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
     uint64_t *flags = (uint64_t *)sc.get(units * 64 + 64);
     if (!d_u || !cnt || !off || !flags) return ms_fail(MS_ERR_NOMEM, "select scratch");
-    hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
-    hipLaunchKernelGGL(k_sel_count_blocks, dim3(units_grid(nb)), dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt,
-                       flags);
+    hipLaunchKernelGGL(k_sel_count_blocks, dim3(units_grid(nb)), dim3(WG), 0, s, v, p, d_u, cnt, flags);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 1, units + 1, sc));
     sf.flags = flags;
@@ -1109,9 +1110,8 @@ int launch_select_positions(const ms_col *in, const ms_pred &p, uint64_t *d_pos,
         uint64_t *off = (uint64_t *)sc.get(units * 8 + 64);
         uint64_t *flags = (uint64_t *)sc.get(units * 64 + 64);
         if (!d_u || !cnt || !off || !flags) return ms_fail(MS_ERR_NOMEM, "select scratch");
-        hipLaunchKernelGGL(k_units_of, dim3(1), dim3(1), 0, s, in->meta, d_u);
         const dim3 grid(units_grid(nb));
-        hipLaunchKernelGGL(k_sel_count_blocks, grid, dim3(WG), 0, s, v, p, (const uint64_t *)d_u, cnt, flags);
+        hipLaunchKernelGGL(k_sel_count_blocks, grid, dim3(WG), 0, s, v, p, d_u, cnt, flags);
         MS_HIP(hipGetLastError());
         MS_TRY(launch_exclusive_scan_n(cnt, off, d_u, 0, units, sc));
         hipLaunchKernelGGL(k_sel_compact_blocks, grid, dim3(WG), 0, s, (const uint64_t *)flags,

@@ -544,20 +544,13 @@ __global__ __launch_bounds__(WG) void k_pack_static(emit_layout l, const uint64_
     }
 }
 
-// delta_f tile carries: the S values preceding each tile's first value.
-__global__ __launch_bounds__(WG) void k_delta_carries_values(emit_layout l, const uint64_t *v, const uint64_t *d_m,
-                                                             uint64_t *refs)
-{
-    const emit_plan pl = plan_of(l, *d_m);
-    const uint64_t ntiles = (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE;
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < ntiles * 8; i += (uint64_t)gridDim.x * WG) {
-        const uint64_t tile = i / 8, j = i % 8;
-        refs[i] = (tile == 0 || j >= l.dstep) ? 0ull : v[tile * MS_DELTA_TILE - l.dstep + j];
-    }
-}
-
-__global__ void k_emit_finish(emit_layout l, const uint64_t *d_m, const uint64_t *start, const uint32_t *bws,
-                              ms_dev_meta *meta, uint64_t *d_append)
+// The end of an emission in one launch: the output's meta (block 0), the
+// delta_f tile carries (carries non-null) and the raw rest behind the gap;
+// every workgroup derives the gap's end itself.  start / bws are null for
+// static_vbp_f.
+__global__ __launch_bounds__(WG) void k_emit_tail(emit_layout l, const uint64_t *v, const uint64_t *d_m,
+                                                  const uint64_t *start, const uint32_t *bws, ms_dev_meta *meta,
+                                                  uint64_t *d_append, uint64_t *carries, uint8_t *out)
 {
     const emit_plan pl = plan_of(l, *d_m);
     uint64_t C;
@@ -566,24 +559,28 @@ __global__ void k_emit_finish(emit_layout l, const uint64_t *d_m, const uint64_t
     else C = start[pl.nb - 1] + hdr_of(l, pl.nb - 1) + l.B * bws[pl.nb - 1] / 8;
     const uint64_t R = ms_align64(C + l.gap);
     const uint64_t rest = pl.m - pl.m_c;
-    if (R + 8 * rest > l.cap) emit_overflow(l);
-    meta->count = pl.m;
-    meta->size_compr = C;
-    meta->size_used = rest ? R + 8 * rest : C;
-    meta->aux[0] = rest ? R + 8 * rest : R;
-    meta->aux[1] = R;
-    if (d_append) *d_append = meta->aux[0];
-}
-
-__global__ __launch_bounds__(WG) void k_emit_rest(emit_layout l, const uint64_t *v, const uint64_t *d_m,
-                                                  const ms_dev_meta *meta, uint8_t *out)
-{
-    const emit_plan pl = plan_of(l, *d_m);
-    const uint64_t rest = pl.m - pl.m_c;
-    if (meta->aux[1] + 8 * rest > l.cap) return;   // reported by k_emit_finish
-    uint64_t *dst = (uint64_t *)(out + meta->aux[1]);
-    for (uint64_t i = (uint64_t)blockIdx.x * WG + threadIdx.x; i < rest; i += (uint64_t)gridDim.x * WG)
-        dst[i] = v[pl.m_c + i];
+    const bool fits = R + 8 * rest <= l.cap;
+    const uint64_t tid = (uint64_t)blockIdx.x * WG + threadIdx.x, nth = (uint64_t)gridDim.x * WG;
+    if (tid == 0) {
+        if (!fits) emit_overflow(l);
+        meta->count = pl.m;
+        meta->size_compr = C;
+        meta->size_used = rest ? R + 8 * rest : C;
+        meta->aux[0] = rest ? R + 8 * rest : R;
+        meta->aux[1] = R;
+        if (d_append) *d_append = meta->aux[0];
+    }
+    if (carries) {
+        const uint64_t ntiles = (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE;
+        for (uint64_t i = tid; i < ntiles * 8; i += nth) {
+            const uint64_t tile = i / 8, j = i % 8;
+            carries[i] = (tile == 0 || j >= l.dstep) ? 0ull : v[tile * MS_DELTA_TILE - l.dstep + j];
+        }
+    }
+    if (fits) {
+        uint64_t *dst = (uint64_t *)(out + R);
+        for (uint64_t i = tid; i < rest; i += nth) dst[i] = v[pl.m_c + i];
+    }
 }
 
 // The emit layout of an output column written as `chunk`-value calls.
@@ -629,8 +626,9 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         hipLaunchKernelGGL(k_pack_static, dim3(grid_for(m_bound * l.bw / 64 + 1, WG * 4ull)), dim3(WG), 0, s, l, d_v,
                            d_m, out->data);
         MS_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)nullptr,
-                           (const uint32_t *)nullptr, out->meta, d_append);
+        hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
+                           (const uint64_t *)nullptr, (const uint32_t *)nullptr, out->meta, d_append,
+                           (uint64_t *)nullptr, out->data);
     } else {
         uint64_t *sz = (uint64_t *)sc.get(nb_bound * 8 + 64);
         uint64_t *start = (uint64_t *)sc.get(nb_bound * 8 + 64);
@@ -656,18 +654,13 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
                            (const uint64_t *)refs, (const uint64_t *)start, (const uint32_t *)bws, out->data,
                            out->index, out->refs);
         MS_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)start,
-                           (const uint32_t *)bws, out->meta, d_append);
-        if (f.kind == MS_DELTA) {
-            if (out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
-                return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
-            hipLaunchKernelGGL(k_delta_carries_values, dim3(grid_for(m_bound / MS_DELTA_TILE * 8 + 8, WG)),
-                               dim3(WG), 0, s, l, d_v, d_m, out->refs);
-        }
+        if (f.kind == MS_DELTA && out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
+            return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
+        const uint64_t tail = std::max<uint64_t>(l.bs + l.K + 1, m_bound / MS_DELTA_TILE * 8 + 8);
+        hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(tail, WG, 1024)), dim3(WG), 0, s, l, d_v, d_m,
+                           (const uint64_t *)start, (const uint32_t *)bws, out->meta, d_append,
+                           f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data);
     }
-    MS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
-                       (const ms_dev_meta *)out->meta, out->data);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
@@ -756,20 +749,15 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uin
     hipLaunchKernelGGL(k_flag_pack, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
                        (const uint64_t *)start, (const uint32_t *)bws, out->data, out->index, out->refs);
     MS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_emit_finish, dim3(1), dim3(1), 0, s, l, d_m, (const uint64_t *)start, (const uint32_t *)bws,
-                       out->meta, (uint64_t *)nullptr);
     hipLaunchKernelGGL(k_flag_ranks, dim3(grid_for(m_bound / MS_DELTA_TILE + 1, 1, 65536)), dim3(64), 0, s, l, fs,
                        d_m, d_total, pos);
     MS_HIP(hipGetLastError());
-    if (f.kind == MS_DELTA) {
-        if (out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
-            return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
-        hipLaunchKernelGGL(k_delta_carries_values, dim3(grid_for(m_bound / MS_DELTA_TILE * 8 + 8, WG)), dim3(WG), 0,
-                           s, l, (const uint64_t *)pos, d_m, out->refs);
-        MS_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_emit_rest, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, (const uint64_t *)pos,
-                       d_m, (const ms_dev_meta *)out->meta, out->data);
+    if (f.kind == MS_DELTA && out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
+        return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
+    const uint64_t tail = std::max<uint64_t>(l.bs + l.K + 1, m_bound / MS_DELTA_TILE * 8 + 8);
+    hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(tail, WG, 1024)), dim3(WG), 0, s, l, (const uint64_t *)pos, d_m,
+                       (const uint64_t *)start, (const uint32_t *)bws, out->meta, (uint64_t *)nullptr,
+                       f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
