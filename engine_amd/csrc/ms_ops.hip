@@ -132,16 +132,19 @@ __global__ __launch_bounds__(WG) void k_isect_ranges(const uint64_t *a, const ui
     }
 }
 
-// The flags of chunk ck as a bitmask: bit k of word w (w < ISECT_WORDS)
-// is a[i0 + 64 w + k]'s flag; cnt[ck] = the chunk's kept values.
+// The flags of chunk ck: cnt[ck] = the chunk's kept values, and the kept
+// values themselves, in order, at kept[ck * ISECT_CHUNK ..] (compacted in
+// the chunk from its flag bitmask: bit k of word w is a[i0 + 64 w + k]'s
+// flag).
 constexpr uint32_t ISECT_WORDS = ISECT_CHUNK / 64;
 
 __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint64_t *d_na, const uint64_t *b,
-                                                   const uint64_t *d_nb, const uint64_t *rng, uint64_t *mask,
+                                                   const uint64_t *d_nb, const uint64_t *rng, uint64_t *kept,
                                                    uint64_t *cnt)
 {
     __shared__ uint64_t sb[ISECT_LDS];
     __shared__ uint32_t s_cnt[ISECT_WORDS];
+    __shared__ uint64_t s_mask[ISECT_WORDS];
     const uint64_t na = *d_na, nb = *d_nb;
     const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
     for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
@@ -185,6 +188,7 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
                 }
             }
         }
+        uint32_t kbits = 0;   // bit q: the thread's value of row q is kept
 #pragma unroll 1
         for (uint32_t q = 0; q < Q; q++) {
             const uint32_t e = q * WG + threadIdx.x;   // index in the chunk
@@ -203,12 +207,25 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
                 keep = rank < c;
             }
             const uint64_t m = __builtin_amdgcn_ballot_w64(keep);
+            kbits |= (uint32_t)keep << q;
             if ((threadIdx.x & 63) == 0) {
-                mask[ck * ISECT_WORDS + (e >> 6)] = m;
+                s_mask[e >> 6] = m;
                 s_cnt[e >> 6] = (uint32_t)__builtin_popcountll(m);
             }
         }
         __syncthreads();
+        // each kept value to its rank in the chunk: the kept values of the
+        // rows before its row, then those before it in the row
+        const uint32_t lane = threadIdx.x & 63;
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+        for (uint32_t q = 0; q < Q; q++) {
+            const uint32_t row = (q * WG + threadIdx.x) >> 6;
+            uint32_t pre = 0;
+            for (uint32_t w = 0; w < row; w++) pre += s_cnt[w];
+            if ((kbits >> q) & 1)
+                kept[ck * ISECT_CHUNK + pre + (uint32_t)__builtin_popcountll(s_mask[row] & below)] = v[q];
+        }
         if (threadIdx.x == 0) {
             uint64_t c = 0;
 #pragma unroll
@@ -219,38 +236,16 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
     }
 }
 
-// Writes chunk ck's kept values from off[ck] on, in order; *d_cnt = all
-// kept values.
-__global__ __launch_bounds__(WG) void k_isect_emit(const uint64_t *a, const uint64_t *d_na, const uint64_t *mask,
-                                                   const uint64_t *cnt, const uint64_t *off, uint64_t *out,
-                                                   uint64_t *d_cnt)
+// Copies chunk ck's kept values to off[ck] on; *d_cnt = all kept values.
+__global__ __launch_bounds__(WG) void k_isect_emit(const uint64_t *d_na, const uint64_t *kept, const uint64_t *cnt,
+                                                   const uint64_t *off, uint64_t *out, uint64_t *d_cnt)
 {
-    __shared__ uint32_t s_pre[ISECT_WORDS];
     const uint64_t na = *d_na;
     const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
-    const uint32_t lane = threadIdx.x & 63;
     if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = nchunks ? off[nchunks - 1] + cnt[nchunks - 1] : 0;
     for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
-        const uint64_t i0 = ck * ISECT_CHUNK, i1 = i0 + ISECT_CHUNK < na ? i0 + ISECT_CHUNK : na;
-        if (threadIdx.x == 0) {
-            uint32_t p = 0;
-#pragma unroll
-            for (uint32_t w = 0; w < ISECT_WORDS; w++) {
-                s_pre[w] = p;
-                p += (uint32_t)__builtin_popcountll(mask[ck * ISECT_WORDS + w]);
-            }
-        }
-        __syncthreads();
-        const uint64_t base = off[ck];
-#pragma unroll 1
-        for (uint32_t q = 0; q < ISECT_CHUNK / WG; q++) {
-            const uint32_t e = q * WG + threadIdx.x;
-            const uint64_t i = i0 + e;
-            const uint64_t m = mask[ck * ISECT_WORDS + (e >> 6)];
-            if (i < i1 && ((m >> lane) & 1))
-                out[base + s_pre[e >> 6] + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = a[i];
-        }
-        __syncthreads();
+        const uint64_t base = off[ck], c = cnt[ck];
+        for (uint64_t k = threadIdx.x; k < c; k += WG) out[base + k] = kept[ck * ISECT_CHUNK + k];
     }
 }
 
@@ -260,19 +255,19 @@ int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound,
     (void)nb_bound;
     const uint64_t chunks = na_bound / ISECT_CHUNK + 1;
     uint64_t *rng = (uint64_t *)sc.get(chunks * 16 + 64);
-    uint64_t *mask = (uint64_t *)sc.get(chunks * ISECT_WORDS * 8 + 64);
+    uint64_t *kept = (uint64_t *)sc.get(chunks * ISECT_CHUNK * 8 + 64);
     uint64_t *cnt = (uint64_t *)sc.get(chunks * 8 + 64);
     uint64_t *off = (uint64_t *)sc.get(chunks * 8 + 64);
     uint64_t *d_nch = (uint64_t *)sc.get(64);
-    if (!rng || !mask || !cnt || !off || !d_nch) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
+    if (!rng || !kept || !cnt || !off || !d_nch) return ms_fail(MS_ERR_NOMEM, "intersect scratch");
     hipLaunchKernelGGL(k_isect_ranges, dim3(grid_for(chunks, WG, 2048)), dim3(WG), 0, sc.s, a, d_na, b, d_nb, rng,
                        d_nch);
     hipLaunchKernelGGL(k_isect_mark, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na, b, d_nb,
-                       (const uint64_t *)rng, mask, cnt);
+                       (const uint64_t *)rng, kept, cnt);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(cnt, off, d_nch, 0, chunks, sc));
-    hipLaunchKernelGGL(k_isect_emit, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, a, d_na,
-                       (const uint64_t *)mask, (const uint64_t *)cnt, (const uint64_t *)off, d_out, d_cnt);
+    hipLaunchKernelGGL(k_isect_emit, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, d_na, (const uint64_t *)kept,
+                       (const uint64_t *)cnt, (const uint64_t *)off, d_out, d_cnt);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
