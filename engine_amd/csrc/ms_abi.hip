@@ -1009,36 +1009,6 @@ __global__ void k_split_select(const ms_dev_meta *in, ms_geom g, uint32_t lanes,
     *d_flag = r != 0;
 }
 
-// k_split_select over a select's flags: the selected rows below the
-// threshold are counted from the unit offsets and the threshold unit's
-// ballots; *d_cnt receives the number of selected rows.
-__global__ void k_split_select_flags(const ms_dev_meta *in, ms_geom g, uint32_t lanes, sel_flags sf, uint64_t *d_cnt,
-                                     uint64_t *d_na, uint64_t *d_flag)
-{
-    const ms_parts pt = ms_parts_of(g, in->count, in->size_used, in->size_compr);
-    const bool has_rest = g.kind == MS_UNCOMPR ? in->count != 0 : in->size_compr != in->size_used;
-    const uint64_t r = has_rest ? pt.n_rest % lanes : 0;
-    const uint64_t thr = pt.n - r;
-    const uint64_t nunits = *sf.d_units, total = sf.uoff[nunits];
-    const uint64_t u = thr / 512, w = thr % 512;
-    uint64_t na = u < nunits ? sf.uoff[u] : total;
-    if (u < nunits && w) {
-        // row 64 gl + 8 m + j of the unit is bit 8 gl + j of ballot m
-        for (uint32_t m = 0; m < 8; m++) {
-            const uint64_t B = sf.flags[8 * u + m];
-            for (uint32_t gl = 0; gl < 8; gl++) {
-                const uint64_t row = 64 * gl + 8 * m;
-                if (row >= w) continue;
-                const uint64_t k = w - row < 8 ? w - row : 8;
-                na += __popcll((B >> (8 * gl)) & ((1ull << k) - 1));
-            }
-        }
-    }
-    *d_cnt = total;
-    *d_na = na;
-    *d_flag = r != 0;
-}
-
 __global__ void k_split_project(const ms_dev_meta *pos, ms_geom g, uint32_t lanes, uint64_t *d_na, uint64_t *d_flag)
 {
     const ms_parts pt = ms_parts_of(g, pos->count, pos->size_used, pos->size_compr);
@@ -1222,14 +1192,11 @@ static int select_pred(ms_col **out, const ms_pred &p, const ms_format_desc *out
         // output format; pos only receives the few read by rank
         sel_flags sf;
         MS_TRY(launch_select_flags(in, p, sf, sc));
-        hipLaunchKernelGGL(k_split_select_flags, dim3(1), dim3(1), 0, S(stream), in->meta, in->g, lanes, sf, d_cnt,
-                           d_na, d_flag);
-        MS_HIP(hipGetLastError());
+        const sel_split sp = {in->meta, in->g, lanes, d_cnt, d_na, d_flag};
         const uint64_t chunk = wit_chunk(*out_f);
         ms_col *c;
         MS_TRY(new_col(*out_f, nb + 64, out_capacity(*out_f, nb + 64, chunk), S(stream), &c));
-        int rc = launch_emit_flags(c, sf, pos, d_na, d_cnt, nb, chunk, sc);
-        if (!rc) rc = finish_split(c, pos, d_na, d_cnt, d_flag, S(stream));
+        const int rc = launch_emit_flags(c, sf, sp, pos, nb, chunk, sc);   // split, emission and raw append
         if (rc) { ms_col_free(c); return rc; }
         *out = c;
         return MS_OK;

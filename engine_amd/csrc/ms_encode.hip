@@ -205,11 +205,45 @@ __device__ __forceinline__ void gather_ranks(const sel_flags &sf, uint64_t nunit
     }
 }
 
+// The select's split (thread 0): the selected rows below the scalar
+// remainder's first row thr are counted from the unit offsets and thr's
+// unit's ballots.
+__device__ __forceinline__ void select_split(const sel_flags &sf, const sel_split &sp, uint64_t nunits)
+{
+    const ms_dev_meta *in = sp.in_meta;
+    const ms_geom &g = sp.in_g;
+    const ms_parts pt = ms_parts_of(g, in->count, in->size_used, in->size_compr);
+    const bool has_rest = g.kind == MS_UNCOMPR ? in->count != 0 : in->size_compr != in->size_used;
+    const uint64_t r = has_rest ? pt.n_rest % sp.lanes : 0;
+    const uint64_t thr = pt.n - r;
+    const uint64_t total = sf.uoff[nunits];
+    const uint64_t u = thr / 512, w = thr % 512;
+    uint64_t na = u < nunits ? sf.uoff[u] : total;
+    if (u < nunits && w) {
+        // row 64 gl + 8 m + j of the unit is bit 8 gl + j of ballot m
+        for (uint32_t m = 0; m < 8; m++) {
+            const uint64_t B = sf.flags[8 * u + m];
+            for (uint32_t gl = 0; gl < 8; gl++) {
+                const uint64_t row = 64 * gl + 8 * m;
+                if (row >= w) continue;
+                const uint64_t k = w - row < 8 ? w - row : 8;
+                na += __popcll((B >> (8 * gl)) & ((1ull << k) - 1));
+            }
+        }
+    }
+    *sp.d_cnt = total;
+    *sp.d_na = na;
+    *sp.d_flag = r != 0;
+}
+
 // ustart[b] for every block whose first read rank r0(b) lies in unit u's
-// ranks [uoff[u], uoff[u+1]); one thread per unit.
-__global__ __launch_bounds__(WG) void k_flag_starts(emit_layout l, flag_src fs, uint64_t cap, uint32_t *ustart)
+// ranks [uoff[u], uoff[u+1]); one thread per unit.  Thread 0 also makes
+// the select's split.
+__global__ __launch_bounds__(WG) void k_flag_starts(emit_layout l, flag_src fs, sel_split sp, uint64_t cap,
+                                                    uint32_t *ustart)
 {
     const uint64_t nunits = *fs.sf.d_units;
+    if (blockIdx.x == 0 && threadIdx.x == 0) select_split(fs.sf, sp, nunits);
     for (uint64_t u = (uint64_t)blockIdx.x * WG + threadIdx.x; u < nunits; u += (uint64_t)gridDim.x * WG) {
         const uint64_t lo = fs.sf.uoff[u], hi = fs.sf.uoff[u + 1];
         if (lo == hi) continue;
@@ -548,9 +582,12 @@ __global__ __launch_bounds__(WG) void k_pack_static(emit_layout l, const uint64_
 // delta_f tile carries (carries non-null) and the raw rest behind the gap;
 // every workgroup derives the gap's end itself.  start / bws are null for
 // static_vbp_f.
+// With d_flag (a select's split): values [*d_m, *d_total) are appended raw
+// behind the output when *d_flag (select_compr.h:209-250).
 __global__ __launch_bounds__(WG) void k_emit_tail(emit_layout l, const uint64_t *v, const uint64_t *d_m,
                                                   const uint64_t *start, const uint32_t *bws, ms_dev_meta *meta,
-                                                  uint64_t *d_append, uint64_t *carries, uint8_t *out)
+                                                  uint64_t *d_append, uint64_t *carries, uint8_t *out,
+                                                  const uint64_t *d_total, const uint64_t *d_flag)
 {
     const emit_plan pl = plan_of(l, *d_m);
     uint64_t C;
@@ -561,14 +598,23 @@ __global__ __launch_bounds__(WG) void k_emit_tail(emit_layout l, const uint64_t 
     const uint64_t rest = pl.m - pl.m_c;
     const bool fits = R + 8 * rest <= l.cap;
     const uint64_t tid = (uint64_t)blockIdx.x * WG + threadIdx.x, nth = (uint64_t)gridDim.x * WG;
+    const uint64_t app = rest ? R + 8 * rest : R;   // the write-iterator's append offset
+    const bool appending = d_flag && *d_flag;
+    const uint64_t total = appending ? *d_total : pl.m;
+    const bool afits = !appending || app + 8 * (total - pl.m) <= l.cap;
     if (tid == 0) {
-        if (!fits) emit_overflow(l);
-        meta->count = pl.m;
+        if (!fits || !afits) emit_overflow(l);
+        const bool app_done = appending && afits;
+        meta->count = app_done ? total : pl.m;
         meta->size_compr = C;
-        meta->size_used = rest ? R + 8 * rest : C;
-        meta->aux[0] = rest ? R + 8 * rest : R;
+        meta->size_used = app_done ? app + 8 * (total - pl.m) : rest ? R + 8 * rest : C;
+        meta->aux[0] = app;
         meta->aux[1] = R;
-        if (d_append) *d_append = meta->aux[0];
+        if (d_append) *d_append = app;
+    }
+    if (appending && afits) {
+        uint64_t *dst = (uint64_t *)(out + app);
+        for (uint64_t i = pl.m + tid; i < total; i += nth) dst[i - pl.m] = v[i];
     }
     if (carries) {
         const uint64_t ntiles = (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE;
@@ -628,7 +674,7 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         MS_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(l.bs + l.K + 1, WG, 64)), dim3(WG), 0, s, l, d_v, d_m,
                            (const uint64_t *)nullptr, (const uint32_t *)nullptr, out->meta, d_append,
-                           (uint64_t *)nullptr, out->data);
+                           (uint64_t *)nullptr, out->data, (const uint64_t *)nullptr, (const uint64_t *)nullptr);
     } else {
         uint64_t *sz = (uint64_t *)sc.get(nb_bound * 8 + 64);
         uint64_t *start = (uint64_t *)sc.get(nb_bound * 8 + 64);
@@ -659,7 +705,8 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
         const uint64_t tail = std::max<uint64_t>(l.bs + l.K + 1, m_bound / MS_DELTA_TILE * 8 + 8);
         hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(tail, WG, 1024)), dim3(WG), 0, s, l, d_v, d_m,
                            (const uint64_t *)start, (const uint32_t *)bws, out->meta, d_append,
-                           f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data);
+                           f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data, (const uint64_t *)nullptr,
+                           (const uint64_t *)nullptr);
     }
     MS_HIP(hipGetLastError());
     return MS_OK;
@@ -705,9 +752,10 @@ bool emit_flags_ok(const ms_format_desc &f)
     return true;
 }
 
-int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uint64_t *d_m, const uint64_t *d_total,
-                      uint64_t m_bound, uint64_t chunk, temps &sc)
+int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uint64_t *pos, uint64_t m_bound,
+                      uint64_t chunk, temps &sc)
 {
+    const uint64_t *d_m = sp.d_na, *d_total = sp.d_cnt;
     hipStream_t s = sc.s;
     const ms_format_desc &f = out->f;
     if (!emit_flags_ok(f)) return ms_fail(MS_ERR_INVALID, "emit from flags: format kind %u", f.kind);
@@ -726,7 +774,7 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uin
     if (sf.units_bound >= (1ull << 32)) return ms_fail(MS_ERR_CAPACITY, "emit from flags: %llu units",
                                                       (unsigned long long)sf.units_bound);
     fs.ustart = ustart;
-    hipLaunchKernelGGL(k_flag_starts, dim3(grid_for(sf.units_bound, WG)), dim3(WG), 0, s, l, fs, starts, ustart);
+    hipLaunchKernelGGL(k_flag_starts, dim3(grid_for(sf.units_bound, WG)), dim3(WG), 0, s, l, fs, sp, starts, ustart);
     MS_HIP(hipGetLastError());
     uint64_t *refs = nullptr;
     if (f.kind == MS_FOR) {
@@ -757,7 +805,8 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uin
     const uint64_t tail = std::max<uint64_t>(l.bs + l.K + 1, m_bound / MS_DELTA_TILE * 8 + 8);
     hipLaunchKernelGGL(k_emit_tail, dim3(grid_for(tail, WG, 1024)), dim3(WG), 0, s, l, (const uint64_t *)pos, d_m,
                        (const uint64_t *)start, (const uint32_t *)bws, out->meta, (uint64_t *)nullptr,
-                       f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data);
+                       f.kind == MS_DELTA ? out->refs : (uint64_t *)nullptr, out->data, d_total,
+                       (const uint64_t *)sp.d_flag);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }

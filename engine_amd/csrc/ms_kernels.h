@@ -89,14 +89,24 @@ int launch_exclusive_scan_n(const uint64_t *d_in, uint64_t *d_out, const uint64_
 // append offset (done()'s second element).
 int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, uint64_t chunk,
                 uint64_t *d_append, temps &sc);
-// launch_emit of the ranks [0, *d_m) of a select's flags (the positions
-// of its selected rows), for the formats emit_flags_ok takes (dynamic_vbp_f
-// and delta_f / for_f over it).  pos (capacity >= m_bound) receives only
-// the positions other kernels read by rank: the delta tile predecessors
-// and ranks [values in the compressed part, *d_total).
+// The select's split of its input (select_compr.h:209-250): rows below
+// the scalar remainder's first row go through the write-iterator, the
+// positions of the remainder's selected rows are appended raw behind it.
+struct sel_split {
+    const ms_dev_meta *in_meta;
+    ms_geom in_g;
+    uint32_t lanes;
+    uint64_t *d_cnt, *d_na, *d_flag;   // out: selected rows, those written by the iterator, remainder present
+};
+// The select's output from its flags (the positions of its selected rows),
+// for the formats emit_flags_ok takes (dynamic_vbp_f and delta_f / for_f
+// over it): the split, the write-iterator's emission of ranks [0, *d_na)
+// and the raw append of ranks [*d_na, *d_cnt).  pos (capacity >= m_bound)
+// receives only the positions other kernels read by rank: the delta tile
+// predecessors and the ranks behind the compressed part.
 bool emit_flags_ok(const ms_format_desc &f);
-int launch_emit_flags(ms_col *out, const sel_flags &sf, uint64_t *pos, const uint64_t *d_m, const uint64_t *d_total,
-                      uint64_t m_bound, uint64_t chunk, temps &sc);
+int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uint64_t *pos, uint64_t m_bound,
+                      uint64_t chunk, temps &sc);
 // rle_f compression of d_v[0 .. *d_m)
 int launch_rle_compress(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t m_bound, temps &sc);
 
