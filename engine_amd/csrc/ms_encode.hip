@@ -56,19 +56,30 @@ struct emit_plan {
     uint64_t nfb;        // outer blocks (for_f)
 };
 
+// a / d and a % d (d > 0) for the layout's block and page counts, which are
+// mostly powers of two: a shift and a mask then, the division otherwise.
+__device__ __forceinline__ uint64_t ldiv(uint64_t a, uint64_t d)
+{
+    return (d & (d - 1)) == 0 ? a >> __builtin_ctzll(d) : a / d;
+}
+__device__ __forceinline__ uint64_t lmod(uint64_t a, uint64_t d)
+{
+    return (d & (d - 1)) == 0 ? a & (d - 1) : a % d;
+}
+
 __device__ __forceinline__ emit_plan plan_of(const emit_layout &l, uint64_t m)
 {
     emit_plan p;
     p.m = m;
     uint64_t mc;
-    if (l.K == 0) mc = m / l.bs * l.bs;
+    if (l.K == 0) mc = ldiv(m, l.bs) * l.bs;
     else {
-        const uint64_t full = m / l.K, rem = m - full * l.K;
-        mc = full * l.K + rem / l.bs * l.bs;
+        const uint64_t full = ldiv(m, l.K), rem = m - full * l.K;
+        mc = full * l.K + ldiv(rem, l.bs) * l.bs;
     }
     p.m_c = mc;
-    p.nb = mc / l.B;
-    p.nfb = mc / l.L;
+    p.nb = ldiv(mc, l.B);
+    p.nfb = ldiv(mc, l.L);
     return p;
 }
 
@@ -85,14 +96,14 @@ __device__ __forceinline__ uint64_t hdr_of(const emit_layout &l, uint64_t b)
     uint64_t h = 0;
     if (l.leaf == MS_DYNAMIC_VBP) {
         uint64_t q;
-        if (l.kind == MS_DYNAMIC_VBP) q = l.K ? b % (l.K / l.B) : b;
-        else q = b % l.bpo;                       // each outer block is one inner call
-        if (q % l.P == 0) h += l.P;
+        if (l.kind == MS_DYNAMIC_VBP) q = l.K ? lmod(b, ldiv(l.K, l.B)) : b;
+        else q = lmod(b, l.bpo);                  // each outer block is one inner call
+        if (lmod(q, l.P) == 0) h += l.P;
     }
-    if (l.kind == MS_FOR && b % l.bpo == 0) {
-        const uint64_t fb = b / l.bpo;
-        const uint64_t fq = l.K ? fb % (l.K / l.L) : fb;
-        if (fq % l.FP == 0) h += 8 * l.FP;
+    if (l.kind == MS_FOR && lmod(b, l.bpo) == 0) {
+        const uint64_t fb = ldiv(b, l.bpo);
+        const uint64_t fq = l.K ? lmod(fb, ldiv(l.K, l.L)) : fb;
+        if (lmod(fq, l.FP) == 0) h += 8 * l.FP;
     }
     return h;
 }
@@ -113,7 +124,7 @@ __device__ __forceinline__ const uint64_t *chunk_src(const emit_layout &l, const
 __device__ __forceinline__ uint64_t out_pos(const emit_layout &l, const uint64_t *start, uint64_t b)
 {
     if (!l.cout) return start[b];
-    const uint64_t bpc = l.K / l.B, c = b / bpc;
+    const uint64_t bpc = ldiv(l.K, l.B), c = ldiv(b, bpc);
     return l.cout[c] + start[b] - start[c * bpc];
 }
 
@@ -316,23 +327,19 @@ __device__ __forceinline__ uint32_t block_bw(const emit_layout &l, const G &get,
     return 64 - __clzll(o);
 }
 
-// Packs innermost block b (one wave, lane t) at image offset off: its
-// interleaved words are assembled in lds (B words) and written out, then
-// the block index entry, dvbp page meta and for_f reference page.  sync()
-// orders the wave's LDS accesses (the workgroup barrier when the
-// workgroup is this wave).
+// Assembles the interleaved words of innermost block b (one wave, lane t)
+// in lds (B words; the first B bw / 64 hold the block).  sync() orders the
+// wave's LDS accesses (the workgroup barrier when the workgroup is this
+// wave); the words are complete when it returns.
 template <class G, class Sync>
-__device__ __forceinline__ void pack_block(const emit_layout &l, const emit_plan &pl, uint64_t b, uint32_t bw,
-                                           uint64_t off, const G &get, uint64_t lo, uint64_t ref, const uint64_t *refs,
-                                           const uint64_t *start, unsigned long long *lds, uint8_t *out,
-                                           uint64_t *index, uint64_t *refs_out, uint32_t t, const Sync &sync)
+__device__ __forceinline__ void assemble_block(const emit_layout &l, uint32_t bw, const G &get, uint64_t lo,
+                                               uint64_t ref, unsigned long long *lds, uint32_t t, const Sync &sync)
 {
     // Lane counts that divide the wave and blocks of whole waves of codes
     // let each thread own a run of consecutive codes of one lane.
     const uint32_t S = (uint32_t)l.S;
     const bool runs = S != 0 && S <= 64 && 64 % S == 0 && l.B % 64 == 0;
     const uint32_t j = runs ? t % S : 0, g = runs ? t / S : 0, C = (uint32_t)(l.B >> 6);
-    const uint64_t fb = b / l.bpo;
     const uint64_t words = l.B * bw / 64;
     for (uint64_t w = t; w < words; w += 64) lds[w] = 0;
     sync();
@@ -399,48 +406,71 @@ __device__ __forceinline__ void pack_block(const emit_layout &l, const emit_plan
         }
     }
     sync();
-    uint64_t *dst = (uint64_t *)(out + off);
-    for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
-    sync();
+}
+
+// What follows block b's words at image offset off: the block index entry,
+// the dvbp page meta and the for_f reference page.
+__device__ __forceinline__ void finish_block(const emit_layout &l, const emit_plan &pl, uint64_t b, uint32_t bw,
+                                             uint64_t off, uint64_t ref, const uint64_t *refs, const uint64_t *start,
+                                             uint8_t *out, uint64_t *index, uint64_t *refs_out, uint32_t t)
+{
+    const uint64_t fb = ldiv(b, l.bpo);
     if (t == 0) index[b] = (off << 8) | bw;
     // dynamic_vbp page meta (dynamic_vbp.h:216-255)
     if (l.leaf == MS_DYNAMIC_VBP) {
         uint64_t q, call_end;
         if (l.kind == MS_DYNAMIC_VBP) {
             if (l.K) {
-                const uint64_t bpc = l.K / l.B;
-                q = b % bpc;
-                call_end = (b / bpc + 1) * bpc;
+                const uint64_t bpc = ldiv(l.K, l.B);
+                q = lmod(b, bpc);
+                call_end = b - q + bpc;
                 if (call_end > pl.nb) call_end = pl.nb;
             } else {
                 q = b;
                 call_end = pl.nb;
             }
         } else {
-            q = b % l.bpo;
-            call_end = (b / l.bpo + 1) * l.bpo;
+            q = lmod(b, l.bpo);
+            call_end = b - q + l.bpo;
         }
-        const uint64_t bpf = b - q % l.P;                                 // first block of the page
+        const uint64_t qp = lmod(q, l.P);
+        const uint64_t bpf = b - qp;                                      // first block of the page
         const uint64_t meta = out_pos(l, start, bpf) + hdr_of(l, bpf) - l.P; // meta bytes end the header
-        if (t == 0) out[meta + q % l.P] = (uint8_t)bw;
+        if (t == 0) out[meta + qp] = (uint8_t)bw;
         if (b == bpf) {
             const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
             for (uint64_t x = npage + t; x < l.P; x += 64) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
         }
     }
     // for_f reference page (for.h:207-233)
-    if (l.kind == MS_FOR && b % l.bpo == 0) {
+    if (l.kind == MS_FOR && lmod(b, l.bpo) == 0) {
         if (t == 0) refs_out[fb] = ref;
-        const uint64_t fpc = l.K ? l.K / l.L : 0;
-        const uint64_t fq = l.K ? fb % fpc : fb;
-        if (fq % l.FP == 0) {
-            uint64_t fcall_end = l.K ? (fb / fpc + 1) * fpc : pl.nfb;
+        const uint64_t fpc = l.K ? ldiv(l.K, l.L) : 0;
+        const uint64_t fq = l.K ? lmod(fb, fpc) : fb;
+        if (lmod(fq, l.FP) == 0) {
+            uint64_t fcall_end = l.K ? fb - lmod(fb, fpc) + fpc : pl.nfb;
             if (fcall_end > pl.nfb) fcall_end = pl.nfb;
             const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
             uint64_t *rp = (uint64_t *)(out + out_pos(l, start, b));
             for (uint64_t x = t; x < l.FP; x += 64) rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
         }
     }
+}
+
+// Packs innermost block b (one wave, lane t) at image offset off: its words
+// are assembled in lds (B words) and written out, then finish_block.
+template <class G, class Sync>
+__device__ __forceinline__ void pack_block(const emit_layout &l, const emit_plan &pl, uint64_t b, uint32_t bw,
+                                           uint64_t off, const G &get, uint64_t lo, uint64_t ref, const uint64_t *refs,
+                                           const uint64_t *start, unsigned long long *lds, uint8_t *out,
+                                           uint64_t *index, uint64_t *refs_out, uint32_t t, const Sync &sync)
+{
+    assemble_block(l, bw, get, lo, ref, lds, t, sync);
+    const uint64_t words = l.B * bw / 64;
+    uint64_t *dst = (uint64_t *)(out + off);
+    for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
+    sync();
+    finish_block(l, pl, b, bw, off, ref, refs, start, out, index, refs_out, t);
 }
 
 // Bit width and header+payload bytes of every innermost block.
@@ -498,14 +528,16 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
     }
 }
 
-// Sizes from a select's flags, one innermost block per wave (dynamic LDS:
-// B + pred gathered values).
-// *d_nb receives the number of blocks (the scan's length).
-__global__ __launch_bounds__(64) void k_flag_sizes(emit_layout l, flag_src fs, const uint64_t *d_m,
+// Sizes and packed words from a select's flags, one innermost block per
+// wave (dynamic LDS: B packed words, then the B + pred gathered values).
+// Block b's words go to stage + b B, its size to sz[b], its width to
+// bws[b]; *d_nb receives the number of blocks (the scan's length).
+__global__ __launch_bounds__(64) void k_flag_stage(emit_layout l, flag_src fs, const uint64_t *d_m,
                                                    const uint64_t *refs, uint64_t *d_nb, uint64_t *sz,
-                                                   uint32_t *bws)
+                                                   uint32_t *bws, uint64_t *stage)
 {
-    extern __shared__ unsigned long long fvals[];
+    extern __shared__ unsigned long long lds[];
+    unsigned long long *fvals = lds + l.B;
     const emit_plan pl = plan_of(l, *d_m);
     const uint32_t t = threadIdx.x;
     const uint64_t nunits = *fs.sf.d_units;
@@ -516,39 +548,48 @@ __global__ __launch_bounds__(64) void k_flag_sizes(emit_layout l, flag_src fs, c
                      [&](uint64_t i, uint64_t row) { fvals[i] = row; });
         __syncthreads();
         const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
-        const uint32_t bw = block_bw(l, [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; }, lo, ref, t);
+        const auto get = [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; };
+        const uint32_t bw = block_bw(l, get, lo, ref, t);
         if (t == 0) {
             sz[b] = hdr_of(l, b) + l.B * bw / 8;
             bws[b] = bw;
         }
+        assemble_block(l, bw, get, lo, ref, lds, t, [] { __syncthreads(); });
+        const uint64_t words = l.B * bw / 64;
+        uint64_t *dst = stage + b * l.B;
+        for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
         __syncthreads();
     }
 }
 
-// Pack from a select's flags, one innermost block per wave (dynamic LDS:
-// B packed words, then the B + pred gathered values).
-__global__ __launch_bounds__(64) void k_flag_pack(emit_layout l, flag_src fs, const uint64_t *d_m,
-                                                  const uint64_t *refs, const uint64_t *start, const uint32_t *bws,
-                                                  uint8_t *out, uint64_t *index, uint64_t *refs_out)
+// Moves each staged block to its offset in the image and writes what
+// follows it (index entry, page meta, reference page); one block per wave,
+// four waves per workgroup.  A block's first 64 staged words are loaded
+// before its width and offset arrive.
+__global__ __launch_bounds__(256) void k_flag_place(emit_layout l, const uint64_t *d_m, const uint64_t *refs,
+                                                    const uint64_t *start, const uint32_t *bws,
+                                                    const uint64_t *stage, uint8_t *out, uint64_t *index,
+                                                    uint64_t *refs_out)
 {
-    extern __shared__ unsigned long long lds[];
-    unsigned long long *fvals = lds + l.B;
     const emit_plan pl = plan_of(l, *d_m);
-    const uint32_t t = threadIdx.x;
-    const uint64_t nunits = *fs.sf.d_units;
-    for (uint64_t b = blockIdx.x; b < pl.nb; b += gridDim.x) {
+    const uint32_t t = threadIdx.x & 63;
+    const uint64_t step = (uint64_t)gridDim.x * 4;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (uint64_t b = (uint64_t)blockIdx.x * 4 + wv; b < pl.nb; b += step) {
+        const uint64_t *src = stage + b * l.B;
+        const uint64_t w0 = t < l.B ? src[t] : 0;
         const uint32_t bw = bws[b];
         const uint64_t off = out_pos(l, start, b) + hdr_of(l, b);
-        if (off + 8 * (l.B * bw / 64) > l.cap) {   // uniform per wave
+        const uint64_t words = l.B * bw / 64;
+        if (off + 8 * words > l.cap) {   // uniform per wave
             if (t == 0) emit_overflow(l);
             continue;
         }
-        const uint64_t r0 = flag_r0(l, fs, b), lo = b * l.B;
-        gather_ranks(fs.sf, nunits, fs.ustart[b], r0, lo + l.B - r0,
-                     [&](uint64_t i, uint64_t row) { fvals[i] = row; });
+        uint64_t *dst = (uint64_t *)(out + off);
+        if (t < words) dst[t] = w0;
+        for (uint64_t w = t + 64; w < words; w += 64) dst[w] = src[w];
         const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
-        pack_block(l, pl, b, bw, off, [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; }, lo, ref, refs, start,
-                   lds, out, index, refs_out, t, [] { __syncthreads(); });
+        finish_block(l, pl, b, bw, off, ref, refs, start, out, index, refs_out, t);
     }
 }
 
@@ -788,14 +829,19 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uin
     const uint32_t fl_grid = grid_for(nb_bound, 1, 65536);
     uint64_t *d_nb = (uint64_t *)sc.get(64);
     if (!d_nb) return ms_fail(MS_ERR_NOMEM, "emit scratch");
-    hipLaunchKernelGGL(k_flag_sizes, dim3(fl_grid), dim3(64), vals_lds, s, l, fs, d_m, (const uint64_t *)refs, d_nb,
-                       sz, bws);
+    // every block's words staged at a fixed slot of B words, so the packing
+    // happens in the same pass as the sizes and placing them is a copy
+    uint64_t *stage = (uint64_t *)sc.get(nb_bound * l.B * 8 + 64);
+    if (!stage) return ms_fail(MS_ERR_NOMEM, "emit scratch");
+    hipLaunchKernelGGL(k_flag_stage, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m,
+                       (const uint64_t *)refs, d_nb, sz, bws, stage);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(sz, start, d_nb, 0, nb_bound, sc));
     if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
         return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
-    hipLaunchKernelGGL(k_flag_pack, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m, (const uint64_t *)refs,
-                       (const uint64_t *)start, (const uint32_t *)bws, out->data, out->index, out->refs);
+    hipLaunchKernelGGL(k_flag_place, dim3(grid_for(nb_bound, 4, 16384)), dim3(256), 0, s, l, d_m,
+                       (const uint64_t *)refs, (const uint64_t *)start, (const uint32_t *)bws,
+                       (const uint64_t *)stage, out->data, out->index, out->refs);
     MS_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_flag_ranks, dim3(grid_for(m_bound / MS_DELTA_TILE + 1, 1, 65536)), dim3(64), 0, s, l, fs,
                        d_m, d_total, pos);
