@@ -409,7 +409,9 @@ __device__ __forceinline__ void assemble_block(const emit_layout &l, uint32_t bw
 }
 
 // What follows block b's words at image offset off: the block index entry,
-// the dvbp page meta and the for_f reference page.
+// the dvbp page meta and the for_f reference page.  Lanes t < NL of a group
+// of NL lanes write them.
+template <uint32_t NL = 64>
 __device__ __forceinline__ void finish_block(const emit_layout &l, const emit_plan &pl, uint64_t b, uint32_t bw,
                                              uint64_t off, uint64_t ref, const uint64_t *refs, const uint64_t *start,
                                              uint8_t *out, uint64_t *index, uint64_t *refs_out, uint32_t t)
@@ -439,7 +441,7 @@ __device__ __forceinline__ void finish_block(const emit_layout &l, const emit_pl
         if (t == 0) out[meta + qp] = (uint8_t)bw;
         if (b == bpf) {
             const uint64_t npage = call_end - bpf < l.P ? call_end - bpf : l.P;
-            for (uint64_t x = npage + t; x < l.P; x += 64) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
+            for (uint64_t x = npage + t; x < l.P; x += NL) out[meta + x] = (uint8_t)MS_VBP_BW_NOBLOCK;
         }
     }
     // for_f reference page (for.h:207-233)
@@ -452,7 +454,7 @@ __device__ __forceinline__ void finish_block(const emit_layout &l, const emit_pl
             if (fcall_end > pl.nfb) fcall_end = pl.nfb;
             const uint64_t nfp = fcall_end - fb < l.FP ? fcall_end - fb : l.FP;
             uint64_t *rp = (uint64_t *)(out + out_pos(l, start, b));
-            for (uint64_t x = t; x < l.FP; x += 64) rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
+            for (uint64_t x = t; x < l.FP; x += NL) rp[x] = x < nfp ? refs[fb + x] : (x == nfp ? ~0ull : 0ull);
         }
     }
 }
@@ -563,33 +565,31 @@ __global__ __launch_bounds__(64) void k_flag_stage(emit_layout l, flag_src fs, c
 }
 
 // Moves each staged block to its offset in the image and writes what
-// follows it (index entry, page meta, reference page); one block per wave,
-// four waves per workgroup.  A block's first 64 staged words are loaded
-// before its width and offset arrive.
+// follows it (index entry, page meta, reference page).  A group of 16
+// lanes takes a block, so a wave has four blocks' loads in flight; four
+// waves per workgroup.
 __global__ __launch_bounds__(256) void k_flag_place(emit_layout l, const uint64_t *d_m, const uint64_t *refs,
                                                     const uint64_t *start, const uint32_t *bws,
                                                     const uint64_t *stage, uint8_t *out, uint64_t *index,
                                                     uint64_t *refs_out)
 {
+    constexpr uint32_t NL = 16;
     const emit_plan pl = plan_of(l, *d_m);
-    const uint32_t t = threadIdx.x & 63;
-    const uint64_t step = (uint64_t)gridDim.x * 4;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (uint64_t b = (uint64_t)blockIdx.x * 4 + wv; b < pl.nb; b += step) {
+    const uint32_t t = threadIdx.x % NL;
+    const uint64_t step = (uint64_t)gridDim.x * (256 / NL);
+    for (uint64_t b = (uint64_t)blockIdx.x * (256 / NL) + threadIdx.x / NL; b < pl.nb; b += step) {
         const uint64_t *src = stage + b * l.B;
-        const uint64_t w0 = t < l.B ? src[t] : 0;
         const uint32_t bw = bws[b];
         const uint64_t off = out_pos(l, start, b) + hdr_of(l, b);
         const uint64_t words = l.B * bw / 64;
-        if (off + 8 * words > l.cap) {   // uniform per wave
+        if (off + 8 * words > l.cap) {
             if (t == 0) emit_overflow(l);
             continue;
         }
         uint64_t *dst = (uint64_t *)(out + off);
-        if (t < words) dst[t] = w0;
-        for (uint64_t w = t + 64; w < words; w += 64) dst[w] = src[w];
-        const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
-        finish_block(l, pl, b, bw, off, ref, refs, start, out, index, refs_out, t);
+        for (uint64_t w = t; w < words; w += NL) dst[w] = src[w];
+        const uint64_t ref = l.kind == MS_FOR ? refs[ldiv(b, l.bpo)] : 0;
+        finish_block<NL>(l, pl, b, bw, off, ref, refs, start, out, index, refs_out, t);
     }
 }
 
@@ -839,7 +839,7 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uin
     MS_TRY(launch_exclusive_scan_n(sz, start, d_nb, 0, nb_bound, sc));
     if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
         return ms_fail(MS_ERR_CAPACITY, "emit: index capacity");
-    hipLaunchKernelGGL(k_flag_place, dim3(grid_for(nb_bound, 4, 16384)), dim3(256), 0, s, l, d_m,
+    hipLaunchKernelGGL(k_flag_place, dim3(grid_for(nb_bound, 16, 16384)), dim3(256), 0, s, l, d_m,
                        (const uint64_t *)refs, (const uint64_t *)start, (const uint32_t *)bws,
                        (const uint64_t *)stage, out->data, out->index, out->refs);
     MS_HIP(hipGetLastError());
