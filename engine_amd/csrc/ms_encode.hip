@@ -533,10 +533,13 @@ __global__ __launch_bounds__(64) void k_pack_blocks(emit_layout l, const uint64_
 // Sizes and packed words from a select's flags, one innermost block per
 // wave (dynamic LDS: B packed words, then the B + pred gathered values).
 // Block b's words go to stage + b B, its size to sz[b], its width to
-// bws[b]; *d_nb receives the number of blocks (the scan's length).
+// bws[b]; *d_nb receives the number of blocks (the scan's length).  The
+// positions read by rank later go to pos: a delta tile's predecessors
+// (gathered with the block the tile starts at) and, by workgroup 0 after
+// its blocks, the tail behind the compressed part.
 __global__ __launch_bounds__(64) void k_flag_stage(emit_layout l, flag_src fs, const uint64_t *d_m,
-                                                   const uint64_t *refs, uint64_t *d_nb, uint64_t *sz,
-                                                   uint32_t *bws, uint64_t *stage)
+                                                   const uint64_t *refs, const uint64_t *d_total, uint64_t *d_nb,
+                                                   uint64_t *sz, uint32_t *bws, uint64_t *stage, uint64_t *pos)
 {
     extern __shared__ unsigned long long lds[];
     unsigned long long *fvals = lds + l.B;
@@ -549,6 +552,8 @@ __global__ __launch_bounds__(64) void k_flag_stage(emit_layout l, flag_src fs, c
         gather_ranks(fs.sf, nunits, fs.ustart[b], r0, lo + l.B - r0,
                      [&](uint64_t i, uint64_t row) { fvals[i] = row; });
         __syncthreads();
+        if (fs.pred && lo > 0 && lo % MS_DELTA_TILE == 0)
+            for (uint32_t i = t; i < fs.pred; i += 64) pos[r0 + i] = fvals[i];
         const uint64_t ref = l.kind == MS_FOR ? refs[b / l.bpo] : 0;
         const auto get = [&](uint64_t gi) -> uint64_t { return fvals[gi - r0]; };
         const uint32_t bw = block_bw(l, get, lo, ref, t);
@@ -561,6 +566,12 @@ __global__ __launch_bounds__(64) void k_flag_stage(emit_layout l, flag_src fs, c
         uint64_t *dst = stage + b * l.B;
         for (uint64_t w = t; w < words; w += 64) dst[w] = lds[w];
         __syncthreads();
+    }
+    if (blockIdx.x == 0) {
+        const uint64_t total = *d_total, r0 = pl.m_c;
+        const uint64_t cnt = total > r0 ? total - r0 : 0;
+        if (cnt) gather_ranks(fs.sf, nunits, fs.ustart[pl.nb], r0, cnt,
+                              [&](uint64_t i, uint64_t row) { pos[r0 + i] = row; });
     }
 }
 
@@ -756,33 +767,6 @@ int launch_emit(ms_col *out, const uint64_t *d_v, const uint64_t *d_m, uint64_t 
 // ------------------------------------------------------------------
 // Emission straight from a select's flags.
 // ------------------------------------------------------------------
-// Writes into pos (indexed by rank) the positions other kernels read by
-// rank: the delta_f tile predecessors [2048 t - pred, 2048 t) of every tile
-// t >= 1 of the compressed part, and the tail [values in the compressed
-// part, *d_total).  One wave per job.
-__global__ __launch_bounds__(64) void k_flag_ranks(emit_layout l, flag_src fs, const uint64_t *d_m,
-                                                   const uint64_t *d_total, uint64_t *pos)
-{
-    const emit_plan pl = plan_of(l, *d_m);
-    const uint64_t total = *d_total, nunits = *fs.sf.d_units;
-    const uint64_t ntiles = fs.pred ? (pl.m_c + MS_DELTA_TILE - 1) / MS_DELTA_TILE : 0;
-    const uint64_t jobs = ntiles;   // tiles 1 .. ntiles-1, then the tail as job 0
-    for (uint64_t jb = blockIdx.x; jb < (jobs ? jobs : 1); jb += gridDim.x) {
-        uint64_t r0, cnt, u;
-        if (jb == 0) {
-            r0 = pl.m_c;
-            cnt = total > r0 ? total - r0 : 0;
-            u = cnt ? fs.ustart[pl.nb] : 0;
-        } else {
-            r0 = jb * MS_DELTA_TILE - fs.pred;
-            cnt = fs.pred;
-            u = fs.ustart[jb * MS_DELTA_TILE / l.B];
-        }
-        uint64_t *dst = pos + r0;
-        gather_ranks(fs.sf, nunits, u, r0, cnt, [&](uint64_t i, uint64_t row) { dst[i] = row; });
-    }
-}
-
 bool emit_flags_ok(const ms_format_desc &f)
 {
     if (f.kind != MS_DYNAMIC_VBP && f.kind != MS_DELTA && f.kind != MS_FOR) return false;
@@ -834,7 +818,7 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uin
     uint64_t *stage = (uint64_t *)sc.get(nb_bound * l.B * 8 + 64);
     if (!stage) return ms_fail(MS_ERR_NOMEM, "emit scratch");
     hipLaunchKernelGGL(k_flag_stage, dim3(fl_grid), dim3(64), l.B * 8 + vals_lds, s, l, fs, d_m,
-                       (const uint64_t *)refs, d_nb, sz, bws, stage);
+                       (const uint64_t *)refs, d_total, d_nb, sz, bws, stage, pos);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(sz, start, d_nb, 0, nb_bound, sc));
     if (out->index_cap < nb_bound - 1 || (f.kind == MS_FOR && out->refs_cap < m_bound / l.L))
@@ -842,9 +826,6 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uin
     hipLaunchKernelGGL(k_flag_place, dim3(grid_for(nb_bound, 16, 16384)), dim3(256), 0, s, l, d_m,
                        (const uint64_t *)refs, (const uint64_t *)start, (const uint32_t *)bws,
                        (const uint64_t *)stage, out->data, out->index, out->refs);
-    MS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_flag_ranks, dim3(grid_for(m_bound / MS_DELTA_TILE + 1, 1, 65536)), dim3(64), 0, s, l, fs,
-                       d_m, d_total, pos);
     MS_HIP(hipGetLastError());
     if (f.kind == MS_DELTA && out->refs_cap < (m_bound / MS_DELTA_TILE + 1) * 8)
         return ms_fail(MS_ERR_CAPACITY, "emit: delta carries capacity");
