@@ -103,8 +103,8 @@ int launch_gather(const ms_col *data, const uint64_t *d_pos, const uint64_t *d_n
 // The same flags, a workgroup per chunk of ISECT_CHUNK values of a: the
 // part of b that the chunk's value range can meet is staged in LDS when
 // it fits, so the per-value searches in b run in LDS.
-constexpr uint32_t ISECT_CHUNK = 1024;
-constexpr uint32_t ISECT_LDS = 4096;   // b values staged per workgroup (32 KiB: 5 workgroups per CU)
+constexpr uint32_t ISECT_CHUNK = 512;
+constexpr uint32_t ISECT_LDS = 2048;   // b values staged per workgroup (16 KiB: 9 workgroups per CU)
 
 template <class Arr>
 __device__ __forceinline__ uint64_t ub_in(const Arr &a, uint64_t n, uint64_t v)
@@ -236,16 +236,19 @@ __global__ __launch_bounds__(WG) void k_isect_mark(const uint64_t *a, const uint
     }
 }
 
-// Copies chunk ck's kept values to off[ck] on; *d_cnt = all kept values.
+// Copies chunk ck's kept values to off[ck] on, a wave per chunk;
+// *d_cnt = all kept values.
 __global__ __launch_bounds__(WG) void k_isect_emit(const uint64_t *d_na, const uint64_t *kept, const uint64_t *cnt,
                                                    const uint64_t *off, uint64_t *out, uint64_t *d_cnt)
 {
     const uint64_t na = *d_na;
     const uint64_t nchunks = (na + ISECT_CHUNK - 1) / ISECT_CHUNK;
     if (blockIdx.x == 0 && threadIdx.x == 0) *d_cnt = nchunks ? off[nchunks - 1] + cnt[nchunks - 1] : 0;
-    for (uint64_t ck = blockIdx.x; ck < nchunks; ck += gridDim.x) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t step = (uint64_t)gridDim.x * (WG / 64);
+    for (uint64_t ck = (uint64_t)blockIdx.x * (WG / 64) + (threadIdx.x >> 6); ck < nchunks; ck += step) {
         const uint64_t base = off[ck], c = cnt[ck];
-        for (uint64_t k = threadIdx.x; k < c; k += WG) out[base + k] = kept[ck * ISECT_CHUNK + k];
+        for (uint64_t k = lane; k < c; k += 64) out[base + k] = kept[ck * ISECT_CHUNK + k];
     }
 }
 
@@ -266,8 +269,8 @@ int launch_intersect(const uint64_t *a, const uint64_t *d_na, uint64_t na_bound,
                        (const uint64_t *)rng, kept, cnt);
     MS_HIP(hipGetLastError());
     MS_TRY(launch_exclusive_scan_n(cnt, off, d_nch, 0, chunks, sc));
-    hipLaunchKernelGGL(k_isect_emit, dim3(grid_for(chunks, 1, 4096)), dim3(WG), 0, sc.s, d_na, (const uint64_t *)kept,
-                       (const uint64_t *)cnt, (const uint64_t *)off, d_out, d_cnt);
+    hipLaunchKernelGGL(k_isect_emit, dim3(grid_for(chunks, WG / 64, 8192)), dim3(WG), 0, sc.s, d_na,
+                       (const uint64_t *)kept, (const uint64_t *)cnt, (const uint64_t *)off, d_out, d_cnt);
     MS_HIP(hipGetLastError());
     return MS_OK;
 }
