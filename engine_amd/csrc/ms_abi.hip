@@ -1195,7 +1195,8 @@ static int select_pred(ms_col **out, const ms_pred &p, const ms_format_desc *out
         const sel_split sp = {in->meta, in->g, lanes, d_cnt, d_na, d_flag};
         const uint64_t chunk = wit_chunk(*out_f);
         ms_col *c;
-        MS_TRY(new_col(*out_f, nb + 64, out_capacity(*out_f, nb + 64, chunk), S(stream), &c));
+        // the emission's first kernel clears the meta
+        MS_TRY(new_col(*out_f, nb + 64, out_capacity(*out_f, nb + 64, chunk), S(stream), &c, nullptr, false));
         const int rc = launch_emit_flags(c, sf, sp, pos, nb, chunk, sc);   // split, emission and raw append
         if (rc) { ms_col_free(c); return rc; }
         *out = c;

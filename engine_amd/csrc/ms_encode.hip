@@ -251,10 +251,12 @@ __device__ __forceinline__ void select_split(const sel_flags &sf, const sel_spli
 // ranks [uoff[u], uoff[u+1]); one thread per unit.  Thread 0 also makes
 // the select's split.
 __global__ __launch_bounds__(WG) void k_flag_starts(emit_layout l, flag_src fs, sel_split sp, uint64_t cap,
-                                                    uint32_t *ustart)
+                                                    uint32_t *ustart, ms_dev_meta *om)
 {
     const uint64_t nunits = *fs.sf.d_units;
     if (blockIdx.x == 0 && threadIdx.x == 0) select_split(fs.sf, sp, nunits);
+    // the output's meta starts cleared: the first kernel of the emission
+    if (blockIdx.x == 0 && threadIdx.x < sizeof(ms_dev_meta) / 8) ((uint64_t *)om)[threadIdx.x] = 0;
     for (uint64_t u = (uint64_t)blockIdx.x * WG + threadIdx.x; u < nunits; u += (uint64_t)gridDim.x * WG) {
         const uint64_t lo = fs.sf.uoff[u], hi = fs.sf.uoff[u + 1];
         if (lo == hi) continue;
@@ -799,7 +801,8 @@ int launch_emit_flags(ms_col *out, const sel_flags &sf, const sel_split &sp, uin
     if (sf.units_bound >= (1ull << 32)) return ms_fail(MS_ERR_CAPACITY, "emit from flags: %llu units",
                                                       (unsigned long long)sf.units_bound);
     fs.ustart = ustart;
-    hipLaunchKernelGGL(k_flag_starts, dim3(grid_for(sf.units_bound, WG)), dim3(WG), 0, s, l, fs, sp, starts, ustart);
+    hipLaunchKernelGGL(k_flag_starts, dim3(grid_for(sf.units_bound, WG)), dim3(WG), 0, s, l, fs, sp, starts, ustart,
+                       out->meta);
     MS_HIP(hipGetLastError());
     uint64_t *refs = nullptr;
     if (f.kind == MS_FOR) {
